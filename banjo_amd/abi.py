@@ -1,0 +1,154 @@
+"""ctypes mirror of include/banjo_lattice.h (the C ABI of the lattice step).
+
+Field order and types follow the header exactly; ctypes applies the same
+natural-alignment rules as the C compiler, so the layouts agree. Each struct
+cites the reference type it mirrors (paths in the reference tree).
+"""
+import ctypes as C
+
+BLX_OK = 0
+BLX_ERR_RUNTIME = -1
+BLX_ERR_INVALID = -2
+BLX_ERR_OVERFLOW = -3
+BLX_ERR_LOGIC = -4
+
+PRECISION_FLOAT = 0
+PRECISION_DOUBLE = 1
+MAX_SUPPORT_PLANES = 2
+MAX_FOOTPRINTS = 4
+MAX_CANDIDATES_PER_BLOCK = 1024
+NO_BOND = 0xFFFFFFFF
+PHASE_COUNT = 16
+NO_STEP = 0xFFFFFFFFFFFFFFFF
+
+u8, u32, u64, f32, f64 = C.c_uint8, C.c_uint32, C.c_uint64, C.c_float, C.c_double
+P = C.POINTER
+
+
+class Footprint(C.Structure):
+    """Footprint<double>, src/fastlattice/LatticePhysics.hpp:114-120."""
+    _fields_ = [("center_t", f64), ("center_b", f64), ("half_t", f64), ("half_b", f64)]
+
+
+class SupportPlane(C.Structure):
+    """SupportPlane<double>, src/fastlattice/LatticePhysics.hpp:122-136."""
+    _fields_ = [("point", f64 * 3), ("normal", f64 * 3), ("tangent", f64 * 3), ("bitangent", f64 * 3),
+                ("restitution", f64), ("static_friction", f64), ("dynamic_friction", f64),
+                ("node_radius", f64), ("reach_capped", u8), ("footprint_count", u32),
+                ("footprints", Footprint * MAX_FOOTPRINTS)]
+
+
+class ContactSettings(C.Structure):
+    """ContactSettings<double>, src/fastlattice/LatticePhysics.hpp:150-158."""
+    _fields_ = [("static_friction", f64), ("dynamic_friction", f64), ("restitution", f64),
+                ("restitution_speed_threshold", f64), ("node_contact_radius", f64),
+                ("contact_margin", f64), ("prefilter_slack", f64)]
+
+
+class NodeContactSettings(C.Structure):
+    """NodeContactSettings<double>, src/fastlattice/LatticePhysics.hpp:188-202."""
+    _fields_ = [("mode", u8), ("radius", f64), ("skin", f64), ("margin", f64), ("restitution", f64),
+                ("restitution_speed_threshold", f64), ("static_friction", f64),
+                ("dynamic_friction", f64), ("bucket_mask", u32)]
+
+
+class Settings(C.Structure):
+    """StepSettings<double>, src/fastlattice/LatticePhysics.hpp:248-284."""
+    _fields_ = [("dt", f64), ("gravity", f64 * 3), ("constraint_iterations", u32),
+                ("bond_integrator", u8), ("damping_fraction", f64), ("sphere_enabled", u8),
+                ("direct_arithmetic", u8), ("audit_energy", u8), ("plastic_yield_stretch", f64),
+                ("plastic_hardening", f64), ("plate_half_thickness", f64),
+                ("contact", ContactSettings), ("node_contact", NodeContactSettings),
+                ("plane_count", u32), ("planes", SupportPlane * MAX_SUPPORT_PLANES)]
+
+
+class Sphere(C.Structure):
+    """SphereState<double>, src/fastlattice/LatticePhysics.hpp:144-148."""
+    _fields_ = [("center", f64 * 3), ("velocity", f64 * 3), ("angular_velocity", f64 * 3),
+                ("radius", f64), ("mass", f64), ("inertia", f64)]
+
+
+class RunControl(C.Structure):
+    """RunControl, src/fastlattice/FastLattice.hpp:79-128."""
+    _fields_ = [("max_steps", u64), ("quiet_steps", u64), ("min_steps", u64),
+                ("no_failure_steps", u64), ("energy_flat_steps", u64),
+                ("energy_flat_fraction", f64), ("calm_steps", u64), ("calm_damage_margin", f64),
+                ("removable_energy_j", f64), ("capture_stride", u64), ("max_frames", u32),
+                ("steps_per_launch", u64), ("abandon", P(u8))]
+
+
+class ContactAccumulators(C.Structure):
+    """ContactAccumulators, src/fastlattice/LatticePhysics.hpp:379-388."""
+    _fields_ = [("impulse_contacts", u64), ("impulse_to_material", f64 * 3),
+                ("dissipated_kinetic_energy_j", f64), ("maximum_penetration_m", f64),
+                ("maximum_position_correction_m", f64), ("maximum_center_shift_m", f64),
+                ("candidate_overflow", u64), ("ball_support_events", u64)]
+
+
+class LoadLedger(C.Structure):
+    """ExternalLoadLedger, src/fastlattice/FastLattice.hpp:140-147."""
+    _fields_ = [("steps", u64), ("elapsed_s", f64), ("requested_impulse_n_s", f64 * 3),
+                ("impulse_n_s", f64 * 3), ("requested_angular_impulse_kg_m2_s", f64 * 3),
+                ("angular_impulse_kg_m2_s", f64 * 3), ("work_j", f64)]
+
+
+class RunStatus(C.Structure):
+    """RunStatus, src/fastlattice/FastLattice.hpp:182-245 (lattice-lane fields)."""
+    _fields_ = [("total_steps", u64), ("first_failure_step", u64), ("last_failure_step", u64),
+                ("failure_rounds", u32), ("broken_bonds", u32), ("removed_energy_j", f64),
+                ("last_energy_gain_step", u64), ("max_damage", f64), ("last_damage_gain_step", u64),
+                ("contact", ContactAccumulators), ("external_load", LoadLedger),
+                ("damping_dissipated_j", f64), ("striker_dissipated_j", f64),
+                ("energy_audited", u8), ("exit_reason", u32), ("launches", u32),
+                ("kernel_seconds", f64), ("wall_seconds", f64), ("frames_captured", u32),
+                ("phase_seconds", f64 * PHASE_COUNT), ("max_tensile_stretch", f32),
+                ("max_compressive_strain", f32), ("max_shear_strain", f32),
+                ("rank_deficient_nodes", u32), ("plastic_work_j", f64), ("max_plastic_stretch", f32)]
+
+
+class AssetView(C.Structure):
+    """LatticeAsset bonds + ActiveMatter state, src/matter/Lattice.hpp:26-44,82-93."""
+    _fields_ = [("node_count", u32), ("bond_count", u32), ("reference_position", P(f64)),
+                ("position", P(f64)), ("velocity", P(f64)), ("mass", P(f64)),
+                ("bond_a", P(u32)), ("bond_b", P(u32)), ("rest_length", P(f64)),
+                ("compliance", P(f64)), ("threshold", P(f64)), ("alive", P(u8)),
+                ("damage", P(f64)), ("failure_mode", P(u8))]
+
+
+class ScheduleView(C.Structure):
+    """LatticeSchedule, src/fastlattice/LatticeSchedule.hpp:28-46."""
+    _fields_ = [("node_count", u32), ("bond_count", u32), ("block_count", u32),
+                ("color_count", u32), ("boundary_bond_count", u32),
+                ("node_block_begin", P(u32)), ("bond_order", P(u32)),
+                ("bond_schedule_index", P(u32)), ("bond_block_begin", P(u32)),
+                ("range_begin", P(u32)), ("range_end", P(u32)), ("bond_color", P(u8))]
+
+
+class StateView(C.Structure):
+    """LatticeState, src/fastlattice/FastLattice.hpp:37-55."""
+    _fields_ = [("origin", f64 * 3), ("node_count", u32), ("bond_count", u32), ("max_degree", u32),
+                ("x0", P(f64)), ("u", P(f64)), ("u_prev", P(f64)), ("v", P(f64)),
+                ("inv_mass", P(f64)), ("mass", P(f64)), ("adj_offsets", P(u32)),
+                ("adj_bonds", P(u32)), ("nbr_bond", P(u32)), ("nbr_other", P(u32)),
+                ("nbr_rest", P(f64)), ("nbr_weight", P(f64)), ("nbr_alive", P(u8)),
+                ("bond_slot_a", P(u32)), ("bond_slot_b", P(u32)), ("bond_a", P(u32)),
+                ("bond_b", P(u32)), ("rest_edge", P(f64)), ("rest_length", P(f64)),
+                ("rest_length_sq_minus", P(f64)), ("weight", P(f64)), ("compliance", P(f64)),
+                ("threshold", P(f64)), ("alive", P(u8)), ("failure_mode", P(u8)),
+                ("damage", P(f64)), ("prev_tensile", P(f64)), ("prev_compressive", P(f64)),
+                ("prev_shear", P(f64)), ("plastic_extension", P(f64)), ("plastic_strain", P(f64))]
+
+
+def status_dict(s):
+    """A RunStatus as a plain dict (nested structs flattened one level)."""
+    out = {}
+    for name, _ in RunStatus._fields_:
+        val = getattr(s, name)
+        if isinstance(val, (ContactAccumulators, LoadLedger)):
+            out[name] = {k: (list(getattr(val, k)) if hasattr(getattr(val, k), "__len__") else getattr(val, k))
+                         for k, _ in val._fields_}
+        elif hasattr(val, "__len__"):
+            out[name] = list(val)
+        else:
+            out[name] = val
+    return out

@@ -1,0 +1,72 @@
+"""Runs a Scene through any library that exports the lattice C ABI.
+
+The sequence is the reference harness's (src/fastlattice/TileImpactScene.cpp:777-868):
+buildLatticeSchedule -> buildLatticeState -> backend upload -> external loads
+-> run -> download -> writeBackLatticeState (bond state back to asset order)
+-> findConnectedComponents.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import abi
+from .lattice import LatticeLibrary, control as make_control
+
+
+def prepare(lib: LatticeLibrary, scene, origin=(0.0, 0.0, 0.0)):
+    """Schedule and state for a scene, with clamps applied to the state."""
+    sched = lib.schedule_build(scene.asset, scene.slabs)
+    state = lib.state_build(scene.asset, sched, origin)
+    if scene.clamped is not None:
+        idx = np.nonzero(scene.clamped)[0]
+        state.inv_mass[idx] = 0.0
+        v = state.v.reshape(-1, 3)
+        v[idx] = 0.0
+    return sched, state
+
+
+def run_scene(lib: LatticeLibrary, scene, precision=abi.PRECISION_FLOAT, substeps=None, backend_args=(),
+              labels="host", chunks=None, control_kw=None, prepared=None, keep_backend=False):
+    """Advance `scene` and return the results in asset order.
+
+    labels: "host" (library's host components), "device" (backend fragment
+    labels), or None. chunks: list of substep counts for successive run()
+    calls (tests the status carrying over between calls).
+    """
+    sched, state = prepared if prepared is not None else prepare(lib, scene)
+    be = lib.backend(precision, sched, *backend_args)
+    be.upload(state, scene.settings, scene.sphere)
+    if scene.forces is not None:
+        be.set_external_forces(scene.forces, scene.force_substeps)
+    total = scene.substeps if substeps is None else substeps
+    plan = chunks if chunks is not None else [total]
+    t0 = time.perf_counter()
+    st = None
+    for n in plan:
+        st = be.run(make_control(n, **(control_kw or {})))
+    wall = time.perf_counter() - t0
+    out = state.copy()
+    sphere = be.download(out)
+    order = sched.bond_order
+    B = scene.bond_count
+    alive = np.empty(B, np.uint8)
+    alive[order] = out.alive
+    damage = np.empty(B)
+    damage[order] = out.damage
+    mode = np.empty(B, np.uint8)
+    mode[order] = out.failure_mode
+    res = dict(u=out.u.reshape(-1, 3), v=out.v.reshape(-1, 3), u_prev=out.u_prev.reshape(-1, 3),
+               alive=alive, damage=damage, mode=mode, status=st, sphere=sphere, wall=wall,
+               first_failure=np.sort(order[be.first_failure_bonds()]), schedule=sched, state=out)
+    if labels == "host":
+        res["labels"], res["components"] = lib.components(scene.node_count, scene.asset.bond_a, scene.asset.bond_b,
+                                                          alive)
+    elif labels == "device":
+        res["labels"], res["components"] = be.fragment_labels()
+    if keep_backend:
+        res["backend"] = be
+    else:
+        be.close()
+    return res

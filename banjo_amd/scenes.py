@@ -1,0 +1,266 @@
+"""Synthetic workloads for the lattice step, seeded and reproducible.
+
+Lattices are simple-cubic boxes with all 26 Moore neighbours. The reference's
+box generator (src/matter/BoxLattice.cpp:49-171) builds bonds from a Euclidean
+horizon, which cannot express the 26-neighbour stencil (horizon 1 gives 6
+neighbours, horizon 2 gives 32), so the stencil here keeps every one of its
+conventions with the Moore offsets instead: node index x + nx*(y + ny*z),
+reference positions centred on the lattice origin, bonds emitted node-major
+over the 13 positive half offsets in the generator's dz, dy, dx loop order
+(so node_a < node_b), rest length |offset| * cell computed in double.
+
+Per-bond randomness is keyed by (seed, node_a, node_b) with the reference's
+splitmix64 mix (src/matter/Lattice.cpp:28-47): value = seed ^ (a << 32) ^ b,
+then splitmix64, then U = (h >> 11) * 2**-53. Each random field uses its own
+stream seed.
+
+Clamps follow the recipe expressible in the reference's XPBD lane: a clamped
+node keeps its mass but gets inv_mass 0 and v 0, StepSettings gravity is zero,
+and gravity is applied as external forces m*g on movable nodes only
+(src/fastlattice/ExternalLoads.hpp:42-46 skips zero forces, so fixed nodes are
+never loaded).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from .lattice import Asset
+
+# The 13 positive half offsets of the Moore stencil in the generator's
+# dz-outer, dy, dx-inner loop order (src/matter/BoxLattice.cpp:141-147 with
+# positiveHalfOffset, src/matter/Lattice.cpp:49-52).
+HALF_OFFSETS = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+                if dz > 0 or (dz == 0 and dy > 0) or (dz == 0 and dy == 0 and dx > 0)]
+
+INF = float("inf")
+
+
+def splitmix64(x):
+    """splitmix64 on a uint64 array (src/matter/Lattice.cpp:28-33)."""
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def bond_uniform(seed, a, b):
+    """U in [0, 1) per bond, keyed like signedVariation (src/matter/Lattice.cpp:35-47)."""
+    v = np.uint64(seed) ^ (np.asarray(a, np.uint64) << np.uint64(32)) ^ np.asarray(b, np.uint64)
+    h = splitmix64(v)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def box26(nx, ny, nz, cell):
+    """Reference positions (N,3) and bond ends/rest lengths of a Moore box."""
+    x = np.arange(nx, dtype=np.float64)
+    y = np.arange(ny, dtype=np.float64)
+    z = np.arange(nz, dtype=np.float64)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    pos = np.stack([(X + 0.5 - 0.5 * nx) * cell, (Y + 0.5 - 0.5 * ny) * cell,
+                    (Z + 0.5 - 0.5 * nz) * cell], axis=-1).reshape(-1, 3)
+    ix = np.arange(nx, dtype=np.int64)
+    iy = np.arange(ny, dtype=np.int64)
+    iz = np.arange(nz, dtype=np.int64)
+    GZ, GY, GX = np.meshgrid(iz, iy, ix, indexing="ij")
+    GX, GY, GZ = GX.reshape(-1), GY.reshape(-1), GZ.reshape(-1)
+    node = GX + nx * (GY + ny * GZ)
+    others, valid, lengths = [], [], []
+    for dx, dy, dz in HALF_OFFSETS:
+        X2, Y2, Z2 = GX + dx, GY + dy, GZ + dz
+        ok = (X2 >= 0) & (Y2 >= 0) & (Z2 >= 0) & (X2 < nx) & (Y2 < ny) & (Z2 < nz)
+        others.append(X2 + nx * (Y2 + ny * Z2))
+        valid.append(ok)
+        lengths.append(math.sqrt(dx * dx + dy * dy + dz * dz) * cell)
+    other = np.stack(others, axis=1)
+    mask = np.stack(valid, axis=1)
+    L = np.broadcast_to(np.array(lengths), other.shape)
+    a = np.broadcast_to(node[:, None], other.shape)[mask].astype(np.uint32)
+    b = other[mask].astype(np.uint32)
+    return pos, a, b, L[mask].astype(np.float64)
+
+
+def box_slabs(nx, ny, nz, horizon, requested):
+    """boxLatticeSlabs (src/matter/BoxLattice.cpp:175-195): z-slab node offsets."""
+    blocks = max(1, requested)
+    blocks = min(blocks, max(1, nz // horizon))
+    layer = nx * ny
+    out = [((nz * b) // blocks) * layer for b in range(blocks + 1)]
+    out[-1] = layer * nz
+    return np.array(out, np.uint32)
+
+
+def ground_plane(y, node_radius, restitution=0.1, static_friction=0.6, dynamic_friction=0.5):
+    """An infinite support plane with normal +y at height y (TileImpactScene.cpp:83-96,317-321)."""
+    p = abi.SupportPlane()
+    p.point[:] = (0.0, y, 0.0)
+    p.normal[:] = (0.0, 1.0, 0.0)
+    p.tangent[:] = (1.0, 0.0, 0.0)
+    p.bitangent[:] = (0.0, 0.0, 1.0)
+    p.restitution, p.static_friction, p.dynamic_friction = restitution, static_friction, dynamic_friction
+    p.node_radius = node_radius
+    p.reach_capped = 0
+    p.footprint_count = 1
+    p.footprints[0].center_t = 0.0
+    p.footprints[0].center_b = 0.0
+    p.footprints[0].half_t = INF
+    p.footprints[0].half_b = INF
+    return p
+
+
+def settings(dt, gravity=(0.0, 0.0, 0.0), iterations=1, planes=(), sphere=False, cell=0.01,
+             ball_radius=0.0, direct=False, damping=0.0, yield_stretch=0.0, hardening=0.0):
+    """StepSettings<double> as TileImpactScene buildSettings fills it (TileImpactScene.cpp:259-336)."""
+    s = abi.Settings()
+    s.dt = dt
+    s.gravity[:] = gravity
+    s.constraint_iterations = iterations
+    s.bond_integrator = 0
+    s.damping_fraction = damping
+    s.sphere_enabled = 1 if sphere else 0
+    s.direct_arithmetic = 1 if direct else 0
+    s.plastic_yield_stretch = yield_stretch
+    s.plastic_hardening = hardening
+    s.contact.static_friction = 0.5
+    s.contact.dynamic_friction = 0.4
+    s.contact.restitution = 0.1
+    s.contact.restitution_speed_threshold = 0.5
+    s.contact.node_contact_radius = 0.5 * cell
+    s.contact.contact_margin = 1.0e-5
+    s.contact.prefilter_slack = 0.05 * ball_radius
+    s.node_contact.mode = 0
+    s.plane_count = len(planes)
+    for k, p in enumerate(planes):
+        s.planes[k] = p
+    return s
+
+
+def sphere_state(center=(0, 0, 0), velocity=(0, 0, 0), radius=1.0, mass=1.0):
+    sp = abi.Sphere()
+    sp.center[:] = center
+    sp.velocity[:] = velocity
+    sp.radius = radius
+    sp.mass = mass
+    sp.inertia = 0.4 * mass * radius * radius
+    return sp
+
+
+@dataclass
+class Scene:
+    """One lattice workload: asset, slabs, settings, sphere, loads and a run length."""
+    name: str
+    dims: tuple
+    cell: float
+    asset: Asset
+    slabs: np.ndarray
+    settings: abi.Settings
+    sphere: abi.Sphere
+    substeps: int
+    clamped: np.ndarray | None = None        # bool (N,)
+    forces: np.ndarray | None = None         # (N,3) world-space external forces
+    force_substeps: int = 0
+    info: dict = field(default_factory=dict)
+
+    @property
+    def node_count(self):
+        return self.asset.node_count
+
+    @property
+    def bond_count(self):
+        return self.asset.bond_count
+
+
+def _thresholds(start, end):
+    B = start.shape[0]
+    t = np.empty((B, 6))
+    t[:, 0] = start
+    t[:, 1] = end
+    t[:, 2:] = INF
+    return t
+
+
+def cantilever(nx, ny, nz, cell=0.01, node_mass=1.0e-3, compliance=None, strain=None, tip_load=5.0,
+               tip_node=None, gravity=9.81, dt=1.0 / 1920.0, substeps=1920, blocks=1, seed=971, name="cantilever"):
+    """A bar clamped on its x=0 face, gravity and a downward point load at the far end.
+
+    compliance/strain: floats (uniform) or callables (a, b) -> per-bond values.
+    """
+    pos, a, b, L = box26(nx, ny, nz, cell)
+    N, B = pos.shape[0], a.shape[0]
+    c = np.full(B, compliance if compliance is not None else 1.0e-6) if not callable(compliance) else compliance(a, b)
+    e = np.full(B, strain if strain is not None else 0.01) if not callable(strain) else strain(a, b)
+    mass = np.full(N, node_mass)
+    asset = Asset(reference_position=pos, mass=mass, bond_a=a, bond_b=b, rest_length=L, compliance=c,
+                  threshold=_thresholds(e, e))
+    gx = np.arange(N) % nx
+    clamped = gx == 0
+    forces = np.zeros((N, 3))
+    forces[~clamped, 1] = -node_mass * gravity
+    if tip_node is None:
+        tip_node = (nx - 1) + nx * ((ny // 2) + ny * (nz // 2))
+    forces[tip_node, 1] -= tip_load
+    st = settings(dt, gravity=(0.0, 0.0, 0.0), cell=cell)
+    return Scene(name=name, dims=(nx, ny, nz), cell=cell, asset=asset, slabs=box_slabs(nx, ny, nz, 1, blocks),
+                 settings=st, sphere=sphere_state(center=(0.0, 100.0, 0.0)), substeps=substeps,
+                 clamped=clamped, forces=forces, force_substeps=substeps,
+                 info=dict(tip_node=int(tip_node), tip_load_n=tip_load))
+
+
+def config1(substeps=1920, blocks=1):
+    """16x16x16 cantilever, 1 cm, 1 g, c 1e-6, break strain 1%, 5 N tip load, dt 1/1920 s."""
+    return cantilever(16, 16, 16, substeps=substeps, blocks=blocks, name="c1_cantilever_16^3")
+
+
+def log_uniform_compliance(seed, lo=1.0e-7, hi=1.0e-5):
+    def f(a, b):
+        U = bond_uniform(seed, a, b)
+        return np.exp(U * math.log(hi / lo)) * lo
+    return f
+
+
+def weibull_strain(seed, k, scale):
+    def f(a, b):
+        U = bond_uniform(seed, a, b)
+        return scale * (-np.log1p(-U)) ** (1.0 / k)
+    return f
+
+
+def impact(nx, ny, nz, cell=0.01, node_mass=1.0e-3, ball_radius=0.2, ball_mass=10.0, ball_speed=8.0,
+           gap=0.002, dt=1.0 / 2400.0, substeps=1200, blocks=1, seed=971, weibull_k=6.0, weibull_scale=0.015,
+           name="impact"):
+    """A block on a ground plane struck on its top-face centre by a rigid sphere.
+
+    Compliance log-uniform in [1e-7, 1e-5]; tensile break strain Weibull(k, scale)
+    with start == end (a step); compression and shear disabled (+inf).
+    """
+    pos, a, b, L = box26(nx, ny, nz, cell)
+    N, B = pos.shape[0], a.shape[0]
+    c = log_uniform_compliance(seed)(a, b)
+    e = weibull_strain(seed + 1, weibull_k, weibull_scale)(a, b)
+    asset = Asset(reference_position=pos, mass=np.full(N, node_mass), bond_a=a, bond_b=b, rest_length=L,
+                  compliance=c, threshold=_thresholds(e, e))
+    top = pos[:, 1].max() + 0.5 * cell
+    bottom = pos[:, 1].min() - 0.5 * cell
+    st = settings(dt, gravity=(0.0, -9.81, 0.0), planes=[ground_plane(bottom, 0.5 * cell)], sphere=True,
+                  cell=cell, ball_radius=ball_radius)
+    sp = sphere_state(center=(0.0, top + ball_radius + gap, 0.0), velocity=(0.0, -ball_speed, 0.0),
+                      radius=ball_radius, mass=ball_mass)
+    return Scene(name=name, dims=(nx, ny, nz), cell=cell, asset=asset, slabs=box_slabs(nx, ny, nz, 1, blocks),
+                 settings=st, sphere=sp, substeps=substeps)
+
+
+def config2(substeps=1200, blocks=1):
+    """128 (x) x 64 (y, vertical) x 128 (z) block, 10 kg r=0.2 m sphere at 8 m/s, dt 1/2400 s."""
+    return impact(128, 64, 128, substeps=substeps, blocks=blocks, name="c2_impact_128x64x128")
+
+
+def small_impact(nx=24, ny=12, nz=24, ball_radius=0.05, ball_mass=2.0, ball_speed=8.0, substeps=200, blocks=1,
+                 seed=971):
+    """A small block of the config-2 family for parity tests."""
+    return impact(nx, ny, nz, ball_radius=ball_radius, ball_mass=ball_mass, ball_speed=ball_speed,
+                  substeps=substeps, blocks=blocks, seed=seed, name=f"impact_{nx}x{ny}x{nz}")

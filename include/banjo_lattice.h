@@ -1,0 +1,327 @@
+/*
+ * banjo_lattice.h -- C ABI of the MI355X bonded-lattice fracture step.
+ *
+ * This is the drop-in boundary of the fast explicit lattice lane. Every entry
+ * point below replaces a piece of the reference's C++ lattice API
+ * (src/fastlattice/FastLattice.hpp), cited per function. The structs mirror
+ * the reference's value types field for field, widened to plain C:
+ *
+ *   blx_settings      <- StepSettings<double>     (src/fastlattice/LatticePhysics.hpp:249-284)
+ *   blx_sphere        <- SphereState<double>      (src/fastlattice/LatticePhysics.hpp:145-148)
+ *   blx_support_plane <- SupportPlane<double>     (src/fastlattice/LatticePhysics.hpp:122-136)
+ *   blx_run_control   <- RunControl               (src/fastlattice/FastLattice.hpp:79-128)
+ *   blx_run_status    <- RunStatus                (src/fastlattice/FastLattice.hpp:182-245)
+ *   blx_schedule_view <- LatticeSchedule          (src/fastlattice/LatticeSchedule.hpp:28-46)
+ *   blx_state_view    <- LatticeState             (src/fastlattice/FastLattice.hpp:37-55)
+ *   blx_asset_view    <- LatticeAsset bonds + ActiveMatter nodes/bonds
+ *                        (src/matter/Lattice.hpp:26-44,82-93; src/fracture/ActiveMatter.hpp:18-57)
+ *
+ * No C++ and no HIP types cross this boundary: plain pointers, sizes and
+ * integer status codes. Every function returns BLX_OK or a negative code; the
+ * message of the last failure on the calling thread is blx_last_error(). The
+ * codes map onto the reference's exception types (FastLattice.hpp:400-406,
+ * CudaLatticeUnavailable.cpp:19-25):
+ *   BLX_ERR_RUNTIME   std::runtime_error   (no device, device failure)
+ *   BLX_ERR_INVALID   std::invalid_argument (bad input, unsupported feature)
+ *   BLX_ERR_OVERFLOW  std::overflow_error  (ledger or counter overflow)
+ *   BLX_ERR_LOGIC     std::logic_error     (misuse: run before upload, ...)
+ *
+ * Threading: one host thread drives a given backend handle; many handles may
+ * coexist, on one device or several.
+ */
+#ifndef BANJO_LATTICE_H
+#define BANJO_LATTICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLX_OK 0
+#define BLX_ERR_RUNTIME (-1)
+#define BLX_ERR_INVALID (-2)
+#define BLX_ERR_OVERFLOW (-3)
+#define BLX_ERR_LOGIC (-4)
+
+#define BLX_PRECISION_FLOAT 0
+#define BLX_PRECISION_DOUBLE 1
+
+#define BLX_MAX_SUPPORT_PLANES 2          /* LatticePhysics.hpp:103 */
+#define BLX_MAX_FOOTPRINTS 4              /* LatticePhysics.hpp:104 */
+#define BLX_MAX_CANDIDATES_PER_BLOCK 1024 /* LatticePhysics.hpp:105 */
+#define BLX_NO_BOND 0xffffffffu           /* LatticePhysics.hpp:106 */
+#define BLX_PHASE_COUNT 16                /* FastLattice.hpp:130 */
+
+#define BLX_NODE_CONTACT_OFF 0
+#define BLX_NODE_CONTACT_MEASURE 1
+#define BLX_NODE_CONTACT_ON 2
+
+#define BLX_BOND_XPBD 0
+#define BLX_BOND_VELOCITY_VERLET 1
+
+typedef struct blx_footprint {
+    double center_t, center_b, half_t, half_b;
+} blx_footprint;
+
+typedef struct blx_support_plane {
+    double point[3], normal[3], tangent[3], bitangent[3];
+    double restitution, static_friction, dynamic_friction;
+    double node_radius;
+    uint8_t reach_capped;
+    uint32_t footprint_count;
+    blx_footprint footprints[BLX_MAX_FOOTPRINTS];
+} blx_support_plane;
+
+typedef struct blx_contact_settings {
+    double static_friction, dynamic_friction, restitution, restitution_speed_threshold;
+    double node_contact_radius, contact_margin, prefilter_slack;
+} blx_contact_settings;
+
+typedef struct blx_node_contact_settings {
+    uint8_t mode;
+    double radius, skin, margin;
+    double restitution, restitution_speed_threshold;
+    double static_friction, dynamic_friction;
+    uint32_t bucket_mask;
+} blx_node_contact_settings;
+
+typedef struct blx_settings {
+    double dt;
+    double gravity[3];
+    uint32_t constraint_iterations;
+    uint8_t bond_integrator;
+    double damping_fraction;
+    uint8_t sphere_enabled;
+    uint8_t direct_arithmetic;
+    uint8_t audit_energy;
+    double plastic_yield_stretch;
+    double plastic_hardening;
+    double plate_half_thickness;
+    blx_contact_settings contact;
+    blx_node_contact_settings node_contact;
+    uint32_t plane_count;
+    blx_support_plane planes[BLX_MAX_SUPPORT_PLANES];
+} blx_settings;
+
+typedef struct blx_sphere {
+    double center[3], velocity[3], angular_velocity[3];
+    double radius, mass, inertia;
+} blx_sphere;
+
+typedef struct blx_run_control {
+    uint64_t max_steps;
+    uint64_t quiet_steps;
+    uint64_t min_steps;
+    uint64_t no_failure_steps;
+    uint64_t energy_flat_steps;
+    double energy_flat_fraction; /* reference default 1e-3 */
+    uint64_t calm_steps;
+    double calm_damage_margin;   /* reference default 0.5 */
+    double removable_energy_j;
+    uint64_t capture_stride;
+    uint32_t max_frames;
+    uint64_t steps_per_launch;   /* substeps per device batch; 0 = library default */
+    const volatile uint8_t *abandon; /* polled between batches; NULL never stops */
+} blx_run_control;
+
+typedef struct blx_contact_accumulators {
+    uint64_t impulse_contacts;
+    double impulse_to_material[3];
+    double dissipated_kinetic_energy_j;
+    double maximum_penetration_m;
+    double maximum_position_correction_m;
+    double maximum_center_shift_m;
+    uint64_t candidate_overflow;
+    uint64_t ball_support_events;
+} blx_contact_accumulators;
+
+typedef struct blx_load_ledger {
+    uint64_t steps;
+    double elapsed_s;
+    double requested_impulse_n_s[3], impulse_n_s[3];
+    double requested_angular_impulse_kg_m2_s[3], angular_impulse_kg_m2_s[3];
+    double work_j;
+} blx_load_ledger;
+
+typedef struct blx_run_status {
+    uint64_t total_steps;
+    uint64_t first_failure_step;   /* UINT64_MAX: none */
+    uint64_t last_failure_step;    /* UINT64_MAX: none */
+    uint32_t failure_rounds;
+    uint32_t broken_bonds;
+    double removed_energy_j;
+    uint64_t last_energy_gain_step; /* UINT64_MAX: none */
+    double max_damage;
+    uint64_t last_damage_gain_step;
+    blx_contact_accumulators contact;
+    blx_load_ledger external_load;
+    double damping_dissipated_j;
+    double striker_dissipated_j;
+    uint8_t energy_audited;
+    uint32_t exit_reason;
+    uint32_t launches;
+    double kernel_seconds;
+    double wall_seconds;
+    uint32_t frames_captured;
+    double phase_seconds[BLX_PHASE_COUNT];
+    float max_tensile_stretch, max_compressive_strain, max_shear_strain;
+    uint32_t rank_deficient_nodes;
+    double plastic_work_j;
+    float max_plastic_stretch;
+} blx_run_status;
+
+/* LatticeAsset bonds + ActiveMatter nodes and bond state, asset order. */
+typedef struct blx_asset_view {
+    uint32_t node_count, bond_count;
+    const double *reference_position; /* 3N world */
+    const double *position;           /* 3N world, current */
+    const double *velocity;           /* 3N */
+    const double *mass;               /* N */
+    const uint32_t *bond_a, *bond_b;  /* B */
+    const double *rest_length;        /* B */
+    const double *compliance;         /* B */
+    const double *threshold;          /* 6B: ds, de, cs, ce, ss, se */
+    const uint8_t *alive;             /* B, NULL = all alive */
+    const double *damage;             /* B, NULL = zero */
+    const uint8_t *failure_mode;      /* B, NULL = none */
+} blx_asset_view;
+
+/* LatticeSchedule: sizes plus caller-owned arrays. */
+typedef struct blx_schedule_view {
+    uint32_t node_count, bond_count;
+    uint32_t block_count, color_count, boundary_bond_count;
+    uint32_t *node_block_begin;    /* block_count + 1 */
+    uint32_t *bond_order;          /* B: schedule index -> asset bond */
+    uint32_t *bond_schedule_index; /* B: asset bond -> schedule index */
+    uint32_t *bond_block_begin;    /* block_count + 1 */
+    uint32_t *range_begin;         /* block_count * 2 * max(color_count, 1) */
+    uint32_t *range_end;           /* same */
+    uint8_t *bond_color;           /* B, schedule order */
+} blx_schedule_view;
+
+/* LatticeState: schedule order, double precision, caller-owned arrays. */
+typedef struct blx_state_view {
+    double origin[3];
+    uint32_t node_count, bond_count, max_degree;
+    double *x0, *u, *u_prev, *v;          /* 3N */
+    double *inv_mass, *mass;              /* N */
+    uint32_t *adj_offsets;                /* N + 1 */
+    uint32_t *adj_bonds;                  /* 2B */
+    uint32_t *nbr_bond, *nbr_other;       /* max_degree * N, k-major */
+    double *nbr_rest;                     /* 3 * max_degree * N */
+    double *nbr_weight;                   /* max_degree * N */
+    uint8_t *nbr_alive;                   /* max_degree * N */
+    uint32_t *bond_slot_a, *bond_slot_b;  /* B */
+    uint32_t *bond_a, *bond_b;            /* B */
+    double *rest_edge;                    /* 3B */
+    double *rest_length, *rest_length_sq_minus, *weight, *compliance; /* B */
+    double *threshold;                    /* 6B */
+    uint8_t *alive, *failure_mode;        /* B */
+    double *damage, *prev_tensile, *prev_compressive, *prev_shear;    /* B */
+    double *plastic_extension, *plastic_strain;                       /* B */
+} blx_state_view;
+
+/* ---- errors and probes ------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" after success). */
+const char *blx_last_error(void);
+
+/* cudaLatticeAvailable (FastLattice.hpp:403): a gfx950 device is present. */
+int blx_available(void);
+/* cudaLatticeDescription (FastLattice.hpp:404): writes a NUL-terminated text. */
+int blx_description(char *out, size_t capacity);
+/* cudaLatticeMultiprocessorCount (FastLattice.hpp:405): compute units. */
+uint32_t blx_multiprocessor_count(void);
+
+/* ---- host builders (lattice build) -------------------------------------- */
+
+/* Sizes of a schedule for this asset and slab split, for the caller to
+ * allocate blx_schedule_view arrays: colour count is only known after the
+ * colouring, so range arrays are sized for 64 colours (the reference cap). */
+int blx_schedule_build(const blx_asset_view *asset, const uint32_t *node_block_begin,
+                       uint32_t block_count, blx_schedule_view *out);
+/* buildLatticeSchedule (LatticeSchedule.cpp:18-104). out arrays must hold:
+ * node_block_begin block_count+1, bond_order B, bond_schedule_index B,
+ * bond_block_begin block_count+1, range_begin/range_end block_count*2*64,
+ * bond_color B. On return color_count/boundary_bond_count are set. */
+
+/* Max degree of the asset's adjacency, to size the state's slot arrays. */
+int blx_state_max_degree(const blx_asset_view *asset, uint32_t *max_degree);
+/* buildLatticeState (FastLattice.cpp:47-169). Every out array must be sized
+ * from node_count, bond_count and max_degree. */
+int blx_state_build(const blx_asset_view *asset, const blx_schedule_view *schedule,
+                    const double origin[3], blx_state_view *out);
+
+/* Box lattice with all 26 Moore neighbours (13 positive half offsets, bond
+ * node_a < node_b, node index x + nx*(y + ny*z)), the conventions of
+ * generateBoxTileLattice (matter/BoxLattice.cpp:49-171) with the Moore
+ * stencil in place of the Euclidean horizon. Writes 3N reference positions
+ * centred on the origin and the bond ends, returns the bond count in
+ * *bond_count (call once with bond_a == NULL to size). */
+int blx_box26_bonds(uint32_t nx, uint32_t ny, uint32_t nz, double cell,
+                    double *reference_position, uint32_t *bond_a, uint32_t *bond_b,
+                    uint64_t *bond_count);
+/* boxLatticeSlabs (matter/BoxLattice.cpp:175-195): block_count+1 offsets. */
+int blx_box_slabs(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t horizon,
+                  uint32_t requested_blocks, uint32_t *out, uint32_t *block_count);
+
+/* findConnectedComponents (fracture/ConnectedComponents.cpp:51-87), host:
+ * label[i] = canonical fragment id (size descending, then smallest node). */
+int blx_components_host(uint32_t node_count, uint32_t bond_count, const uint32_t *bond_a,
+                        const uint32_t *bond_b, const uint8_t *alive, uint32_t *label,
+                        uint32_t *component_count);
+
+/* ---- device backend (LatticeBackend, FastLattice.hpp:255-369) ----------- */
+
+typedef struct blx_backend blx_backend;
+
+/* makeCudaLatticeBackend (FastLattice.hpp:401-402): the schedule is copied. */
+int blx_backend_create(int device, int precision, const blx_schedule_view *schedule,
+                       blx_backend **out);
+int blx_backend_destroy(blx_backend *backend);
+/* LatticeBackend::name. */
+int blx_backend_name(const blx_backend *backend, char *out, size_t capacity);
+/* LatticeBackend::upload (FastLattice.hpp:267-268): deep copy to the device,
+ * resets status, loads, lambda and the start sample. */
+int blx_backend_upload(blx_backend *backend, const blx_state_view *state,
+                       const blx_settings *settings, const blx_sphere *sphere);
+/* LatticeBackend::setExternalForces (FastLattice.hpp:274-277): 3N world-space
+ * forces in node order held for `substeps` substeps; NULL with 0 cancels. */
+int blx_backend_set_external_forces(blx_backend *backend, const double *forces_3n,
+                                    uint64_t substeps);
+/* LatticeBackend::run (FastLattice.hpp:360): continue from the current state;
+ * the status accumulates across calls. */
+int blx_backend_run(blx_backend *backend, const blx_run_control *control,
+                    blx_run_status *status);
+/* LatticeBackend::status (FastLattice.hpp:364). */
+int blx_backend_status(const blx_backend *backend, blx_run_status *status);
+/* LatticeBackend::download (FastLattice.hpp:361): the dynamic fields of the
+ * state (u, u_prev, v, alive, failure_mode, damage, prev_*, plastic_*) and
+ * the sphere, widened to double. Static fields are left untouched. */
+int blx_backend_download(blx_backend *backend, blx_state_view *state, blx_sphere *sphere);
+/* LatticeBackend::firstFailureBonds (FastLattice.hpp:368): schedule indices
+ * removed in the first failure round; *count is set, at most capacity are
+ * written (call with capacity 0 to size). */
+int blx_backend_first_failure_bonds(const blx_backend *backend, uint32_t *out,
+                                    uint64_t capacity, uint64_t *count);
+/* LatticeBackend::takeFrames (FastLattice.hpp:363): frames captured since the
+ * last call. Query with NULL buffers to get frame count; each frame holds
+ * step, 3N float u, B alive, B float damage and the sphere. */
+int blx_backend_frame_count(const blx_backend *backend, uint32_t *count);
+int blx_backend_take_frame(blx_backend *backend, uint32_t index, uint64_t *step, float *u,
+                           uint8_t *alive, float *damage, blx_sphere *sphere);
+int blx_backend_clear_frames(blx_backend *backend);
+
+/* Fragments on the device: integer union-find over the live bonds and the
+ * canonical relabel of findConnectedComponents (ConnectedComponents.cpp:51-87),
+ * label[i] for node i; *component_count pieces. */
+int blx_backend_fragment_labels(blx_backend *backend, uint32_t *label,
+                                uint32_t *component_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BANJO_LATTICE_H */
