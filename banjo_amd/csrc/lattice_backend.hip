@@ -9,6 +9,7 @@
 #include "lattice_kernels.hpp"
 
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -79,6 +80,8 @@ public:
     virtual void takeFrame(uint32_t, uint64_t *, float *, uint8_t *, float *, blx_sphere *) = 0;
     virtual void clearFrames() = 0;
     virtual uint32_t labels(uint32_t *) = 0;
+    virtual void setProfiling(bool) = 0;
+    virtual uint32_t kernelStats(uint32_t, std::string &, double &, uint64_t &) const = 0;
 };
 
 struct Schedule {
@@ -117,6 +120,7 @@ public:
         if (stream_) (void)hipStreamSynchronize(stream_);
         if (ev0_) (void)hipEventDestroy(ev0_);
         if (ev1_) (void)hipEventDestroy(ev1_);
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
 
@@ -305,7 +309,7 @@ public:
         ctl.calm_margin = c.calm_damage_margin;
         ctl.removable = c.removable_energy_j;
         writeCtl(ctl);
-        const uint64_t batch = c.steps_per_launch ? c.steps_per_launch : kDefaultBatch;
+        const uint64_t batch = profilingBatch(c.steps_per_launch ? c.steps_per_launch : kDefaultBatch);
         uint64_t done = 0;
         unsigned reason = 0;
         while (done < c.max_steps) {
@@ -326,6 +330,7 @@ public:
             float ms = 0.f;
             HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
             run_status_.kernel_seconds += double(ms) * 1e-3;
+            profCollect();
             ++run_status_.launches;
             ctl = readCtl();
             done += ctl.step - step;
@@ -443,30 +448,49 @@ public:
     uint32_t labels(uint32_t *label) override {
         if (!uploaded_) throw Logic("fragments before upload");
         HIP_OK(hipSetDevice(device_));
-        DevBuf parent, size, rank, lab;
-        parent.alloc(size_t(N_) * 4);
-        size.alloc(size_t(N_) * 4);
-        rank.alloc(size_t(N_) * 4);
-        lab.alloc(size_t(N_) * 4);
-        HIP_OK(hipMemsetAsync(size.p, 0, size_t(N_) * 4, stream_));
-        const uint32_t g = gridFor(N_);
-        hipLaunchKernelGGL(k_cc_init, dim3(g), dim3(kWG), 0, stream_, parent.as<uint32_t>(), N_);
-        hipLaunchKernelGGL(k_cc_hook<R>, dim3(gridFor(B_)), dim3(kWG), 0, stream_, L_, parent.as<uint32_t>());
-        hipLaunchKernelGGL(k_cc_compress, dim3(g), dim3(kWG), 0, stream_, parent.as<uint32_t>(), size.as<uint32_t>(), N_);
-        HIP_OK(hipGetLastError());
-        std::vector<uint32_t> root(N_), sz(N_);
-        HIP_OK(hipMemcpyAsync(root.data(), parent.p, size_t(N_) * 4, hipMemcpyDeviceToHost, stream_));
-        HIP_OK(hipMemcpyAsync(sz.data(), size.p, size_t(N_) * 4, hipMemcpyDeviceToHost, stream_));
+        const uint32_t c = labelsOnDevice();
+        HIP_OK(hipMemcpyAsync(label, cc_label_.p, size_t(N_) * 4, hipMemcpyDeviceToHost, stream_));
         HIP_OK(hipStreamSynchronize(stream_));
-        // Canonical ids: size descending, then smallest node (ConnectedComponents.cpp:77-85).
-        std::vector<uint32_t> roots;
-        for (uint32_t i = 0; i < N_; ++i)
-            if (root[i] == i) roots.push_back(i);
-        std::stable_sort(roots.begin(), roots.end(), [&](uint32_t l, uint32_t r) { return sz[l] > sz[r]; });
-        std::vector<uint32_t> rk(N_, 0);
-        for (uint32_t k = 0; k < roots.size(); ++k) rk[roots[k]] = k;
-        for (uint32_t i = 0; i < N_; ++i) label[i] = rk[root[i]];
-        return uint32_t(roots.size());
+        return c;
+    }
+
+    // Union-find, roots, canonical rank and labels, all on the device; the
+    // labels are left in cc_label_. Returns the component count.
+    uint32_t labelsOnDevice() {
+        const size_t n4 = size_t(N_) * 4;
+        cc_parent_.alloc(n4);
+        cc_root_.alloc(n4);
+        cc_size_.alloc(n4);
+        cc_rank_.alloc(n4);
+        cc_label_.alloc(n4);
+        cc_keys_.alloc(size_t(N_) * 8);
+        cc_sorted_.alloc(size_t(N_) * 8);
+        cc_count_.alloc(4);
+        size_t temp = 0;
+        HIP_OK(rocprim::radix_sort_keys(nullptr, temp, cc_keys_.as<uint64_t>(), cc_sorted_.as<uint64_t>(), N_, 0, 64,
+                                        stream_));
+        cc_temp_.alloc(temp);
+        const uint32_t g = gridFor(N_);
+        auto *parent = cc_parent_.as<uint32_t>(), *root = cc_root_.as<uint32_t>(), *size = cc_size_.as<uint32_t>();
+        HIP_OK(hipMemsetAsync(cc_count_.p, 0, 4, stream_));
+        hipLaunchKernelGGL(k_cc_init, dim3(g), dim3(kWG), 0, stream_, parent, size, N_);
+        hipLaunchKernelGGL(k_cc_hook<R>, dim3(gridFor(B_)), dim3(kWG), 0, stream_, L_, parent);
+        hipLaunchKernelGGL(k_cc_roots, dim3(g), dim3(kWG), 0, stream_, parent, root, size, N_);
+        hipLaunchKernelGGL(k_cc_keys, dim3(g), dim3(kWG), 0, stream_, root, size, cc_keys_.as<uint64_t>(),
+                           cc_count_.as<uint32_t>(), N_);
+        HIP_OK(hipGetLastError());
+        uint32_t c = 0;
+        HIP_OK(hipMemcpyAsync(&c, cc_count_.p, 4, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        size_t bytes = cc_temp_.bytes;
+        HIP_OK(rocprim::radix_sort_keys(cc_temp_.p, bytes, cc_keys_.as<uint64_t>(), cc_sorted_.as<uint64_t>(), c, 0, 64,
+                                        stream_));
+        hipLaunchKernelGGL(k_cc_rank, dim3(gridFor(c)), dim3(kWG), 0, stream_, cc_sorted_.as<uint64_t>(),
+                           cc_rank_.as<uint32_t>(), c);
+        hipLaunchKernelGGL(k_cc_label, dim3(g), dim3(kWG), 0, stream_, root, cc_rank_.as<uint32_t>(),
+                           cc_label_.as<uint32_t>(), N_);
+        HIP_OK(hipGetLastError());
+        return c;
     }
 
 private:
@@ -490,12 +514,14 @@ private:
     // Stage tables: per colour stage, a run of workgroups over [begin, end) bond ranges.
     std::vector<uint2> wg_host_;
     std::vector<std::pair<uint32_t, uint32_t>> stages_; // (first workgroup, workgroup count)
+    size_t interior_stages_ = 0;
     DevBuf wg_;
     DevBuf x04_, u4_, up4_, v4_, rinv_, unm_, sa_, sb_, valid_, dirty_, approach_, engaged_, cand_;
     DevBuf nbr_other_, nbr_rx_, nbr_ry_, nbr_rz_, nbr_w_, nbr_alive_;
     DevBuf rec_, alive_, lambda_, pext_, pstr_, thr_, pvd_, mode_, slots_, stamp_, removed_;
     DevBuf tile_cand_, tile_bits_, tile_plastic_, load_part_, fext_, freq_, cand_all_, cand_lo_, cand_cnt_, nbb_;
     DevBuf first_fail_, ctl_, sphere_;
+    DevBuf cc_parent_, cc_root_, cc_size_, cc_rank_, cc_label_, cc_keys_, cc_sorted_, cc_count_, cc_temp_;
 
     template <class T> void copyDown(std::vector<T> &dst, const DevBuf &src) {
         if (!dst.empty()) HIP_OK(hipMemcpy(dst.data(), src.p, dst.size() * sizeof(T), hipMemcpyDeviceToHost));
@@ -581,6 +607,7 @@ private:
             }
         };
         add(0, 0, 1);
+        interior_stages_ = stages_.size();
         add(1, 0, 2);
         add(1, 1, 2);
         toDevice(wg_, wg_host_);
@@ -687,46 +714,121 @@ private:
         frames_.push_back(std::move(f));
     }
 
+    // Kernel kinds for the optional per-launch event timing, each with the
+    // reference phase id it belongs to (FastLattice.cpp:22-26).
+    enum Kernel : uint32_t { KStrainStart, KBondStart, KPredict, KPass1, KSolve, KProject, KVelocity, KDamp,
+                             KSupportVelocity, KPass2, KStrain, KBondEnd, KFinish, KCount };
+    static const char *kernelName(uint32_t k) {
+        static const char *names[KCount] = {"k_strain_start", "k_bond_start", "k_predict", "k_pass1", "k_solve",
+                                            "k_project", "k_velocity", "k_damp", "k_support_velocity", "k_pass2",
+                                            "k_strain", "k_bond_end", "k_finish"};
+        return k < KCount ? names[k] : "unknown";
+    }
+    struct ProfRecord { uint32_t kernel, phase; hipEvent_t a, b; };
+    bool prof_ = false;
+    std::vector<hipEvent_t> pool_;
+    std::vector<ProfRecord> records_;
+    double kernel_seconds_[KCount] = {};
+    uint64_t kernel_launches_[KCount] = {};
+
+    void profBegin(uint32_t kernel, uint32_t phase) {
+        if (!prof_) return;
+        const size_t need = 2 * (records_.size() + 1);
+        while (pool_.size() < need) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            pool_.push_back(e);
+        }
+        ProfRecord r{kernel, phase, pool_[need - 2], pool_[need - 1]};
+        HIP_OK(hipEventRecord(r.a, stream_));
+        records_.push_back(r);
+    }
+    void profEnd() {
+        if (!prof_) return;
+        HIP_OK(hipEventRecord(records_.back().b, stream_));
+    }
+    void profCollect() {
+        for (const ProfRecord &r : records_) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+            kernel_seconds_[r.kernel] += double(ms) * 1e-3;
+            kernel_launches_[r.kernel] += 1;
+            run_status_.phase_seconds[r.phase] += double(ms) * 1e-3;
+        }
+        records_.clear();
+    }
+
+public:
+    void setProfiling(bool on) override { prof_ = on; }
+    uint32_t kernelStats(uint32_t k, std::string &name, double &seconds, uint64_t &launches) const override {
+        if (k < KCount) {
+            name = kernelName(k);
+            seconds = kernel_seconds_[k];
+            launches = kernel_launches_[k];
+        }
+        return KCount;
+    }
+    uint64_t profilingBatch(uint64_t batch) const { return prof_ ? std::min<uint64_t>(batch, 4) : batch; }
+
+private:
+#define BLX_LAUNCH(kernel, phase, ...) \
+    do {                                \
+        profBegin(kernel, phase);       \
+        __VA_ARGS__;                    \
+        profEnd();                      \
+    } while (0)
+
     template <bool DIRECT> void enqueueT() {
         const DevLattice<R> L = L_;
         const Step<R> S = S_;
         const uint32_t gn = gridFor(N_), gb = gridFor(B_);
         const bool sphere = S.sphere_enabled != 0;
-        hipLaunchKernelGGL((k_strain<R, DIRECT, true>), dim3(gn), dim3(kWG), 0, stream_, L);
-        hipLaunchKernelGGL((k_bond_start<R, DIRECT>), dim3(gb), dim3(kWG), 0, stream_, L);
+        BLX_LAUNCH(KStrainStart, 1, hipLaunchKernelGGL((k_strain<R, DIRECT, true>), dim3(gn), dim3(kWG), 0, stream_, L));
+        BLX_LAUNCH(KBondStart, 2, hipLaunchKernelGGL((k_bond_start<R, DIRECT>), dim3(gb), dim3(kWG), 0, stream_, L));
         const dim3 pg(node_tiles_);
+        profBegin(KPredict, 3);
         if (sphere && loads_set_) hipLaunchKernelGGL((k_predict<R, true, true, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
         else if (sphere) hipLaunchKernelGGL((k_predict<R, true, false, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
         else if (loads_set_) hipLaunchKernelGGL((k_predict<R, false, true, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
         else hipLaunchKernelGGL((k_predict<R, false, false, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
-        if (sphere) hipLaunchKernelGGL((k_pass<R, 1>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
+        profEnd();
+        if (sphere) BLX_LAUNCH(KPass1, 4, hipLaunchKernelGGL((k_pass<R, 1>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
         const uint2 *wg = wg_.as<uint2>();
         for (uint32_t it = 0; it < iterations_; ++it) {
             const int first = it == 0;
-            for (const auto &st : stages_) {
+            for (size_t k = 0; k < stages_.size(); ++k) {
+                const auto &st = stages_[k];
+                profBegin(KSolve, k < interior_stages_ ? 5 : 6);
                 if (plastic_)
                     hipLaunchKernelGGL((k_solve<R, DIRECT, true>), dim3(st.second), dim3(kWG), 0, stream_, L, S, wg, st.first, first);
                 else
                     hipLaunchKernelGGL((k_solve<R, DIRECT, false>), dim3(st.second), dim3(kWG), 0, stream_, L, S, wg, st.first, first);
+                profEnd();
             }
             if (S.plane_count > 0 && it + 1 < iterations_)
-                hipLaunchKernelGGL((k_project<R>), dim3(gn), dim3(kWG), 0, stream_, L, S);
+                BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), dim3(gn), dim3(kWG), 0, stream_, L, S));
         }
+        profBegin(KVelocity, 8);
         if (S.plane_count > 0) hipLaunchKernelGGL((k_velocity<R, true>), dim3(gn), dim3(kWG), 0, stream_, L, S);
         else hipLaunchKernelGGL((k_velocity<R, false>), dim3(gn), dim3(kWG), 0, stream_, L, S);
+        profEnd();
         if (S.damping > R(0)) {
             for (const auto &st : stages_)
-                hipLaunchKernelGGL((k_damp<R, DIRECT>), dim3(st.second), dim3(kWG), 0, stream_, L, S, wg, st.first);
-            hipLaunchKernelGGL((k_support_velocity<R>), dim3(gn), dim3(kWG), 0, stream_, L, S);
+                BLX_LAUNCH(KDamp, 9, hipLaunchKernelGGL((k_damp<R, DIRECT>), dim3(st.second), dim3(kWG), 0, stream_, L, S, wg, st.first));
+            BLX_LAUNCH(KSupportVelocity, 9, hipLaunchKernelGGL((k_support_velocity<R>), dim3(gn), dim3(kWG), 0, stream_, L, S));
         }
-        if (sphere) hipLaunchKernelGGL((k_pass<R, 2>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
-        hipLaunchKernelGGL((k_strain<R, DIRECT, false>), dim3(gn), dim3(kWG), 0, stream_, L);
+        if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
+        BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, false>), dim3(gn), dim3(kWG), 0, stream_, L));
+        profBegin(KBondEnd, 12);
         if (plastic_) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true>), dim3(gb), dim3(kWG), 0, stream_, L, S);
         else hipLaunchKernelGGL((k_bond_end<R, DIRECT, false>), dim3(gb), dim3(kWG), 0, stream_, L, S);
+        profEnd();
+        profBegin(KFinish, 13);
         if (loads_set_ && plastic_) hipLaunchKernelGGL((k_finish<R, true, true>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
         else if (loads_set_) hipLaunchKernelGGL((k_finish<R, true, false>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
         else if (plastic_) hipLaunchKernelGGL((k_finish<R, false, true>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
         else hipLaunchKernelGGL((k_finish<R, false, false>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
+        profEnd();
         HIP_OK(hipGetLastError());
     }
 
@@ -853,6 +955,55 @@ int blx_backend_take_frame(blx_backend *b, uint32_t k, uint64_t *step, float *u,
 int blx_backend_clear_frames(blx_backend *b) { return guard([&] { IMPL(b)->clearFrames(); }); }
 int blx_backend_fragment_labels(blx_backend *b, uint32_t *label, uint32_t *count) {
     return guard([&] { *count = IMPL(b)->labels(label); });
+}
+
+int blx_hbm_copy_probe(int device, uint64_t bytes, uint32_t repeats, double *seconds_per_copy) {
+    return guard([&] {
+        if (deviceCount() <= device || device < 0) throw Runtime("no HIP device for the copy probe");
+        if (bytes < 16 || repeats == 0) throw Invalid("copy probe needs at least 16 bytes and one repeat");
+        HIP_OK(hipSetDevice(device));
+        const size_t n = size_t(bytes / 16);
+        DevBuf src, dst;
+        src.alloc(n * 16);
+        dst.alloc(n * 16);
+        HIP_OK(hipMemset(src.p, 1, n * 16));
+        hipDeviceProp_t prop{};
+        HIP_OK(hipGetDeviceProperties(&prop, device));
+        const size_t grid = std::min<size_t>((n + kWG - 1) / kWG, size_t(prop.multiProcessorCount) * 32);
+        hipEvent_t a, b;
+        HIP_OK(hipEventCreate(&a));
+        HIP_OK(hipEventCreate(&b));
+        hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(kWG), 0, nullptr, src.as<uint4>(), dst.as<uint4>(), n);
+        HIP_OK(hipEventRecord(a, nullptr));
+        for (uint32_t r = 0; r < repeats; ++r)
+            hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(kWG), 0, nullptr, src.as<uint4>(), dst.as<uint4>(), n);
+        HIP_OK(hipEventRecord(b, nullptr));
+        HIP_OK(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *seconds_per_copy = double(ms) * 1e-3 / repeats;
+    });
+}
+
+int blx_backend_set_profiling(blx_backend *b, int on) { return guard([&] { IMPL(b)->setProfiling(on != 0); }); }
+int blx_backend_kernel_stats(const blx_backend *b, uint32_t index, char *name, size_t cap, double *seconds,
+                             uint64_t *launches, uint32_t *count) {
+    return guard([&] {
+        std::string n;
+        double s = 0.0;
+        uint64_t l = 0;
+        *count = reinterpret_cast<const Handle *>(b)->impl->kernelStats(index, n, s, l);
+        if (index < *count) {
+            if (name && cap) {
+                std::strncpy(name, n.c_str(), cap - 1);
+                name[cap - 1] = 0;
+            }
+            if (seconds) *seconds = s;
+            if (launches) *launches = l;
+        }
+    });
 }
 
 } // extern "C"

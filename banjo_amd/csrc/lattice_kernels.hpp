@@ -968,23 +968,33 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step
 }
 
 // ---------------------------------------------------------------------------
-// Fragments: integer union-find over the live bonds. Roots are linked larger
-// index under smaller with a compare-and-swap, so every root ends as its
-// component's smallest node: the tie-break of the canonical order.
+// Fragments: integer union-find over the live bonds, then the canonical
+// relabel of findConnectedComponents (ConnectedComponents.cpp:77-85). Roots are
+// linked larger index under smaller with a compare-and-swap, so every root
+// ends as its component's smallest node: the tie-break of the canonical order.
+// Parent reads are agent-scope atomic loads, never a stale per-CU cache line.
+
+__device__ __forceinline__ u32 loadParent(u32 *parent, u32 x) {
+    return __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ u32 findRoot(u32 *parent, u32 x) {
-    u32 p = parent[x];
+    u32 p = loadParent(parent, x);
     while (p != x) {
-        const u32 g = parent[p];
-        if (g != p) parent[x] = g;  // path halving; benign races only shorten paths
+        const u32 g = loadParent(parent, p);
+        // Path halving: g is an ancestor of x, so the store only shortens the path.
+        if (g != p) __hip_atomic_store(&parent[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         x = p;
         p = g;
     }
     return x;
 }
 
-__global__ void k_cc_init(u32 *parent, u32 n) {
-    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) parent[i] = i;
+__global__ void k_cc_init(u32 *parent, u32 *size, u32 n) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        parent[i] = i;
+        size[i] = 0;
+    }
 }
 
 template <class R> __global__ void k_cc_hook(DevLattice<R> L, u32 *parent) {
@@ -1004,16 +1014,37 @@ template <class R> __global__ void k_cc_hook(DevLattice<R> L, u32 *parent) {
     }
 }
 
-__global__ void k_cc_compress(u32 *parent, u32 *size, u32 n) {
+// Roots go to their own array: a root stored into parent[] could be overwritten
+// by another thread's path halving of the same node.
+__global__ void k_cc_roots(u32 *parent, u32 *root, u32 *size, u32 n) {
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const u32 r = findRoot(parent, i);
-        parent[i] = r;
+        root[i] = r;
         atomicAdd(&size[r], 1U);
     }
 }
 
-__global__ void k_cc_label(const u32 *parent, const u32 *rank, u32 *label, u32 n) {
-    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) label[i] = rank[parent[i]];
+// One sort key per component: (n - size) << 32 | root. Ascending keys are the
+// canonical order, size descending then smallest node; the append order of
+// the keys does not matter because the sort is on the whole key.
+__global__ void k_cc_keys(const u32 *root, const u32 *size, u64 *keys, u32 *count, u32 n) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (root[i] == i) keys[atomicAdd(count, 1U)] = (u64(n - size[i]) << 32) | u64(i);
+}
+
+__global__ void k_cc_rank(const u64 *sorted, u32 *rank, u32 c) {
+    for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < c; k += gridDim.x * blockDim.x)
+        rank[u32(sorted[k] & 0xffffffffULL)] = k;
+}
+
+__global__ void k_cc_label(const u32 *root, const u32 *rank, u32 *label, u32 n) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) label[i] = rank[root[i]];
+}
+
+// A plain streaming copy, 16 bytes per lane per iteration: the known-size
+// workload rocprofv3's FETCH_SIZE and WRITE_SIZE counters are scaled against.
+__global__ __launch_bounds__(kWG) void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * kWG + threadIdx.x; i < n; i += size_t(gridDim.x) * kWG) dst[i] = src[i];
 }
 
 } // namespace blx

@@ -264,3 +264,36 @@ def small_impact(nx=24, ny=12, nz=24, ball_radius=0.05, ball_mass=2.0, ball_spee
     """A small block of the config-2 family for parity tests."""
     return impact(nx, ny, nz, ball_radius=ball_radius, ball_mass=ball_mass, ball_speed=ball_speed,
                   substeps=substeps, blocks=blocks, seed=seed, name=f"impact_{nx}x{ny}x{nz}")
+
+
+def ledge_plane(y, node_radius, footprints, restitution=0.1, static_friction=0.6, dynamic_friction=0.5):
+    """A reach-capped support plane with finite footprints (TileImpactScene.cpp:322-332)."""
+    p = ground_plane(y, node_radius, restitution, static_friction, dynamic_friction)
+    p.reach_capped = 1
+    p.footprint_count = len(footprints)
+    for k, (ct, cb, ht, hb) in enumerate(footprints):
+        p.footprints[k].center_t, p.footprints[k].center_b = ct, cb
+        p.footprints[k].half_t, p.footprints[k].half_b = ht, hb
+    return p
+
+
+def bridge_impact(nx=20, ny=6, nz=10, cell=0.01, substeps=150, blocks=1, seed=971, ball_speed=6.0):
+    """A block resting on two ledges under its x ends, struck at mid-span; ground below."""
+    sc = impact(nx, ny, nz, cell=cell, ball_radius=0.04, ball_mass=1.0, ball_speed=ball_speed, substeps=substeps,
+                blocks=blocks, seed=seed, name=f"bridge_{nx}x{ny}x{nz}")
+    bottom = sc.asset.reference_position[:, 1].min() - 0.5 * cell
+    half_x = 0.5 * nx * cell
+    ledge = ledge_plane(bottom, 0.5 * cell, [(-half_x, 0.0, 3 * cell, INF), (half_x, 0.0, 3 * cell, INF)])
+    sc.settings.plane_count = 2
+    sc.settings.planes[0] = ledge
+    sc.settings.planes[1] = ground_plane(bottom - 0.05, 0.5 * cell)
+    return sc
+
+
+def plastic_impact(nx=16, ny=8, nz=16, substeps=120, blocks=1, seed=975):
+    """The config-5 material on an impact block: yield 0.4 %, hardening 0.1, Weibull(k=4, 2 %)."""
+    sc = impact(nx, ny, nz, ball_radius=0.04, ball_mass=1.5, ball_speed=6.0, substeps=substeps, blocks=blocks,
+                seed=seed, weibull_k=4.0, weibull_scale=0.02, name=f"plastic_{nx}x{ny}x{nz}")
+    sc.settings.plastic_yield_stretch = 0.004
+    sc.settings.plastic_hardening = 0.1
+    return sc

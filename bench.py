@@ -1,0 +1,236 @@
+"""Benchmark of the MI355X lattice step: bond-solves/s on the 1M-node impact block.
+
+    python bench.py [--gpus 1] [--steps 120] [--warmup 12]
+
+Workload: a simple-cubic block of 128 (x) x 64 (y, vertical) x 128 (z) nodes,
+26-neighbour bonds (13,338,492), per-bond compliance log-uniform in
+[1e-7, 1e-5] and Weibull(k=6, 1.5%) break strain from seed 971, struck on its
+top-face centre by a 10 kg, 0.2 m sphere at 8 m/s, on a ground plane. One step
+is 1/240 s of simulated time: 10 substeps of dt = 1/2400 s; --steps 120 is the
+whole 0.5 s run. Float arithmetic (the reference's fast displacement path).
+
+A bond-solve is one bond carried through one whole substep; the rate is
+bonds x substeps / seconds, the reference's bond_updates_per_s
+(src/fastlattice/TileImpactScene.cpp:867-868). Inputs are resident on the
+device before the timed region; the timed region is the backend's run() call
+bracketed by device synchronisation.
+
+Printed JSON carries, besides the rate:
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                mean HIP-event duration (a second, profiled pass) against
+                MI355X's 8 TB/s HBM peak;
+  step_roofline the whole substep at the algorithmic bytes per bond-solve;
+  cpu_baseline  the reference's threaded CPU lane (oracle/_ref) on this host's
+                cores over a bounded window of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+SUBSTEPS_PER_STEP = 10
+METRIC = "bond-solves/s @1M particles (%HBM roof); trials/s + p99, batched vs plain"
+
+
+def algorithmic_bytes(N, B, stages, sphere=True, plastic=False, iterations=1):
+    """Algorithmic HBM bytes per kernel launch at fp32: every array of the API data
+    model read or written once per pass, node gathers counted once per node per pass.
+
+    Per bond / per node, by pass:
+      predict          -         / u r+w 24, v r 12, u_prev w 12 (+ x0 12 with a sphere)
+      sweep (x iter)   37 (+4+4) / u r+w 24, inv_mass 4
+      velocity         -         / u, u_prev r 24, v w 12, flags 2
+      end strain       42        / u 12, rinv 36, unmeasured 12, strain w 24
+      sample + break   88 (+16)  / u 12, strain 24, valid 1
+    Summed: 167 + 223 N/B per bond-solve (the same count for one iteration).
+    """
+    per = {}
+    per["k_predict"] = N * (48 + (12 if sphere else 0))
+    sweep_bond = 37 + (4 if iterations > 1 else 0) + (4 if plastic else 0)
+    per["k_solve"] = (B * sweep_bond + N * 28) / max(stages, 1)
+    per["k_velocity"] = N * 38
+    per["k_strain"] = B * 42 + N * 84
+    per["k_bond_end"] = B * (88 + (16 if plastic else 0)) + N * 37
+    step = B * (167 + (4 if plastic else 0) + 16 * plastic) + N * (223 + (12 if sphere else 0))
+    return per, step
+
+
+def make_workload(args):
+    from banjo_amd import scenes
+    sc = scenes.impact(128, 64, 128, substeps=args.steps * SUBSTEPS_PER_STEP, blocks=args.blocks,
+                       name="impact_128x64x128")
+    return sc
+
+
+def gpu_leg(args, sc):
+    from banjo_amd import abi
+    from banjo_amd.lattice import control, product
+    from banjo_amd.runner import prepare
+
+    hip = product()
+    if not hip.lib.blx_available():
+        raise SystemExit("bench.py: no gfx950 device visible to the HIP lattice library")
+    sched, state = prepare(hip, sc)
+    be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+    substeps = args.steps * SUBSTEPS_PER_STEP
+    if args.warmup:
+        be.upload(state, sc.settings, sc.sphere)
+        be.run(control(args.warmup * SUBSTEPS_PER_STEP))
+    be.upload(state, sc.settings, sc.sphere)
+    import ctypes as C
+    t0 = time.perf_counter()
+    st = be.run(control(substeps))
+    seconds = time.perf_counter() - t0
+    done = st.total_steps
+    result = dict(seconds=seconds, substeps=done, broken=st.broken_bonds, kernel_seconds=st.kernel_seconds,
+                  launches=st.launches, contacts=st.contact.impulse_contacts,
+                  overflow=st.contact.candidate_overflow)
+    # Profiled pass: per-launch HIP events on the backend's own stream.
+    prof_substeps = min(substeps, args.profile_substeps)
+    be.upload(state, sc.settings, sc.sphere)
+    hip.check(hip.lib.blx_backend_set_profiling(be.h, C.c_int(1)), "set_profiling")
+    be.run(control(prof_substeps))
+    kernels = {}
+    count = C.c_uint32()
+    hip.check(hip.lib.blx_backend_kernel_stats(be.h, C.c_uint32(0), None, C.c_size_t(0), None, None,
+                                                C.byref(count)), "kernel_stats")
+    for k in range(count.value):
+        name = C.create_string_buffer(64)
+        secs = C.c_double()
+        launches = C.c_uint64()
+        hip.check(hip.lib.blx_backend_kernel_stats(be.h, C.c_uint32(k), name, C.c_size_t(64), C.byref(secs),
+                                                    C.byref(launches), C.byref(count)), "kernel_stats")
+        if launches.value:
+            kernels[name.value.decode()] = dict(seconds=secs.value, launches=launches.value)
+    be.close()
+    ranges = sched.range_end.astype(int) - sched.range_begin.astype(int)
+    cc = max(sched.color_count, 1)
+    # Stages per iteration: interior colours plus even/odd boundary colours that are non-empty.
+    r = ranges.reshape(sched.block_count, 2, cc)
+    stages = int((r[:, 0, :].sum(0) > 0).sum() + (r[0::2, 1, :].sum(0) > 0).sum() + (r[1::2, 1, :].sum(0) > 0).sum())
+    result["kernels"] = kernels
+    result["stages"] = stages
+    result["prof_substeps"] = prof_substeps
+    return result
+
+
+def cpu_leg(args, sc):
+    """The reference's threaded CPU lane over a bounded window of the same workload."""
+    from banjo_amd import abi
+    from banjo_amd.lattice import LatticeLibrary, control
+    from banjo_amd.runner import prepare
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libbanjo_ref.so")
+    orc_path = os.path.join(ROOT, "oracle", "build", "liboracle.so")
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if os.path.exists(ref_path):
+        lib, kind, extra, cores, window = LatticeLibrary(ref_path, "ref_", "ref"), "reference", (1, threads), threads, args.cpu_substeps
+    elif os.path.exists(orc_path):
+        lib, kind, extra, cores, window = LatticeLibrary(orc_path, "orc_", "oracle"), "port", (), 1, max(1, args.cpu_substeps // 5)
+    else:
+        return None
+    sched, state = prepare(lib, sc)
+    be = lib.backend(abi.PRECISION_FLOAT, sched, *extra)
+    be.upload(state, sc.settings, sc.sphere)
+    t0 = time.perf_counter()
+    st = be.run(control(window))
+    seconds = time.perf_counter() - t0
+    be.close()
+    value = sc.bond_count * st.total_steps / seconds
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": value, "unit": "bond-solves/s", "cores": cores, "kind": kind,
+            "sample": f"first {st.total_steps} substeps of the same 128x64x128 impact from t=0, float, "
+                      f"{'ParallelCpuLatticeBackend' if kind == 'reference' else 'serial oracle'} with {cores} "
+                      f"thread(s); {seconds:.2f} s; host {cpu_model}".strip()}
+
+
+def copy_probe(nbytes, repeats):
+    import ctypes as C
+    from banjo_amd.lattice import product
+    hip = product()
+    secs = C.c_double()
+    hip.check(hip.lib.blx_hbm_copy_probe(C.c_int(0), C.c_uint64(nbytes), C.c_uint32(repeats), C.byref(secs)),
+              "hbm_copy_probe")
+    return {"copy_bytes": nbytes, "read_bytes": nbytes, "write_bytes": nbytes, "seconds": secs.value,
+            "gbps_read_plus_write": 2 * nbytes / secs.value / 1e9}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=12)
+    ap.add_argument("--blocks", type=int, default=1)
+    ap.add_argument("--profile-substeps", type=int, default=1200)
+    ap.add_argument("--cpu-substeps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run the HBM copy probe of this size (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_probe_mib:
+        print(json.dumps(copy_probe(args.copy_probe_mib << 20, 20)))
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the lattice step would shard by z-slabs across GPUs, which is designed but not built)",
+              file=sys.stderr)
+        raise SystemExit(2)
+    t_build = time.perf_counter()
+    sc = make_workload(args)
+    N, B = sc.node_count, sc.bond_count
+    print(f"# workload {sc.name}: N={N} B={B} built in {time.perf_counter() - t_build:.1f} s", file=sys.stderr,
+          flush=True)
+    g = gpu_leg(args, sc)
+    print(f"# gpu: {g['substeps']} substeps in {g['seconds']:.3f} s, broken {g['broken']}", file=sys.stderr, flush=True)
+    rate = B * g["substeps"] / g["seconds"]
+    per, step_bytes = algorithmic_bytes(N, B, g["stages"], sphere=True)
+    kernels = g["kernels"]
+    dominant = max(kernels, key=lambda k: kernels[k]["seconds"])
+    avg = kernels[dominant]["seconds"] / kernels[dominant]["launches"]
+    bytes_launch = per.get(dominant)
+    roofline = None
+    if bytes_launch is not None:
+        achieved = bytes_launch / avg
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                    "bytes_per_launch": bytes_launch, "mean_launch_us": avg * 1e6}
+    per_bond_solve = step_bytes / B
+    step_roofline = {"bytes_per_bond_solve": per_bond_solve, "achieved": per_bond_solve * rate / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": per_bond_solve * rate / HBM_PEAK}
+    kernel_table = {k: {"mean_us": v["seconds"] / v["launches"] * 1e6, "launches": v["launches"],
+                        "share": v["seconds"] / sum(x["seconds"] for x in kernels.values())}
+                    for k, v in kernels.items()}
+    probe = copy_probe(1 << 30, 10)
+    cpu = None if args.no_cpu else cpu_leg(args, sc)
+    out = {
+        "metric": METRIC, "value": rate, "unit": "bond-solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": g["seconds"] / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "impact 128x64x128 (x, y vertical, z), 10 kg r=0.2 m sphere at 8 m/s, "
+                               "dt 1/2400 s, 10 substeps per step", "nodes": N, "bonds": B,
+                   "substeps": g["substeps"], "blocks": args.blocks, "colour_stages": g["stages"],
+                   "broken_bonds": g["broken"], "striker_contacts": g["contacts"],
+                   "candidate_overflow": g["overflow"]},
+        "roofline": roofline, "step_roofline": step_roofline, "kernels": kernel_table,
+        "copy_probe_gbps": probe["gbps_read_plus_write"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

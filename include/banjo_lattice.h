@@ -320,6 +320,23 @@ int blx_backend_clear_frames(blx_backend *backend);
 int blx_backend_fragment_labels(blx_backend *backend, uint32_t *label,
                                 uint32_t *component_count);
 
+/* ---- measurement (no counterpart in the reference's API) ---------------- */
+
+/* Per-launch HIP-event timing of every kernel (off by default: it shortens
+ * device batches to 4 substeps). With it on, run() also fills
+ * blx_run_status.phase_seconds with the reference's phase ids
+ * (FastLattice.cpp:22-26, RunStatus::phase_seconds at FastLattice.hpp:226-228). */
+int blx_backend_set_profiling(blx_backend *backend, int on);
+/* Accumulated event time and launch count of kernel kind `index`; *count is
+ * the number of kinds (call with index 0 to learn it). */
+int blx_backend_kernel_stats(const blx_backend *backend, uint32_t index, char *name, size_t capacity,
+                             double *seconds, uint64_t *launches, uint32_t *count);
+
+/* A streaming device-to-device copy of `bytes` (read once, written once),
+ * repeated; the mean seconds per copy. Used to calibrate the hardware
+ * FETCH_SIZE / WRITE_SIZE counters and as the achievable-bandwidth yardstick. */
+int blx_hbm_copy_probe(int device, uint64_t bytes, uint32_t repeats, double *seconds_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
