@@ -108,6 +108,9 @@ public:
     Backend(int device, Schedule s) : device_(device), sch_(std::move(s)) {
         HIP_OK(hipSetDevice(device_));
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&ev_bond_end_, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&ev_rinv_, hipEventDisableTiming));
         HIP_OK(hipEventCreate(&ev0_));
         HIP_OK(hipEventCreate(&ev1_));
         hipDeviceProp_t prop{};
@@ -121,6 +124,10 @@ public:
         if (ev0_) (void)hipEventDestroy(ev0_);
         if (ev1_) (void)hipEventDestroy(ev1_);
         for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+        if (side_) (void)hipStreamSynchronize(side_);
+        if (ev_bond_end_) (void)hipEventDestroy(ev_bond_end_);
+        if (ev_rinv_) (void)hipEventDestroy(ev_rinv_);
+        if (side_) (void)hipStreamDestroy(side_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
 
@@ -156,16 +163,18 @@ public:
         toDevice(up4_, up4);
         toDevice(v4_, v4);
         {
-            std::vector<uint32_t> other(P);
+            std::vector<uint32_t> other(P), bond(P);
             std::vector<R> rx(P), ry(P), rz(P), w(P);
             for (size_t k = 0; k < P; ++k) {
                 other[k] = st.nbr_bond[k] == BLX_NO_BOND ? kNone : st.nbr_other[k];
+                bond[k] = st.nbr_bond[k] == BLX_NO_BOND ? kNone : st.nbr_bond[k];
                 rx[k] = R(st.nbr_rest[3 * k]);
                 ry[k] = R(st.nbr_rest[3 * k + 1]);
                 rz[k] = R(st.nbr_rest[3 * k + 2]);
                 w[k] = R(st.nbr_weight[k]);
             }
             toDevice(nbr_other_, other);
+            toDevice(nbr_bond_, bond);
             toDevice(nbr_rx_, rx);
             toDevice(nbr_ry_, ry);
             toDevice(nbr_rz_, rz);
@@ -214,8 +223,6 @@ public:
         zero<R4<R>>(sa_, N);
         zero<R4<R>>(sb_, N);
         zero<uint8_t>(valid_, N);
-        dirty_.alloc(N);
-        HIP_OK(hipMemset(dirty_.p, 1, N));
         zero<R>(approach_, 2 * size_t(N));
         zero<uint8_t>(engaged_, 2 * size_t(N));
         zero<uint8_t>(cand_, N);
@@ -224,11 +231,24 @@ public:
         bitmap_words_ = (bond_tiles_ + 63) / 64;
         zero<uint32_t>(tile_cand_, node_tiles_);
         zero<uint64_t>(tile_bits_, bitmap_words_);
+        zero<uint32_t>(tile_events_, bond_tiles_);
+        zero<uint32_t>(ev_j_, size_t(bond_tiles_) * kWG);
+        zero<float>(ev_d_, size_t(bond_tiles_) * kWG);
+        zero<R>(ev_e_, size_t(bond_tiles_) * kWG);
+        zero<uint32_t>(flag_list_, bond_tiles_);
         zero<double>(tile_plastic_, bond_tiles_);
         zero<double>(load_part_, 13 * size_t(node_tiles_));
         zero<R4<R>>(fext_, N);
         zero<double>(freq_, 3 * size_t(N));
-        zero<uint32_t>(cand_all_, N);
+        zero<uint32_t>(cand_all_, std::max<size_t>(N, size_t(sch_.blocks) * kCap));
+        zero<uint32_t>(rank_stamp_, N);
+        zero<uint32_t>(cand_tile_, size_t(node_tiles_) * kWG);
+        {
+            std::vector<uint32_t> all(N);
+            for (uint32_t i = 0; i < N; ++i) all[i] = i;
+            toDevice(dirty_list_, all);  // upload marks every node dirty
+        }
+        zero<uint32_t>(dirty_stamp_, N);
         zero<uint32_t>(cand_lo_, sch_.blocks);
         zero<uint32_t>(cand_cnt_, sch_.blocks);
         toDevice(nbb_, sch_.nbb);
@@ -236,6 +256,9 @@ public:
         DevCtl c{};
         c.first_failure_step = c.last_failure_step = c.last_energy_gain_step = ~0ULL;
         c.dirty_start = 1;
+        c.dirty_count = N;
+        c.rank_slot = 1;            // booked by the first substep's k_finish
+        c.rinv_stamp = 0xffffffffU;
         c.flat_fraction = 1.0e-3;
         c.calm_margin = 0.5;
         ctl_host_ = c;
@@ -251,6 +274,10 @@ public:
         run_status_ = blx_run_status{};
         run_status_.first_failure_step = run_status_.last_failure_step = run_status_.last_energy_gain_step = ~0ULL;
         bind();
+        // Every node starts dirty: its inverse rest covariance before the first substep.
+        hipLaunchKernelGGL((k_rinv<R>), dim3(gridFor(N)), dim3(kWG), 0, stream_, L_);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(stream_));
         uploaded_ = true;
         HIP_OK(hipDeviceSynchronize());
     }
@@ -498,7 +525,8 @@ private:
 
     int device_;
     Schedule sch_;
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr, side_ = nullptr;
+    hipEvent_t ev_bond_end_ = nullptr, ev_rinv_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t cus_ = 256;
     uint32_t N_ = 0, B_ = 0, D_ = 0, iterations_ = 1;
@@ -516,7 +544,8 @@ private:
     std::vector<std::pair<uint32_t, uint32_t>> stages_; // (first workgroup, workgroup count)
     size_t interior_stages_ = 0;
     DevBuf wg_;
-    DevBuf x04_, u4_, up4_, v4_, rinv_, unm_, sa_, sb_, valid_, dirty_, approach_, engaged_, cand_;
+    DevBuf x04_, u4_, up4_, v4_, rinv_, unm_, sa_, sb_, valid_, approach_, engaged_, cand_;
+    DevBuf rank_stamp_, dirty_list_, dirty_stamp_, cand_tile_, nbr_bond_, tile_events_, ev_j_, ev_d_, ev_e_, flag_list_;
     DevBuf nbr_other_, nbr_rx_, nbr_ry_, nbr_rz_, nbr_w_, nbr_alive_;
     DevBuf rec_, alive_, lambda_, pext_, pstr_, thr_, pvd_, mode_, slots_, stamp_, removed_;
     DevBuf tile_cand_, tile_bits_, tile_plastic_, load_part_, fext_, freq_, cand_all_, cand_lo_, cand_cnt_, nbb_;
@@ -546,6 +575,8 @@ private:
         if (st.node_count != sch_.N || st.bond_count != sch_.B) throw Invalid("state does not match the schedule");
         if (size_t(st.max_degree) * st.node_count > 0xffffffffULL) throw Invalid("neighbour slots exceed 32-bit indices");
         if (!(s.dt > 0.0) || !std::isfinite(s.dt)) throw Invalid("the substep must be finite and positive");
+        if (s.sphere_enabled && (st.node_count + kWG - 1) / kWG > kMaxNodeTiles)
+            throw Invalid("the striker pass holds at most 4,194,304 nodes");
     }
 
     static Step<R> convertStep(const blx_settings &s) {
@@ -631,7 +662,6 @@ private:
         L.sa = sa_.as<R4<R>>();
         L.sb = sb_.as<R4<R>>();
         L.valid = valid_.as<uint8_t>();
-        L.dirty = dirty_.as<uint8_t>();
         L.approach = approach_.as<R>();
         L.engaged = engaged_.as<uint8_t>();
         L.cand = cand_.as<uint8_t>();
@@ -654,11 +684,21 @@ private:
         L.removed = removed_.as<R>();
         L.tile_cand = tile_cand_.as<uint32_t>();
         L.tile_bits = tile_bits_.as<uint64_t>();
+        L.nbr_bond = nbr_bond_.as<uint32_t>();
+        L.tile_events = tile_events_.as<uint32_t>();
+        L.ev_j = ev_j_.as<uint32_t>();
+        L.ev_d = ev_d_.as<float>();
+        L.ev_e = ev_e_.as<R>();
+        L.flag_list = flag_list_.as<uint32_t>();
         L.tile_plastic = tile_plastic_.as<double>();
         L.load_part = load_part_.as<double>();
         L.fext = fext_.as<R4<R>>();
         L.freq = freq_.as<double>();
         L.cand_all = cand_all_.as<uint32_t>();
+        L.cand_tile = cand_tile_.as<uint32_t>();
+        L.dirty_list = dirty_list_.as<uint32_t>();
+        L.dirty_stamp = dirty_stamp_.as<uint32_t>();
+        L.rank_stamp = rank_stamp_.as<uint32_t>();
         L.cand_lo = cand_lo_.as<uint32_t>();
         L.cand_cnt = cand_cnt_.as<uint32_t>();
         L.nbb = nbb_.as<uint32_t>();
@@ -717,11 +757,11 @@ private:
     // Kernel kinds for the optional per-launch event timing, each with the
     // reference phase id it belongs to (FastLattice.cpp:22-26).
     enum Kernel : uint32_t { KStrainStart, KBondStart, KPredict, KPass1, KSolve, KProject, KVelocity, KDamp,
-                             KSupportVelocity, KPass2, KStrain, KBondEnd, KFinish, KCount };
+                             KSupportVelocity, KPass2, KStrain, KBondEnd, KFinish, KRinv, KCount };
     static const char *kernelName(uint32_t k) {
         static const char *names[KCount] = {"k_strain_start", "k_bond_start", "k_predict", "k_pass1", "k_solve",
                                             "k_project", "k_velocity", "k_damp", "k_support_velocity", "k_pass2",
-                                            "k_strain", "k_bond_end", "k_finish"};
+                                            "k_strain", "k_bond_end", "k_finish", "k_rinv"};
         return k < KCount ? names[k] : "unknown";
     }
     struct ProfRecord { uint32_t kernel, phase; hipEvent_t a, b; };
@@ -731,8 +771,9 @@ private:
     double kernel_seconds_[KCount] = {};
     uint64_t kernel_launches_[KCount] = {};
 
-    void profBegin(uint32_t kernel, uint32_t phase) {
+    void profBegin(uint32_t kernel, uint32_t phase, hipStream_t on = nullptr) {
         if (!prof_) return;
+        if (!on) on = stream_;
         const size_t need = 2 * (records_.size() + 1);
         while (pool_.size() < need) {
             hipEvent_t e;
@@ -740,12 +781,12 @@ private:
             pool_.push_back(e);
         }
         ProfRecord r{kernel, phase, pool_[need - 2], pool_[need - 1]};
-        HIP_OK(hipEventRecord(r.a, stream_));
+        HIP_OK(hipEventRecord(r.a, on));
         records_.push_back(r);
     }
-    void profEnd() {
+    void profEnd(hipStream_t on = nullptr) {
         if (!prof_) return;
-        HIP_OK(hipEventRecord(records_.back().b, stream_));
+        HIP_OK(hipEventRecord(records_.back().b, on ? on : stream_));
     }
     void profCollect() {
         for (const ProfRecord &r : records_) {
@@ -783,9 +824,10 @@ private:
         const Step<R> S = S_;
         const uint32_t gn = gridFor(N_), gb = gridFor(B_);
         const bool sphere = S.sphere_enabled != 0;
+        const uint32_t gnt = xcdGrid(node_tiles_), gbt = xcdGrid(bond_tiles_);
         BLX_LAUNCH(KStrainStart, 1, hipLaunchKernelGGL((k_strain<R, DIRECT, true>), dim3(gn), dim3(kWG), 0, stream_, L));
-        BLX_LAUNCH(KBondStart, 2, hipLaunchKernelGGL((k_bond_start<R, DIRECT>), dim3(gb), dim3(kWG), 0, stream_, L));
-        const dim3 pg(node_tiles_);
+        BLX_LAUNCH(KBondStart, 2, hipLaunchKernelGGL((k_bond_start<R, DIRECT>), dim3(gn), dim3(kWG), 0, stream_, L));
+        const dim3 pg(gnt);
         profBegin(KPredict, 3);
         if (sphere && loads_set_) hipLaunchKernelGGL((k_predict<R, true, true, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
         else if (sphere) hipLaunchKernelGGL((k_predict<R, true, false, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
@@ -800,17 +842,19 @@ private:
                 const auto &st = stages_[k];
                 profBegin(KSolve, k < interior_stages_ ? 5 : 6);
                 if (plastic_)
-                    hipLaunchKernelGGL((k_solve<R, DIRECT, true>), dim3(st.second), dim3(kWG), 0, stream_, L, S, wg, st.first, first);
+                    hipLaunchKernelGGL((k_solve<R, DIRECT, true>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L, S,
+                                       wg, st.first, st.second, first);
                 else
-                    hipLaunchKernelGGL((k_solve<R, DIRECT, false>), dim3(st.second), dim3(kWG), 0, stream_, L, S, wg, st.first, first);
+                    hipLaunchKernelGGL((k_solve<R, DIRECT, false>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L, S,
+                                       wg, st.first, st.second, first);
                 profEnd();
             }
             if (S.plane_count > 0 && it + 1 < iterations_)
                 BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), dim3(gn), dim3(kWG), 0, stream_, L, S));
         }
         profBegin(KVelocity, 8);
-        if (S.plane_count > 0) hipLaunchKernelGGL((k_velocity<R, true>), dim3(gn), dim3(kWG), 0, stream_, L, S);
-        else hipLaunchKernelGGL((k_velocity<R, false>), dim3(gn), dim3(kWG), 0, stream_, L, S);
+        if (S.plane_count > 0) hipLaunchKernelGGL((k_velocity<R, true>), dim3(gnt), dim3(kWG), 0, stream_, L, S);
+        else hipLaunchKernelGGL((k_velocity<R, false>), dim3(gnt), dim3(kWG), 0, stream_, L, S);
         profEnd();
         if (S.damping > R(0)) {
             for (const auto &st : stages_)
@@ -818,17 +862,25 @@ private:
             BLX_LAUNCH(KSupportVelocity, 9, hipLaunchKernelGGL((k_support_velocity<R>), dim3(gn), dim3(kWG), 0, stream_, L, S));
         }
         if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
-        BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, false>), dim3(gn), dim3(kWG), 0, stream_, L));
+        BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, false>), dim3(gnt), dim3(kWG), 0, stream_, L));
         profBegin(KBondEnd, 12);
-        if (plastic_) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true>), dim3(gb), dim3(kWG), 0, stream_, L, S);
-        else hipLaunchKernelGGL((k_bond_end<R, DIRECT, false>), dim3(gb), dim3(kWG), 0, stream_, L, S);
+        if (plastic_) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true>), dim3(gbt), dim3(kWG), 0, stream_, L, S);
+        else hipLaunchKernelGGL((k_bond_end<R, DIRECT, false>), dim3(gbt), dim3(kWG), 0, stream_, L, S);
         profEnd();
+        // The dirty nodes' inverse rest covariances overlap the ordered folds.
+        HIP_OK(hipEventRecord(ev_bond_end_, stream_));
+        HIP_OK(hipStreamWaitEvent(side_, ev_bond_end_, 0));
+        profBegin(KRinv, 1, side_);
+        hipLaunchKernelGGL((k_rinv<R>), dim3(gridFor(N_)), dim3(kWG), 0, side_, L);
+        profEnd(side_);
+        HIP_OK(hipEventRecord(ev_rinv_, side_));
         profBegin(KFinish, 13);
         if (loads_set_ && plastic_) hipLaunchKernelGGL((k_finish<R, true, true>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
         else if (loads_set_) hipLaunchKernelGGL((k_finish<R, true, false>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
         else if (plastic_) hipLaunchKernelGGL((k_finish<R, false, true>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
         else hipLaunchKernelGGL((k_finish<R, false, false>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
         profEnd();
+        HIP_OK(hipStreamWaitEvent(stream_, ev_rinv_, 0));
         HIP_OK(hipGetLastError());
     }
 

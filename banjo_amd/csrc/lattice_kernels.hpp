@@ -34,7 +34,6 @@ namespace blx {
 
 constexpr int kWG = 256;
 constexpr u32 kCap = 1024;        // striker candidates per block (LatticePhysics.hpp:105)
-constexpr u32 kStage = 512;       // candidates staged in LDS at a time
 constexpr int kFinishThreads = 1024;
 
 // Device-resident run state. Everything that changes from one substep to the
@@ -63,12 +62,27 @@ struct DevCtl {
     u32 pstretch_bits;
     u32 rank_deficient;
     u32 exit_reason;
+    u32 dirty_count;     // entries in dirty_list
+    u32 rank_slot;       // parity slot k_rinv counts into (set by k_bond_end)
+    u32 rank_pending[2]; // rank-deficient recomputations not yet booked
+    u32 rinv_stamp;      // break round the dirty list belongs to
     double plastic_work;
     u64 first_failure_count;
     // RunControl of the current run() call.
     u64 quiet, min_steps, no_failure, energy_flat, calm;
     double flat_fraction, calm_margin, removable;
 };
+
+// Workgroup w runs on XCD w % 8, and every XCD has its own L2. This maps the
+// workgroups of one launch so that each XCD walks a contiguous eighth of the
+// tiles, which keeps the node data that neighbouring tiles share in one L2.
+// Launch xcdGrid(tiles) workgroups; a result >= tiles is an idle workgroup.
+constexpr u32 kXcds = 8;
+__host__ __device__ __forceinline__ u32 xcdGrid(u32 tiles) { return (tiles + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ u32 xcdTile(u32 w, u32 tiles) {
+    const u32 per = (tiles + kXcds - 1) / kXcds;
+    return (w % kXcds) * per + w / kXcds;
+}
 
 template <class R> struct DevLattice {
     u32 N, B, D, blocks, node_tiles, bond_tiles, bitmap_words;
@@ -79,7 +93,7 @@ template <class R> struct DevLattice {
     R *rinv;          // 9 planes of N: cached inverse rest covariance
     R *unm;           // 3 planes of N: unmeasured normal
     R4<R> *sa, *sb;   // strain (xx, yy, zz, xy), (xz, yz, valid, 0)
-    u8 *valid, *dirty;
+    u8 *valid;
     R *approach;      // 2 planes of N
     u8 *engaged;      // 2 planes of N
     u8 *cand;         // N
@@ -95,13 +109,23 @@ template <class R> struct DevLattice {
     const uint2 *slots;
     u32 *stamp;       // substep (+1, low 32 bits) a bond broke in
     R *removed;       // stored energy of a bond at its break
+    const u32 *nbr_bond;  // D planes of N: schedule index of the slot's bond (start sample)
     u32 *tile_cand;   // candidates per node tile
     u64 *tile_bits;   // bond tiles with events this substep
+    u32 *tile_events; // events per bond tile
+    u32 *ev_j;        // per tile, its events in bond order: schedule index, bit 31 = broke
+    float *ev_d;      // the event bond's damage as the float the fold compares
+    R *ev_e;          // stored energy removed by a break
+    u32 *flag_list;   // scratch: flagged tiles in order
+    u32 *dirty_list;  // nodes a break made dirty, each once per substep (unordered)
+    u32 *dirty_stamp; // substep stamp of a node's entry in dirty_list
+    u32 *rank_stamp;  // break round a node was last counted rank-deficient in
     double *tile_plastic; // plastic work per bond tile
     double *load_part;    // ledger partial sums per node tile (13 each)
     R4<R> *fext;      // applied external force in backend precision, w = 1 where loaded
     const double *freq; // requested external force (3N double)
     u32 *cand_all;    // ordered candidates, all blocks
+    u32 *cand_tile;   // per node tile, its candidates in node order
     u32 *cand_lo, *cand_cnt;
     const u32 *nbb;   // node block begin
     u32 *first_fail;  // schedule indices broken in the first failure round
@@ -121,56 +145,75 @@ template <class R, bool DIRECT> __device__ __forceinline__ V3<R> nodePos(const D
 // LatticePhysics.hpp:792-907 (plate bending is refused at upload).
 // Dead and absent neighbour slots contribute an exact zero to every sum in the
 // reference, so they are skipped; the result is the same bits.
+// The cached inverse rest covariance of a node whose live neighbour set
+// changed (the dirty branch of LatticePhysics.hpp:796-832).
+template <class R> __device__ void rinvNode(const DevLattice<R> &L, u32 i, u32 *rank_counter, u32 stamp) {
+    const u32 N = L.N, D = L.D;
+    R cov[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    u32 live = 0;
+    for (u32 k = 0; k < D; ++k) {
+        const size_t slot = size_t(k) * N + i;
+        if (L.nbr_other[slot] == kNone) break;
+        if (!L.nbr_alive[slot]) continue;
+        const R w = L.nbr_w[slot];
+        if (w <= R(0)) continue;
+        const R rr[3] = {L.nbr_rx[slot], L.nbr_ry[slot], L.nbr_rz[slot]};
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) cov[r * 3 + q] += w * rr[r] * rr[q];
+        ++live;
+    }
+    u8 valid = 0;
+    if (live >= 1) {
+        R inv[9], un[3];
+        const int rank = pinv3(cov, inv, un);
+        if (rank >= 2) {
+            for (int k = 0; k < 9; ++k) L.rinv[size_t(k) * N + i] = inv[k];
+            for (int k = 0; k < 3; ++k) L.unm[size_t(k) * N + i] = un[k];
+            valid = 1;
+        }
+        // Counted once per node and break round, however often the node is recomputed.
+        if (rank < 3 && atomicExch(&L.rank_stamp[i], stamp) != stamp) atomicAdd(rank_counter, 1U);
+    }
+    L.valid[i] = valid;
+    if (!valid) {
+        R4<R> sb = L.sb[i];
+        sb.z = R(0);
+        L.sb[i] = sb;
+    }
+}
+
+// The nodal strain from the cached inverse (LatticePhysics.hpp:833-906).
 template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &L, u32 i) {
     const u32 N = L.N, D = L.D;
-    if (L.dirty[i]) {
-        R cov[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        u32 live = 0;
-        for (u32 k = 0; k < D; ++k) {
-            const size_t slot = size_t(k) * N + i;
-            if (L.nbr_other[slot] == kNone) break;
-            if (!L.nbr_alive[slot]) continue;
-            const R w = L.nbr_w[slot];
-            if (w <= R(0)) continue;
-            const R rr[3] = {L.nbr_rx[slot], L.nbr_ry[slot], L.nbr_rz[slot]};
-            for (int r = 0; r < 3; ++r)
-                for (int q = 0; q < 3; ++q) cov[r * 3 + q] += w * rr[r] * rr[q];
-            ++live;
-        }
-        u8 valid = 0;
-        if (live >= 1) {
-            R inv[9], un[3];
-            const int rank = pinv3(cov, inv, un);
-            if (rank >= 2) {
-                for (int k = 0; k < 9; ++k) L.rinv[size_t(k) * N + i] = inv[k];
-                for (int k = 0; k < 3; ++k) L.unm[size_t(k) * N + i] = un[k];
-                valid = 1;
-            }
-            if (rank < 3) atomicAdd(&L.ctl->rank_deficient, 1U);
-        }
-        L.valid[i] = valid;
-        L.dirty[i] = 0;
-        if (!valid) {
-            R4<R> sb = L.sb[i];
-            sb.z = R(0);
-            L.sb[i] = sb;
-        }
-    }
     if (!L.valid[i]) return;
     R acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const V3<R> xi = nodePos<R, DIRECT>(L, i, L.u4[i]);
-    for (u32 k = 0; k < D; ++k) {
-        const size_t slot = size_t(k) * N + i;
-        const u32 other = L.nbr_other[slot];
-        if (other == kNone) break;
-        if (!L.nbr_alive[slot]) continue;
-        const R w = L.nbr_w[slot];
-        const V3<R> rest{L.nbr_rx[slot], L.nbr_ry[slot], L.nbr_rz[slot]};
-        const V3<R> xo = nodePos<R, DIRECT>(L, other, L.u4[other]);
-        const V3<R> cur = DIRECT ? (xo - xi) - rest : xo - xi;
-        const R cc[3] = {cur.x, cur.y, cur.z}, rr[3] = {rest.x, rest.y, rest.z};
-        for (int r = 0; r < 3; ++r)
-            for (int q = 0; q < 3; ++q) acc[r * 3 + q] += w * cc[r] * rr[q];
+    // Slots in chunks of 8: every load of a chunk is issued before the sums,
+    // which stay in slot order.
+    constexpr u32 kChunk = 8;
+    for (u32 k0 = 0; k0 < D; k0 += kChunk) {
+        u32 other[kChunk];
+        bool live[kChunk];
+        for (u32 q = 0; q < kChunk; ++q) {
+            const u32 k = k0 + q;
+            const size_t slot = size_t(k) * N + i;
+            other[q] = k < D ? L.nbr_other[slot] : kNone;
+            live[q] = other[q] != kNone && L.nbr_alive[slot];
+        }
+        R4<R> xo4[kChunk];
+        for (u32 q = 0; q < kChunk; ++q) xo4[q] = live[q] ? L.u4[other[q]] : R4<R>{R(0), R(0), R(0), R(0)};
+        for (u32 q = 0; q < kChunk; ++q) {
+            if (!live[q]) continue;
+            const size_t slot = size_t(k0 + q) * N + i;
+            const R w = L.nbr_w[slot];
+            const V3<R> rest{L.nbr_rx[slot], L.nbr_ry[slot], L.nbr_rz[slot]};
+            const V3<R> xo = nodePos<R, DIRECT>(L, other[q], xo4[q]);
+            const V3<R> cur = DIRECT ? (xo - xi) - rest : xo - xi;
+            const R cc[3] = {cur.x, cur.y, cur.z}, rr[3] = {rest.x, rest.y, rest.z};
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) acc[r * 3 + c] += w * cc[r] * rr[c];
+        }
+        if (other[kChunk - 1] == kNone) break;
     }
     R inv[9], un[3], e[6];
     for (int k = 0; k < 9; ++k) inv[k] = L.rinv[size_t(k) * N + i];
@@ -180,11 +223,38 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
     L.sb[i] = R4<R>{e[4], e[5], R(1), R(0)};
 }
 
+// GATED is the start sample after a failure round (CpuLatticeBackend.cpp:363-370):
+// only nodes a break made dirty can have a different strain from the one the
+// previous substep ended with (their neighbours and positions are unchanged),
+// so only they are recomputed, and they stay marked for k_bond_start.
 template <class R, bool DIRECT, bool GATED>
 __global__ __launch_bounds__(kWG) void k_strain(DevLattice<R> L) {
     if (L.ctl->stop) return;
-    if (GATED && !L.ctl->dirty_start) return;
-    for (u32 i = blockIdx.x * kWG + threadIdx.x; i < L.N; i += gridDim.x * kWG) strainNode<R, DIRECT>(L, i);
+    if (GATED) {
+        if (!L.ctl->dirty_start) return;
+        const u32 count = L.ctl->dirty_count;
+        for (u32 k = blockIdx.x * kWG + threadIdx.x; k < count; k += gridDim.x * kWG)
+            strainNode<R, DIRECT>(L, L.dirty_list[k]);
+        return;
+    }
+    const u32 tile = xcdTile(blockIdx.x, L.node_tiles);
+    const u32 i = tile * kWG + threadIdx.x;
+    if (tile < L.node_tiles && i < L.N) strainNode<R, DIRECT>(L, i);
+}
+
+// Inverse rest covariances of the nodes the last breaks made dirty. It needs
+// only the live set k_bond_end left, so it runs on a side stream while
+// k_finish folds the same substep. The reference recomputes these at the start
+// of the next substep and counts rank-deficient ones there, so the count goes
+// to a per-parity slot that the next substep's k_finish books.
+// It ignores the stop flag, which k_finish may raise while it runs: the
+// recomputation is idempotent and the count is stamped per node.
+template <class R> __global__ __launch_bounds__(kWG) void k_rinv(DevLattice<R> L) {
+    const u32 count = L.ctl->dirty_count;
+    u32 *counter = &L.ctl->rank_pending[L.ctl->rank_slot];
+    const u32 stamp = L.ctl->rinv_stamp;
+    for (u32 k = blockIdx.x * kWG + threadIdx.x; k < count; k += gridDim.x * kWG)
+        rinvNode<R>(L, L.dirty_list[k], counter, stamp);
 }
 
 // ---------------------------------------------------------------------------
@@ -231,18 +301,25 @@ __device__ void sampleBond(const DevLattice<R> &L, const BondRec<R> &r, const R4
     }
 }
 
+// The start sample of the live bonds at dirty nodes (LatticePhysics.hpp:1118-1126).
+// A bond with no dirty end samples exactly what its end sample stored, so
+// only the bonds in dirty nodes' slots are resampled; a bond with both ends
+// dirty is written twice with the same bits.
 template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_start(DevLattice<R> L) {
     if (L.ctl->stop || !L.ctl->dirty_start) return;
-    for (u32 j = blockIdx.x * kWG + threadIdx.x; j < L.B; j += gridDim.x * kWG) {
-        if (!L.alive[j]) continue;
+    const u64 pairs = u64(L.ctl->dirty_count) * L.D;
+    for (u64 p = u64(blockIdx.x) * kWG + threadIdx.x; p < pairs; p += u64(gridDim.x) * kWG) {
+        const u32 i = L.dirty_list[p / L.D], k = u32(p % L.D);
+        const u32 j = L.nbr_bond[size_t(k) * L.N + i];
+        if (j == kNone || !L.alive[j]) continue;
         const BondRec<R> r = L.rec[j];
         R t, c, s;
         sampleBond<R, DIRECT>(L, r, L.u4[r.a], L.u4[r.b], t, c, s);
-        R4<R> p = L.pvd[j];
-        p.x = maxR(R(0), t);
-        p.y = maxR(R(0), c);
-        p.z = maxR(R(0), s);
-        L.pvd[j] = p;
+        R4<R> pv = L.pvd[j];
+        pv.x = maxR(R(0), t);
+        pv.y = maxR(R(0), c);
+        pv.z = maxR(R(0), s);
+        L.pvd[j] = pv;
     }
 }
 
@@ -265,9 +342,14 @@ template <class R, bool SPHERE, bool LOADS, bool DIRECT>
 __global__ __launch_bounds__(kWG) void k_predict(DevLattice<R> L, Step<R> S) {
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->dirty_start = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->dirty_start = 0;
+        ctl->dirty_count = 0;
+    }
     const bool loads = LOADS && ctl->load_remaining > 0;
-    const u32 i = blockIdx.x * kWG + threadIdx.x;
+    const u32 tile = xcdTile(blockIdx.x, L.node_tiles);
+    if (tile >= L.node_tiles) return;
+    const u32 i = tile * kWG + threadIdx.x;
     bool cand = false;
     double led[13];
     if (LOADS)
@@ -325,46 +407,105 @@ __global__ __launch_bounds__(kWG) void k_predict(DevLattice<R> L, Step<R> S) {
         }
     }
     if (SPHERE) {
-        const int n = __syncthreads_count(cand);
-        if (threadIdx.x == 0) L.tile_cand[blockIdx.x] = u32(n);
+        // The tile's candidates in node order (ballot per wave, waves in order),
+        // so the striker pass only concatenates tile lists.
+        __shared__ u32 s_wc[kWG / 64];
+        const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
+        const u64 ballot = __ballot(cand);
+        if (lane == 0) s_wc[wave] = u32(__popcll(ballot));
+        __syncthreads();
+        u32 before = 0, total = 0;
+        for (u32 w = 0; w < kWG / 64; ++w) {
+            before += w < wave ? s_wc[w] : 0U;
+            total += s_wc[w];
+        }
+        if (cand) L.cand_tile[tile * kWG + before + u32(__popcll(ballot & ((1ULL << lane) - 1ULL)))] = i;
+        if (threadIdx.x == 0) L.tile_cand[tile] = total;
     }
     if (LOADS && loads) {
         __shared__ double sh[13][kWG];
         blockSum13<R>(sh, led);
         if (threadIdx.x == 0)
-            for (int k = 0; k < 13; ++k) L.load_part[size_t(blockIdx.x) * 13 + k] = sh[k][0];
+            for (int k = 0; k < 13; ++k) L.load_part[size_t(tile) * 13 + k] = sh[k][0];
     }
+}
+
+// Exclusive prefix sum over the 1024 threads of a workgroup (wave shuffles,
+// then the 16 wave totals); returns the sum before this thread, sets *total.
+__device__ __forceinline__ u32 blockScan1024(u32 v, u32 *s_waves, u32 *total) {
+    const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
+    u32 x = v;
+    for (u32 o = 1; o < 64; o <<= 1) {
+        const u32 y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_waves[wave] = x;
+    __syncthreads();
+    u32 base = 0, all = 0;
+    for (u32 w = 0; w < kFinishThreads / 64; ++w) {
+        base += w < wave ? s_waves[w] : 0U;
+        all += s_waves[w];
+    }
+    __syncthreads();
+    *total = all;
+    return base + x - v;
 }
 
 // ---------------------------------------------------------------------------
 // Striker contact (LatticePhysics.hpp:1324-1387), one node, serial caller.
+//
+// The node's offset from the sphere centre, its distance, the contact normal,
+// the gap and |r x n|^2 depend on the centre only, and the centre is constant
+// through pass 1 and moves in pass 2 only when a position correction is
+// applied. They are computed for all candidates in parallel ("speculated")
+// and the serial chain uses them while the centre is the one they were
+// computed at; every value is the same expression on the same inputs, so the
+// bits are the reference's.
 
 template <class R> struct ContactAcc {
     u64 contacts;
     double imp[3], dissipated, max_pen, max_corr;
 };
 
+__device__ __forceinline__ bool bitsEqual(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+__device__ __forceinline__ bool bitsEqual(double a, double b) {
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+template <class R> struct Spec {
+    V3<R> r, n;
+    R gap, crn;  // crn = |r x n|^2
+};
+
+template <class R> __device__ __forceinline__ Spec<R> speculate(const Step<R> &S, const Sphere<R> &sp, V3<R> x0, V3<R> u) {
+    Spec<R> o;
+    o.r = (x0 + u) - sp.center;
+    const R dist = length(o.r);
+    o.n = normalized(o.r);
+    o.gap = dist - (sp.radius + S.c_node_radius);
+    o.crn = length2(cross(o.r, o.n));
+    return o;
+}
+
+// Returns true when a position correction moved the sphere centre.
 template <class R>
-__device__ void contactNode(const Step<R> &S, Sphere<R> &sp, V3<R> x0, V3<R> &u, V3<R> &v, R mass, bool correct,
-                            ContactAcc<R> &acc) {
-    if (mass <= R(0)) return;
+__device__ bool contactNode(const Step<R> &S, Sphere<R> &sp, const Spec<R> &q, V3<R> &u, V3<R> &v, R mass,
+                            bool correct, ContactAcc<R> &acc) {
+    if (mass <= R(0)) return false;
     const R dt = S.dt;
     const R ism = R(1) / sp.mass;
     const R iin = R(1) / sp.inertia;
-    const R cr = sp.radius + S.c_node_radius;
-    const V3<R> r = (x0 + u) - sp.center;
-    const R dist = length(r);
-    const V3<R> n = normalized(r);
-    const R gap = dist - cr;
+    const V3<R> r = q.r, n = q.n;
+    const R gap = q.gap;
     V3<R> nv = v;
     const V3<R> rel = nv - sp.velocity - cross(sp.omega, r);
     const R vn = dot(rel, n);
-    if (gap > maxR(S.c_margin, -vn * dt)) return;
+    if (gap > maxR(S.c_margin, -vn * dt)) return false;
     if (-gap > R(acc.max_pen)) acc.max_pen = double(-gap);
     const R inm = R(1) / mass;
     const R rest = -vn > S.c_rest_thr ? S.c_rest : R(0);
     const R dvn = gap > S.c_margin ? -gap / dt : -rest * minR(vn, R(0));
-    const R ni = maxR(R(0), (dvn - vn) / (inm + ism + iin * length2(cross(r, n))));
+    const R ni = maxR(R(0), (dvn - vn) / (inm + ism + iin * q.crn));
     if (ni > R(0)) {
         const R before = R(0.5) * mass * length2(nv);
         V3<R> imp = ni * n;
@@ -399,129 +540,180 @@ __device__ void contactNode(const Step<R> &S, Sphere<R> &sp, V3<R> x0, V3<R> &u,
         const V3<R> ns = (corr * inm / sum) * n;
         const V3<R> ss = -((corr * ism / sum) * n);
         u = u + ns;
+        const V3<R> before_center = sp.center;
         sp.center = sp.center + ss;
         if (corr > R(acc.max_corr)) acc.max_corr = double(corr);
+        // Bitwise: a shift below one ulp leaves the speculated terms valid.
+        return !bitsEqual(sp.center.x, before_center.x) || !bitsEqual(sp.center.y, before_center.y) ||
+               !bitsEqual(sp.center.z, before_center.z);
     }
+    return false;
 }
 
-// The serial striker pass of one substep (pass 1 or 2) on one workgroup:
-// pass 1 first builds the ordered per-block candidate lists from the tile
-// counts k_predict left (CpuLatticeBackend.cpp:375-386, cap 1024 per block),
-// then walks them; every node's data is staged into LDS by the whole
-// workgroup and one thread runs the order-dependent chain.
+// Broadcast of one lane's value to the wave (the lane index is uniform).
+__device__ __forceinline__ float laneValue(float x, u32 lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), int(lane)));
+}
+__device__ __forceinline__ double laneValue(double x, u32 lane) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffffLL), int(lane));
+    const int hi = __builtin_amdgcn_readlane(int(b >> 32), int(lane));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <class R> __device__ __forceinline__ V3<R> laneValue(V3<R> v, u32 lane) {
+    return {laneValue(v.x, lane), laneValue(v.y, lane), laneValue(v.z, lane)};
+}
+
+constexpr u32 kMaxNodeTiles = 16384;  // candidate tile prefix held in LDS: up to 4,194,304 nodes
+
+// The serial striker pass of one substep (pass 1 or 2) on one workgroup.
+// Pass 1 first forms the ordered per-block candidate lists from the per-tile
+// lists k_predict left (CpuLatticeBackend.cpp:375-386): only the first 1024
+// of each block are ever contacted, so only those are staged. The chain then
+// runs in wave 0: each lane holds one candidate of a 64-wide window in
+// registers (node data and the centre-only terms), and the whole wave walks
+// the window in order with the walked candidate's values broadcast, so the
+// sphere state is uniform and nothing on the chain waits on memory. A centre
+// move (pass 2) re-speculates the window from the next candidate.
 template <class R, int PASS>
 __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R> S) {
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
-    __shared__ R4<R> s_x0[kStage], s_u[kStage], s_v[kStage];
-    __shared__ u32 s_scan[kFinishThreads];
-    __shared__ Sphere<R> s_sp;
+    __shared__ u32 s_prefix[kMaxNodeTiles];
+    __shared__ u32 s_waves[kFinishThreads / 64];
+    __shared__ u32 s_ids[kCap];
+    __shared__ R4<R> s_u[kCap], s_v[kCap];
     __shared__ u32 s_total;
-    const u32 tid = threadIdx.x;
-    if (tid == 0) {
-        Sphere<R> sp = *L.sphere;
-        if (PASS == 1) sp.velocity = sp.velocity + S.dt * S.gravity;
-        s_sp = sp;
-    }
+    const u32 tid = threadIdx.x, lane = tid & 63U;
     if (PASS == 1) {
-        // Ordered compaction of the candidate flags, tile by tile.
         u32 running = 0;
         for (u32 base = 0; base < L.node_tiles; base += kFinishThreads) {
             const u32 t = base + tid;
             const u32 cnt = t < L.node_tiles ? L.tile_cand[t] : 0U;
-            s_scan[tid] = cnt;
-            __syncthreads();
-            for (u32 w = 1; w < kFinishThreads; w <<= 1) {
-                const u32 add = tid >= w ? s_scan[tid - w] : 0U;
-                __syncthreads();
-                s_scan[tid] += add;
-                __syncthreads();
-            }
-            const u32 total = s_scan[kFinishThreads - 1];
-            if (cnt > 0) {
-                u32 pos = running + s_scan[tid] - cnt;
-                const u32 end = min(L.N, (t + 1) * kWG);
-                for (u32 i = t * kWG; i < end; ++i)
-                    if (L.cand[i]) L.cand_all[pos++] = i;
-            }
+            u32 total = 0;
+            const u32 at = running + blockScan1024(cnt, s_waves, &total);
+            if (t < L.node_tiles) s_prefix[t] = at;
             running += total;
-            __syncthreads();
         }
         if (tid == 0) s_total = running;
         __syncthreads();
-        __threadfence_block();
+        const u32 K = s_total;
+        // Candidate g (in node order over the whole lattice) is entry g - prefix[t] of tile t.
+        auto tileOf = [&](u32 g) {
+            u32 lo = 0, hi = L.node_tiles;
+            while (hi - lo > 1) {
+                const u32 mid = (lo + hi) >> 1;
+                if (s_prefix[mid] <= g) lo = mid;
+                else hi = mid;
+            }
+            return lo;
+        };
+        auto idOf = [&](u32 g) {
+            const u32 t = tileOf(g);
+            return L.cand_tile[t * kWG + (g - s_prefix[t])];
+        };
         for (u32 b = tid; b < L.blocks; b += kFinishThreads) {
-            const u32 K = s_total;
             auto lower = [&](u32 key) {
                 u32 lo = 0, hi = K;
                 while (lo < hi) {
                     const u32 mid = (lo + hi) >> 1;
-                    if (L.cand_all[mid] < key) lo = mid + 1;
+                    if (idOf(mid) < key) lo = mid + 1;
                     else hi = mid;
                 }
                 return lo;
             };
-            const u32 lo = lower(L.nbb[b]), hi = lower(L.nbb[b + 1]);
+            const u32 lo = b == 0 ? 0U : lower(L.nbb[b]);
+            const u32 hi = b + 1 == L.blocks ? K : lower(L.nbb[b + 1]);
             const u32 count = hi - lo;
             L.cand_lo[b] = lo;
             L.cand_cnt[b] = count < kCap ? count : kCap;
             if (count > kCap) atomicAdd((unsigned long long *)&ctl->candidate_overflow, (unsigned long long)(count - kCap));
         }
-        __threadfence_block();
-    }
-    __syncthreads();
-    ContactAcc<R> acc;
-    Sphere<R> sp;
-    V3<R> center_before;
-    R energy_before;
-    if (tid == 0) {
-        acc.contacts = ctl->impulse_contacts;
-        for (int k = 0; k < 3; ++k) acc.imp[k] = ctl->impulse[k];
-        acc.dissipated = ctl->dissipated;
-        acc.max_pen = ctl->max_penetration;
-        acc.max_corr = ctl->max_correction;
-        sp = s_sp;
-        center_before = sp.center;
-        energy_before = R(0.5) * sp.mass * length2(sp.velocity) + R(0.5) * sp.inertia * length2(sp.omega);
-    }
-    for (u32 b = 0; b < L.blocks; ++b) {
-        const u32 lo = L.cand_lo[b], cnt = L.cand_cnt[b];
-        for (u32 c0 = 0; c0 < cnt; c0 += kStage) {
-            const u32 n = min(kStage, cnt - c0);
-            for (u32 k = tid; k < n; k += kFinishThreads) {
-                const u32 i = L.cand_all[lo + c0 + k];
-                s_x0[k] = L.x04[i];
-                s_u[k] = L.u4[i];
-                s_v[k] = L.v4[i];
-            }
-            __syncthreads();
-            if (tid == 0) {
-                for (u32 k = 0; k < n; ++k) {
-                    V3<R> u = xyz(s_u[k]), v = xyz(s_v[k]);
-                    contactNode<R>(S, sp, xyz(s_x0[k]), u, v, s_x0[k].w, PASS == 2, acc);
-                    s_u[k] = with(u, s_u[k].w);
-                    s_v[k] = with(v, R(0));
-                }
-            }
-            __syncthreads();
-            for (u32 k = tid; k < n; k += kFinishThreads) {
-                const u32 i = L.cand_all[lo + c0 + k];
-                V3<R> v = xyz(s_v[k]);
-                R4<R> u = s_u[k];
-                if (PASS == 1) {
-                    // nodeRecordApproach and the prediction (LatticePhysics.hpp:1408-1411).
-                    for (u32 p = 0; p < S.plane_count; ++p) L.approach[size_t(p) * L.N + i] = dot(v, S.planes[p].normal);
-                    u = with(xyz(u) + S.dt * v, u.w);
-                } else {
-                    // Candidates take their support velocity response here (LatticePhysics.hpp:1419-1425).
-                    for (u32 p = 0; p < S.plane_count; ++p)
-                        if (L.engaged[size_t(p) * L.N + i]) v = supportVelocity(S.planes[p], v, L.approach[size_t(p) * L.N + i]);
-                }
-                L.u4[i] = u;
-                L.v4[i] = with(v, R(0));
-            }
-            __syncthreads();
+        __syncthreads();
+        for (u32 b = 0; b < L.blocks; ++b) {
+            const u32 lo = L.cand_lo[b], cnt = L.cand_cnt[b];
+            for (u32 e = tid; e < cnt; e += kFinishThreads) L.cand_all[size_t(b) * kCap + e] = idOf(lo + e);
         }
+        __syncthreads();
+    }
+    ContactAcc<R> acc;
+    Sphere<R> sp = *L.sphere;
+    if (PASS == 1) sp.velocity = sp.velocity + S.dt * S.gravity;
+    acc.contacts = ctl->impulse_contacts;
+    for (int k = 0; k < 3; ++k) acc.imp[k] = ctl->impulse[k];
+    acc.dissipated = ctl->dissipated;
+    acc.max_pen = ctl->max_penetration;
+    acc.max_corr = ctl->max_correction;
+    const V3<R> center_before = sp.center;
+    const R energy_before = R(0.5) * sp.mass * length2(sp.velocity) + R(0.5) * sp.inertia * length2(sp.omega);
+    for (u32 b = 0; b < L.blocks; ++b) {
+        const u32 n = L.cand_cnt[b];
+        for (u32 k = tid; k < n; k += kFinishThreads) {
+            const u32 i = L.cand_all[size_t(b) * kCap + k];
+            s_ids[k] = i;
+            s_u[k] = L.u4[i];
+            s_v[k] = L.v4[i];
+        }
+        __syncthreads();
+        if (tid < 64) {
+            for (u32 k0 = 0; k0 < n;) {
+                // This lane's candidate of the window, in registers.
+                const u32 k = k0 + lane;
+                const bool have = k < n;
+                V3<R> x0{R(0), R(0), R(0)}, mu{R(0), R(0), R(0)}, mv{R(0), R(0), R(0)};
+                R mass = R(0);
+                if (have) {
+                    const R4<R> x = L.x04[s_ids[k]];
+                    x0 = xyz(x);
+                    mass = x.w;
+                    mu = xyz(s_u[k]);
+                    mv = xyz(s_v[k]);
+                }
+                const Spec<R> q = speculate(S, sp, x0, mu);
+                const u32 span = min(64U, n - k0);
+                u32 next = k0 + span;
+                for (u32 w = 0; w < span; ++w) {
+                    Spec<R> qw;
+                    qw.r = laneValue(q.r, w);
+                    qw.n = laneValue(q.n, w);
+                    qw.gap = laneValue(q.gap, w);
+                    qw.crn = laneValue(q.crn, w);
+                    V3<R> u = laneValue(mu, w), v = laneValue(mv, w);
+                    const bool moved = contactNode<R>(S, sp, qw, u, v, laneValue(mass, w), PASS == 2, acc);
+                    if (lane == w) {
+                        mu = u;
+                        mv = v;
+                    }
+                    if (moved) {
+                        next = k0 + w + 1;
+                        break;
+                    }
+                }
+                if (have && k < next) {
+                    s_u[k] = with(mu, s_u[k].w);
+                    s_v[k] = with(mv, R(0));
+                }
+                k0 = next;
+            }
+        }
+        __syncthreads();
+        for (u32 k = tid; k < n; k += kFinishThreads) {
+            const u32 i = s_ids[k];
+            V3<R> v = xyz(s_v[k]);
+            R4<R> u = s_u[k];
+            if (PASS == 1) {
+                // nodeRecordApproach and the prediction (LatticePhysics.hpp:1408-1411).
+                for (u32 p = 0; p < S.plane_count; ++p) L.approach[size_t(p) * L.N + i] = dot(v, S.planes[p].normal);
+                u = with(xyz(u) + S.dt * v, u.w);
+            } else {
+                // Candidates take their support velocity response here (LatticePhysics.hpp:1419-1425).
+                for (u32 p = 0; p < S.plane_count; ++p)
+                    if (L.engaged[size_t(p) * L.N + i]) v = supportVelocity(S.planes[p], v, L.approach[size_t(p) * L.N + i]);
+            }
+            L.u4[i] = u;
+            L.v4[i] = with(v, R(0));
+        }
+        __syncthreads();
     }
     if (tid == 0) {
         acc.dissipated += double(energy_before - (R(0.5) * sp.mass * length2(sp.velocity) + R(0.5) * sp.inertia * length2(sp.omega)));
@@ -541,9 +733,12 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R
 // stage's bonds share no node, so every thread owns its two ends.
 
 template <class R, bool DIRECT, bool PLASTIC>
-__global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const uint2 *wg_range, u32 wg_base, int first) {
+__global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const uint2 *wg_range, u32 wg_base,
+                                                u32 wg_count, int first) {
     if (L.ctl->stop) return;
-    const uint2 rg = wg_range[wg_base + blockIdx.x];
+    const u32 w = xcdTile(blockIdx.x, wg_count);
+    if (w >= wg_count) return;
+    const uint2 rg = wg_range[wg_base + w];
     const u32 j = rg.x + threadIdx.x;
     if (j >= rg.y) return;
     if (!L.alive[j]) return;
@@ -629,7 +824,10 @@ template <class R> __global__ __launch_bounds__(kWG) void k_project(DevLattice<R
 template <class R, bool PROJECT>
 __global__ __launch_bounds__(kWG) void k_velocity(DevLattice<R> L, Step<R> S) {
     if (L.ctl->stop) return;
-    for (u32 i = blockIdx.x * kWG + threadIdx.x; i < L.N; i += gridDim.x * kWG) {
+    const u32 tile = xcdTile(blockIdx.x, L.node_tiles);
+    {
+        const u32 i = tile * kWG + threadIdx.x;
+        if (tile >= L.node_tiles || i >= L.N) return;
         R4<R> u = L.u4[i];
         if (PROJECT) {
             bool moved = false;
@@ -671,13 +869,23 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
     if (ctl->stop) return;
     __shared__ float s_max[4][kWG / 64];
     __shared__ double s_pw[kWG];
+    __shared__ u32 s_wave_events[kWG / 64];
     const double m0 = ctl->max_damage;
     const u32 stamp = u32(ctl->step) + 1U;
     const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->rank_slot = u32(ctl->step & 1U);
+        ctl->rinv_stamp = stamp;
+    }
     float mt = 0.f, mc = 0.f, ms = 0.f, mp = 0.f;
-    for (u32 tile = blockIdx.x; tile < L.bond_tiles; tile += gridDim.x) {
+    const u32 tile = xcdTile(blockIdx.x, L.bond_tiles);
+    if (tile >= L.bond_tiles) return;
+    {
         const u32 j = tile * kWG + threadIdx.x;
-        bool event = false;
+        bool event = false, broke = false;
+        u32 claim_a = kNone, claim_b = kNone;
+        float event_damage = 0.f;
+        R event_energy = R(0);
         double pw = 0.0;
         if (j < L.B && L.alive[j]) {
             const BondRec<R> r = L.rec[j];
@@ -725,22 +933,59 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
                 L.mode[j] = md;
             }
             L.pvd[j] = pv;
-            event = double(float(pv.w)) > m0 + 1.0e-6;
+            event_damage = float(pv.w);
+            event = double(event_damage) > m0 + 1.0e-6;
             if (pv.w >= R(1)) {
                 const R ext = length(bondVec<R, DIRECT>(L, r, ua, ub)) - r.L - plastic;
-                L.removed[j] = r.c <= R(0) ? R(0) : R(0.5) * ext * ext / r.c;
+                event_energy = r.c <= R(0) ? R(0) : R(0.5) * ext * ext / r.c;
+                L.removed[j] = event_energy;
+                broke = true;
                 L.alive[j] = 0;
                 const uint2 sl = L.slots[j];
                 L.nbr_alive[sl.x] = 0;
                 L.nbr_alive[sl.y] = 0;
-                L.dirty[r.a] = 1;
-                L.dirty[r.b] = 1;
                 L.stamp[j] = stamp;
+                claim_a = atomicExch(&L.dirty_stamp[r.a], stamp) != stamp ? r.a : kNone;
+                claim_b = atomicExch(&L.dirty_stamp[r.b], stamp) != stamp ? r.b : kNone;
                 event = true;
             }
         }
-        if (__syncthreads_or(event) && threadIdx.x == 0)
+        {
+            const u32 mine = (claim_a != kNone) + (claim_b != kNone);
+            u32 x = mine;
+            for (u32 o = 1; o < 64; o <<= 1) {
+                const u32 y = __shfl_up(x, o);
+                if (lane >= o) x += y;
+            }
+            const u32 wave_total = __shfl(x, 63);
+            u32 base = 0;
+            if (lane == 63 && wave_total) base = atomicAdd(&ctl->dirty_count, wave_total);
+            base = __shfl(base, 63);
+            u32 at = base + x - mine;
+            if (claim_a != kNone) L.dirty_list[at++] = claim_a;
+            if (claim_b != kNone) L.dirty_list[at] = claim_b;
+        }
+        // Ordered compaction of the tile's events: wave ballots, then the
+        // waves' counts in order, so the list is in bond (schedule) order.
+        const u64 ballot = __ballot(event);
+        if (lane == 0) s_wave_events[wave] = u32(__popcll(ballot));
+        __syncthreads();
+        u32 before = 0, total = 0;
+        for (u32 w = 0; w < kWG / 64; ++w) {
+            before += w < wave ? s_wave_events[w] : 0U;
+            total += s_wave_events[w];
+        }
+        if (event) {
+            const u32 pos = tile * kWG + before + u32(__popcll(ballot & ((1ULL << lane) - 1ULL)));
+            L.ev_j[pos] = j | (broke ? 0x80000000U : 0U);
+            L.ev_d[pos] = event_damage;
+            L.ev_e[pos] = event_energy;
+        }
+        if (threadIdx.x == 0 && total) {
+            L.tile_events[tile] = total;
             atomicOr((unsigned long long *)&L.tile_bits[tile >> 6], 1ULL << (tile & 63U));
+        }
+        __syncthreads();
         if (PLASTIC) {
             s_pw[threadIdx.x] = pw;
             __syncthreads();
@@ -769,8 +1014,10 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
     if (threadIdx.x < 4) {
         float m = 0.f;
         for (int w = 0; w < kWG / 64; ++w) m = fmaxf(m, s_max[threadIdx.x][w]);
-        if (m > 0.f) {
-            u32 *dst = threadIdx.x < 3 ? &ctl->peak_bits[threadIdx.x] : &ctl->pstretch_bits;
+        u32 *dst = threadIdx.x < 3 ? &ctl->peak_bits[threadIdx.x] : &ctl->pstretch_bits;
+        // The running maxima rarely move: read first, so tens of thousands of
+        // workgroups do not queue atomics on the same four words.
+        if (m > 0.f && __float_as_uint(m) > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             atomicMax(dst, __float_as_uint(m));
         }
     }
@@ -840,74 +1087,104 @@ __device__ double fixedOrderSum(const double *part, u32 count, u32 stride, u32 o
     return total;
 }
 
+constexpr u32 kEventBatch = 4096;
+
 template <class R, bool LOADS, bool PLASTIC>
 __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step<R> S) {
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
-    __shared__ u64 s_words[kFinishThreads];
     __shared__ double s_red[kFinishThreads];
+    __shared__ u32 s_waves[kFinishThreads / 64];
+    __shared__ u32 s_tile[kFinishThreads], s_incl[kFinishThreads];
+    __shared__ u32 s_j[kEventBatch];
+    __shared__ float s_d[kEventBatch];
+    __shared__ R s_e[kEventBatch];
+    __shared__ u32 s_take;
     const u32 tid = threadIdx.x, lane = tid & 63U;
     const u64 step = ctl->step;
-    const u32 stamp = u32(step) + 1U;
-    // Wave 0 carries the folds; every lane holds the same running values.
+    // Wave 0 carries the folds; its lanes all hold the same running values.
     double m = ctl->max_damage, removed = ctl->removed_energy_j;
     bool damage_gain = false, energy_gain = false;
     u32 broken_now = 0;
     const bool first_round = ctl->broken_bonds == 0;
     u64 ff = ctl->first_failure_count;
     const double flat = ctl->flat_fraction;
-    for (u32 base = 0; base < L.bitmap_words; base += kFinishThreads) {
-        const u32 w = base + tid;
+    for (u32 wbase = 0; wbase < L.bitmap_words; wbase += kFinishThreads) {
+        // Flagged tiles of these bitmap words, in order, into flag_list.
+        const u32 w = wbase + tid;
         u64 bits = 0;
         if (w < L.bitmap_words) {
             bits = L.tile_bits[w];
             if (bits) L.tile_bits[w] = 0;
         }
-        s_words[tid] = bits;
+        u32 nflag = 0;
+        u32 at = blockScan1024(u32(__popcll(bits)), s_waves, &nflag);
+        while (bits) {
+            L.flag_list[at++] = w * 64U + u32(__ffsll((long long)bits) - 1);
+            bits &= bits - 1;
+        }
         __syncthreads();
-        if (tid < 64) {
-            const u32 nw = min(u32(kFinishThreads), L.bitmap_words - base);
-            for (u32 k = 0; k < nw; ++k) {
-                u64 word = s_words[k];
-                while (word) {
-                    const u32 tile = (base + k) * 64U + u32(__ffsll((long long)word) - 1);
-                    word &= word - 1;
-                    for (u32 sub = 0; sub < kWG; sub += 64) {
-                        const u32 j = tile * kWG + sub + lane;
-                        bool eval = false, broke = false;
-                        float dmg = 0.f;
-                        double e = 0.0;
-                        if (j < L.B) {
-                            broke = L.stamp[j] == stamp && !L.alive[j];
-                            eval = L.alive[j] || broke;
-                            dmg = float(L.pvd[j].w);
-                            if (broke) e = double(L.removed[j]);
-                        }
-                        // max_damage rises only by more than 1e-6 (CpuLatticeBackend.cpp:462-465).
-                        u64 cand = __ballot(eval && double(dmg) > m + 1.0e-6);
-                        while (cand) {
-                            const int l = __ffsll((long long)cand) - 1;
-                            m = double(__shfl(dmg, l));
-                            damage_gain = true;
-                            cand = __ballot(int(lane) > l && eval && double(dmg) > m + 1.0e-6);
-                        }
-                        // Removed energy summed in bond order (CpuLatticeBackend.cpp:466-477).
-                        u64 bmask = __ballot(broke);
-                        while (bmask) {
-                            const int l = __ffsll((long long)bmask) - 1;
-                            bmask &= bmask - 1;
-                            const double el = __shfl(e, l);
-                            removed += el;
-                            if (el > flat * (removed > 1.0e-12 ? removed : 1.0e-12)) energy_gain = true;
-                            if (first_round && lane == 0) L.first_fail[ff] = tile * kWG + sub + u32(l);
-                            ++ff;
-                            ++broken_now;
-                        }
+        for (u32 fb = 0; fb < nflag;) {
+            // As many whole tiles as fit in the event batch.
+            const u32 mcount = min(u32(kFinishThreads), nflag - fb);
+            const u32 t = tid < mcount ? L.flag_list[fb + tid] : 0U;
+            const u32 c = tid < mcount ? L.tile_events[t] : 0U;
+            u32 all = 0;
+            const u32 incl = blockScan1024(c, s_waves, &all) + c;
+            s_tile[tid] = t;
+            s_incl[tid] = incl;
+            __syncthreads();
+            if (tid < mcount && incl <= kEventBatch && (tid + 1 == mcount || s_incl[tid + 1] > kEventBatch)) s_take = tid + 1;
+            __syncthreads();
+            const u32 take = s_take;
+            const u32 nev = s_incl[take - 1];
+            for (u32 e = tid; e < nev; e += kFinishThreads) {
+                u32 lo = 0, hi = take;
+                while (lo < hi) {
+                    const u32 mid = (lo + hi) >> 1;
+                    if (s_incl[mid] <= e) lo = mid + 1;
+                    else hi = mid;
+                }
+                const u32 slot = s_tile[lo] * kWG + (e - (lo ? s_incl[lo - 1] : 0U));
+                s_j[e] = L.ev_j[slot];
+                s_d[e] = L.ev_d[slot];
+                s_e[e] = L.ev_e[slot];
+            }
+            __syncthreads();
+            if (tid < 64) {
+                for (u32 c0 = 0; c0 < nev; c0 += 64) {
+                    const u32 e = c0 + lane;
+                    const bool ok = e < nev;
+                    const u32 jr = ok ? s_j[e] : 0U;
+                    const float dmg = ok ? s_d[e] : 0.f;
+                    const double en = ok ? double(s_e[e]) : 0.0;
+                    // max_damage rises only by more than 1e-6 (CpuLatticeBackend.cpp:462-465).
+                    u64 cand = __ballot(ok && double(dmg) > m + 1.0e-6);
+                    while (cand) {
+                        const int l = __ffsll((long long)cand) - 1;
+                        m = double(__shfl(dmg, l));
+                        damage_gain = true;
+                        cand = __ballot(ok && int(lane) > l && double(dmg) > m + 1.0e-6);
+                    }
+                    // Removed energy summed in bond order (CpuLatticeBackend.cpp:466-477).
+                    const u64 bmask = __ballot(ok && (jr >> 31));
+                    if (first_round && ok && (jr >> 31))
+                        L.first_fail[ff + __popcll(bmask & ((1ULL << lane) - 1ULL))] = jr & 0x7fffffffU;
+                    ff += __popcll(bmask);
+                    broken_now += u32(__popcll(bmask));
+                    u64 left = bmask;
+                    while (left) {
+                        const int l = __ffsll((long long)left) - 1;
+                        left &= left - 1;
+                        const double el = __shfl(en, l);
+                        removed += el;
+                        if (el > flat * (removed > 1.0e-12 ? removed : 1.0e-12)) energy_gain = true;
                     }
                 }
             }
+            __syncthreads();
+            fb += take;
         }
-        __syncthreads();
     }
     double ledger[13];
     const bool loads = LOADS && ctl->load_remaining > 0;
@@ -916,6 +1193,11 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step
     double plastic = 0.0;
     if (PLASTIC) plastic = fixedOrderSum<R>(L.tile_plastic, L.bond_tiles, 1, 0, s_red);
     if (tid != 0) return;
+    {
+        const u32 slot = u32((step + 1U) & 1U);  // (step - 1) & 1: the slot the previous substep's k_rinv used
+        ctl->rank_deficient += ctl->rank_pending[slot];
+        ctl->rank_pending[slot] = 0;
+    }
     if (damage_gain) {
         ctl->max_damage = m;
         ctl->last_damage_gain_step = step;
