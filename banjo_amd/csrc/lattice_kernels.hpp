@@ -573,8 +573,10 @@ constexpr u32 kMaxNodeTiles = 16384;  // candidate tile prefix held in LDS: up t
 // runs in wave 0: each lane holds one candidate of a 64-wide window in
 // registers (node data and the centre-only terms), and the whole wave walks
 // the window in order with the walked candidate's values broadcast, so the
-// sphere state is uniform and nothing on the chain waits on memory. A centre
-// move (pass 2) re-speculates the window from the next candidate.
+// sphere state is uniform and nothing on the chain waits on memory. Candidates
+// whose contact test fails at the current sphere state are skipped by a ballot
+// over the window. A centre move (pass 2) re-speculates the window from the
+// next candidate.
 template <class R, int PASS>
 __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R> S) {
     DevCtl *ctl = L.ctl;
@@ -672,7 +674,21 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R
                 const Spec<R> q = speculate(S, sp, x0, mu);
                 const u32 span = min(64U, n - k0);
                 u32 next = k0 + span;
-                for (u32 w = 0; w < span; ++w) {
+                // A candidate that fails the contact test (LatticePhysics.hpp:1341)
+                // changes nothing, and the test reads only its own terms and the
+                // sphere's velocity and spin: every lane evaluates it at the
+                // current sphere state, and the chain jumps to the first lane
+                // that passes. After a contact the state may have changed, so the
+                // remaining lanes are evaluated again.
+                u32 pos = 0;
+                while (pos < span) {
+                    const V3<R> rel = mv - sp.velocity - cross(sp.omega, q.r);
+                    const R vn = dot(rel, q.n);
+                    const bool passes = lane >= pos && lane < span && !(mass <= R(0)) &&
+                                        !(q.gap > maxR(S.c_margin, -vn * S.dt));
+                    const u64 m = __ballot(passes);
+                    if (m == 0) break;
+                    const u32 w = u32(__ffsll((long long)m) - 1);
                     Spec<R> qw;
                     qw.r = laneValue(q.r, w);
                     qw.n = laneValue(q.n, w);
@@ -688,6 +704,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R
                         next = k0 + w + 1;
                         break;
                     }
+                    pos = w + 1;
                 }
                 if (have && k < next) {
                     s_u[k] = with(mu, s_u[k].w);
@@ -741,8 +758,12 @@ __global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const
     const uint2 rg = wg_range[wg_base + w];
     const u32 j = rg.x + threadIdx.x;
     if (j >= rg.y) return;
-    if (!L.alive[j]) return;
+    // The record is loaded with the alive flag rather than after it, so the
+    // node gathers wait on one memory round trip, not two.
     const BondRec<R> r = L.rec[j];
+    const u8 live = L.alive[j];
+    const R lam = first ? R(0) : L.lambda[j];
+    if (!live) return;
     const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
     const V3<R> du = xyz(ub) - xyz(ua);
     const V3<R> e{r.ex, r.ey, r.ez};
@@ -762,7 +783,6 @@ __global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const
     const V3<R> dir = delta / len;
     const R wa = ua.w, wb = ub.w;
     const R alpha = r.c / (S.dt * S.dt);
-    const R lam = first ? R(0) : L.lambda[j];
     const R dl = (-C - alpha * lam) / (wa + wb + alpha);
     L.lambda[j] = lam + dl;
     L.u4[r.a] = with(xyz(ua) - wa * dl * dir, ua.w);
@@ -887,8 +907,20 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
         float event_damage = 0.f;
         R event_energy = R(0);
         double pw = 0.0;
-        if (j < L.B && L.alive[j]) {
-            const BondRec<R> r = L.rec[j];
+        // The bond's streamed data is loaded together with its alive flag,
+        // so the node gathers wait on one memory round trip.
+        const bool in_range = j < L.B;
+        BondRec<R> r{};
+        R4<R> pv{};
+        Thresholds<R> th{};
+        u8 live = 0;
+        if (in_range) {
+            r = L.rec[j];
+            pv = L.pvd[j];
+            th = L.thr[j];
+            live = L.alive[j];
+        }
+        if (live) {
             const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
             R plastic = R(0);
             if (PLASTIC) {
@@ -914,7 +946,6 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
             }
             R t, c, s;
             sampleBond<R, DIRECT>(L, r, ua, ub, t, c, s);
-            R4<R> pv = L.pvd[j];
             const R pt = maxR(R(0), maxR(pv.x, t)), pc = maxR(R(0), maxR(pv.y, c)), pss = maxR(R(0), maxR(pv.z, s));
             mt = fmaxf(mt, float(pt));
             mc = fmaxf(mc, float(pc));
@@ -922,7 +953,6 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
             pv.x = maxR(R(0), t);
             pv.y = maxR(R(0), c);
             pv.z = maxR(R(0), s);
-            const Thresholds<R> th = L.thr[j];
             const R td = progress(pt, th.ds, th.de), cd = progress(pc, th.cs, th.ce), sd = progress(pss, th.ss, th.se);
             R dmg = td;
             u8 md = 1;
