@@ -12,6 +12,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -54,6 +55,48 @@ struct DevBuf {
         bytes = n;
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Distinct keys, compared bit for bit, numbered in first-seen order: an
+// open-addressing table that stays in cache while the distinct count is small.
+template <size_t W> class Dedup {
+public:
+    using Key = std::array<uint32_t, W>;
+    Dedup() : slots_(1024, kEmpty) {}
+    uint32_t index(const Key &k) {
+        if ((keys_.size() + 1) * 2 > slots_.size()) grow();
+        const size_t mask = slots_.size() - 1;
+        for (size_t h = hash(k) & mask;; h = (h + 1) & mask) {
+            const uint32_t s = slots_[h];
+            if (s == kEmpty) {
+                slots_[h] = uint32_t(keys_.size());
+                keys_.push_back(k);
+                return slots_[h];
+            }
+            if (keys_[s] == k) return s;
+        }
+    }
+    const std::vector<Key> &keys() const { return keys_; }
+
+private:
+    static constexpr uint32_t kEmpty = 0xffffffffU;
+    std::vector<uint32_t> slots_;
+    std::vector<Key> keys_;
+    static size_t hash(const Key &k) {
+        uint64_t h = 0x243f6a8885a308d3ULL;
+        for (uint32_t w : k) h = (h ^ w) * 0x9e3779b97f4a7c15ULL;
+        return size_t(h ^ (h >> 29));
+    }
+    void grow() {
+        std::vector<uint32_t> next(slots_.size() * 2, kEmpty);
+        const size_t mask = next.size() - 1;
+        for (uint32_t s = 0; s < keys_.size(); ++s) {
+            size_t h = hash(keys_[s]) & mask;
+            while (next[h] != kEmpty) h = (h + 1) & mask;
+            next[h] = s;
+        }
+        slots_.swap(next);
+    }
 };
 
 template <class T> void toDevice(DevBuf &b, const std::vector<T> &v) {
@@ -162,61 +205,142 @@ public:
         toDevice(u4_, u4);
         toDevice(up4_, up4);
         toDevice(v4_, v4);
+        // Bonds in node-major order for the end sample: a counting sort by
+        // the first end, stable in schedule index.
+        std::vector<uint32_t> node_of(B);  // node-major position -> schedule index
+        std::vector<uint32_t> pos_of(B);   // schedule index -> node-major position
         {
-            std::vector<uint32_t> other(P), bond(P);
-            std::vector<R> rx(P), ry(P), rz(P), w(P);
-            for (size_t k = 0; k < P; ++k) {
-                other[k] = st.nbr_bond[k] == BLX_NO_BOND ? kNone : st.nbr_other[k];
-                bond[k] = st.nbr_bond[k] == BLX_NO_BOND ? kNone : st.nbr_bond[k];
-                rx[k] = R(st.nbr_rest[3 * k]);
-                ry[k] = R(st.nbr_rest[3 * k + 1]);
-                rz[k] = R(st.nbr_rest[3 * k + 2]);
-                w[k] = R(st.nbr_weight[k]);
+            std::vector<uint32_t> start(size_t(N) + 1, 0U);
+            for (uint32_t k = 0; k < B; ++k) {
+                if (st.bond_a[k] >= N || st.bond_b[k] >= N) throw Invalid("bond end out of range");
+                ++start[st.bond_a[k] + 1];
             }
-            toDevice(nbr_other_, other);
+            for (uint32_t i = 0; i < N; ++i) start[i + 1] += start[i];
+            for (uint32_t k = 0; k < B; ++k) {
+                const uint32_t o = start[st.bond_a[k]]++;
+                node_of[o] = k;
+                pos_of[k] = o;
+            }
+        }
+        node_j_host_ = node_of;
+        // Neighbour slots as slot palette indices.
+        {
+            Dedup<4 * sizeof(R) / 4 + 1> table;
+            std::vector<uint32_t> key(P), bond(P);
+            for (size_t k = 0; k < P; ++k) {
+                if (st.nbr_bond[k] == BLX_NO_BOND) {
+                    key[k] = kNone;
+                    bond[k] = kNone;
+                    continue;
+                }
+                if (st.nbr_bond[k] >= B || st.nbr_other[k] >= N) throw Invalid("neighbour slot out of range");
+                const uint32_t i = uint32_t(k % N);
+                const R rw[4] = {R(st.nbr_rest[3 * k]), R(st.nbr_rest[3 * k + 1]), R(st.nbr_rest[3 * k + 2]),
+                                 R(st.nbr_weight[k])};
+                const int32_t delta = int32_t(int64_t(st.nbr_other[k]) - int64_t(i));
+                typename Dedup<4 * sizeof(R) / 4 + 1>::Key kk{};
+                std::memcpy(kk.data(), rw, sizeof(rw));
+                std::memcpy(kk.data() + 4 * sizeof(R) / 4, &delta, 4);
+                const uint32_t id = table.index(kk);
+                if (id >= kDeadSlot) throw Overflow("more than 2^31 distinct neighbour slots");
+                key[k] = id | (st.nbr_alive[k] ? 0U : kDeadSlot);
+                bond[k] = pos_of[st.nbr_bond[k]];
+            }
+            const auto &keys = table.keys();
+            std::vector<R4<R>> pal(keys.size());
+            std::vector<int32_t> dl(keys.size());
+            for (size_t q = 0; q < keys.size(); ++q) {
+                std::memcpy(&pal[q], keys[q].data(), 4 * sizeof(R));
+                std::memcpy(&dl[q], keys[q].data() + 4 * sizeof(R) / 4, 4);
+            }
+            slot_palette_ = uint32_t(keys.size());
+            toDevice(nbr_key_, key);
             toDevice(nbr_bond_, bond);
-            toDevice(nbr_rx_, rx);
-            toDevice(nbr_ry_, ry);
-            toDevice(nbr_rz_, rz);
-            toDevice(nbr_w_, w);
-            toDevice(nbr_alive_, std::vector<uint8_t>(st.nbr_alive, st.nbr_alive + P));
+            toDevice(spal_, pal);
+            toDevice(sdelta_, dl);
         }
         {
+            // Edge palette: the distinct (rest edge, rest length, |e|^2 - L^2) in
+            // backend precision; the direction terms are filled on the device.
+            Dedup<5 * sizeof(R) / 4> table;
+            std::vector<uint32_t> pal_of(B);
+            for (uint32_t k = 0; k < B; ++k) {
+                const R v[5] = {R(st.rest_edge[3 * size_t(k)]), R(st.rest_edge[3 * size_t(k) + 1]),
+                                R(st.rest_edge[3 * size_t(k) + 2]), R(st.rest_length[k]), R(st.rest_length_sq_minus[k])};
+                typename Dedup<5 * sizeof(R) / 4>::Key kk{};
+                std::memcpy(kk.data(), v, sizeof(v));
+                pal_of[k] = table.index(kk);
+                if (pal_of[k] >= kNoDir) throw Overflow("more than 2^31 distinct bond edges");
+            }
+            const auto &keys = table.keys();
+            const uint32_t np = uint32_t(keys.size());
+            edge_palette_ = np;
+            std::vector<EdgePal<R>> pal(np);
+            for (uint32_t q = 0; q < np; ++q) {
+                R v[5];
+                std::memcpy(v, keys[q].data(), sizeof(v));
+                pal[q].e = {v[0], v[1], v[2], v[3]};
+                pal[q].d = {v[4], R(0), R(0), R(0)};
+            }
+            toDevice(epal_, pal);
+            DevBuf nodir;
+            nodir.alloc(size_t(np) * 4);
+            hipLaunchKernelGGL((k_edge_palette<R>), dim3(gridFor(np)), dim3(kWG), 0, stream_, epal_.as<EdgePal<R>>(),
+                               nodir.as<uint32_t>(), np);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipStreamSynchronize(stream_));
+            std::vector<uint32_t> no_dir(np);
+            HIP_OK(hipMemcpy(no_dir.data(), nodir.p, size_t(np) * 4, hipMemcpyDeviceToHost));
+            for (uint32_t k = 0; k < B; ++k) pal_of[k] |= no_dir[pal_of[k]] ? kNoDir : 0U;
+
             std::vector<BondRec<R>> rec(B);
-            std::vector<Thresholds<R>> thr(B);
+            std::vector<BondNode> bnd(B);
             std::vector<R4<R>> pvd(B);
             std::vector<R> pext(B), pstr(B);
             std::vector<uint2> slots(B);
+            std::vector<uint8_t> alive_n(B), mode(B);
+            bool tensile = true;
+            for (uint32_t k = 0; k < B && tensile; ++k) {
+                const double *t = st.threshold + 6 * size_t(k);
+                tensile = !std::isfinite(R(t[2])) && !std::isfinite(R(t[4]));
+            }
+            tensile_only_ = tensile;
+            std::vector<Thresholds<R>> thr(tensile ? 0 : B);
+            std::vector<Tensile<R>> thr2(tensile ? B : 0);
             for (uint32_t k = 0; k < B; ++k) {
                 BondRec<R> r{};
                 r.a = st.bond_a[k];
                 r.b = st.bond_b[k];
-                r.ex = R(st.rest_edge[3 * size_t(k)]);
-                r.ey = R(st.rest_edge[3 * size_t(k) + 1]);
-                r.ez = R(st.rest_edge[3 * size_t(k) + 2]);
-                r.L = R(st.rest_length[k]);
-                r.rsm = R(st.rest_length_sq_minus[k]);
+                r.p = pal_of[k];
                 r.c = R(st.compliance[k]);
                 rec[k] = r;
-                const double *t = st.threshold + 6 * size_t(k);
-                thr[k] = {R(t[0]), R(t[1]), R(t[2]), R(t[3]), R(t[4]), R(t[5])};
-                pvd[k] = {R(st.prev_tensile[k]), R(st.prev_compressive[k]), R(st.prev_shear[k]), R(st.damage[k])};
                 pext[k] = R(st.plastic_extension[k]);
-                pstr[k] = R(st.plastic_strain[k]);
-                slots[k] = make_uint2(st.bond_slot_a[k], st.bond_slot_b[k]);
+            }
+            for (uint32_t o = 0; o < B; ++o) {
+                const uint32_t k = node_of[o];
+                bnd[o] = BondNode{st.bond_a[k], st.bond_b[k], k, pal_of[k]};
+                const double *t = st.threshold + 6 * size_t(k);
+                if (tensile) thr2[o] = {R(t[0]), R(t[1])};
+                else thr[o] = {R(t[0]), R(t[1]), R(t[2]), R(t[3]), R(t[4]), R(t[5])};
+                pvd[o] = {R(st.prev_tensile[k]), R(st.prev_compressive[k]), R(st.prev_shear[k]), R(st.damage[k])};
+                pstr[o] = R(st.plastic_strain[k]);
+                slots[o] = make_uint2(st.bond_slot_a[k], st.bond_slot_b[k]);
+                alive_n[o] = st.alive[k];
+                mode[o] = st.failure_mode[k];
             }
             toDevice(rec_, rec);
+            toDevice(bnd_, bnd);
             toDevice(thr_, thr);
+            toDevice(thr2_, thr2);
             toDevice(pvd_, pvd);
             toDevice(pext_, pext);
             toDevice(pstr_, pstr);
             toDevice(slots_, slots);
             toDevice(alive_, std::vector<uint8_t>(st.alive, st.alive + B));
-            toDevice(mode_, std::vector<uint8_t>(st.failure_mode, st.failure_mode + B));
+            toDevice(alive_n_, alive_n);
+            toDevice(mode_, mode);
         }
         zero<R>(lambda_, B);
-        zero<uint32_t>(stamp_, B);
-        zero<R>(removed_, B);
         zero<uint32_t>(first_fail_, B);
         zero<R>(rinv_, 9 * size_t(N));
         zero<R>(unm_, 3 * size_t(N));
@@ -228,14 +352,15 @@ public:
         zero<uint8_t>(cand_, N);
         node_tiles_ = (N + kWG - 1) / kWG;
         bond_tiles_ = (B + kWG - 1) / kWG;
-        bitmap_words_ = (bond_tiles_ + 63) / 64;
+        const uint32_t event_words = (B + 63) / 64;
+        group_words_ = (event_words + 63) / 64;
         zero<uint32_t>(tile_cand_, node_tiles_);
-        zero<uint64_t>(tile_bits_, bitmap_words_);
-        zero<uint32_t>(tile_events_, bond_tiles_);
-        zero<uint32_t>(ev_j_, size_t(bond_tiles_) * kWG);
-        zero<float>(ev_d_, size_t(bond_tiles_) * kWG);
-        zero<R>(ev_e_, size_t(bond_tiles_) * kWG);
-        zero<uint32_t>(flag_list_, bond_tiles_);
+        zero<uint64_t>(ev_bits_, event_words);
+        zero<uint64_t>(brk_bits_, event_words);
+        zero<uint64_t>(grp_bits_, group_words_);
+        zero<float>(ev_d_, B);
+        zero<R>(ev_e_, B);
+        zero<uint32_t>(flag_list_, event_words);
         zero<double>(tile_plastic_, bond_tiles_);
         zero<double>(load_part_, 13 * size_t(node_tiles_));
         zero<R4<R>>(fext_, N);
@@ -421,14 +546,15 @@ public:
         HIP_OK(hipStreamSynchronize(stream_));
         std::vector<R4<R>> u4(N_), up4(N_), v4(N_), pvd(B_);
         std::vector<R> pext(B_), pstr(B_);
+        std::vector<uint8_t> mode(B_);
         copyDown(u4, u4_);
         copyDown(up4, up4_);
         copyDown(v4, v4_);
         copyDown(pvd, pvd_);
         copyDown(pext, pext_);
         copyDown(pstr, pstr_);
+        copyDown(mode, mode_);
         HIP_OK(hipMemcpy(st.alive, alive_.p, B_, hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(st.failure_mode, mode_.p, B_, hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < N_; ++i) {
             st.u[3 * size_t(i)] = double(u4[i].x);
             st.u[3 * size_t(i) + 1] = double(u4[i].y);
@@ -440,13 +566,16 @@ public:
             st.v[3 * size_t(i) + 1] = double(v4[i].y);
             st.v[3 * size_t(i) + 2] = double(v4[i].z);
         }
-        for (uint32_t k = 0; k < B_; ++k) {
-            st.prev_tensile[k] = double(pvd[k].x);
-            st.prev_compressive[k] = double(pvd[k].y);
-            st.prev_shear[k] = double(pvd[k].z);
-            st.damage[k] = double(pvd[k].w);
-            st.plastic_extension[k] = double(pext[k]);
-            st.plastic_strain[k] = double(pstr[k]);
+        for (uint32_t k = 0; k < B_; ++k) st.plastic_extension[k] = double(pext[k]);
+        // The end-sample arrays are node-major on the device.
+        for (uint32_t o = 0; o < B_; ++o) {
+            const uint32_t k = node_j_host_[o];
+            st.prev_tensile[k] = double(pvd[o].x);
+            st.prev_compressive[k] = double(pvd[o].y);
+            st.prev_shear[k] = double(pvd[o].z);
+            st.damage[k] = double(pvd[o].w);
+            st.plastic_strain[k] = double(pstr[o]);
+            st.failure_mode[k] = mode[o];
         }
         sphereOut(readSphere(), sp);
     }
@@ -530,8 +659,10 @@ private:
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t cus_ = 256;
     uint32_t N_ = 0, B_ = 0, D_ = 0, iterations_ = 1;
-    uint32_t node_tiles_ = 0, bond_tiles_ = 0, bitmap_words_ = 0;
-    bool direct_ = false, plastic_ = false, uploaded_ = false, loads_set_ = false;
+    uint32_t node_tiles_ = 0, bond_tiles_ = 0, group_words_ = 0;
+    uint32_t slot_palette_ = 0, edge_palette_ = 0;
+    bool direct_ = false, plastic_ = false, uploaded_ = false, loads_set_ = false, tensile_only_ = false;
+    std::vector<uint32_t> node_j_host_;  // node-major bond position -> schedule index
     Step<R> S_{};
     double origin_[3]{};
     std::vector<R> mass_host_, inv_mass_host_;
@@ -545,10 +676,10 @@ private:
     size_t interior_stages_ = 0;
     DevBuf wg_;
     DevBuf x04_, u4_, up4_, v4_, rinv_, unm_, sa_, sb_, valid_, approach_, engaged_, cand_;
-    DevBuf rank_stamp_, dirty_list_, dirty_stamp_, cand_tile_, nbr_bond_, tile_events_, ev_j_, ev_d_, ev_e_, flag_list_;
-    DevBuf nbr_other_, nbr_rx_, nbr_ry_, nbr_rz_, nbr_w_, nbr_alive_;
-    DevBuf rec_, alive_, lambda_, pext_, pstr_, thr_, pvd_, mode_, slots_, stamp_, removed_;
-    DevBuf tile_cand_, tile_bits_, tile_plastic_, load_part_, fext_, freq_, cand_all_, cand_lo_, cand_cnt_, nbb_;
+    DevBuf rank_stamp_, dirty_list_, dirty_stamp_, cand_tile_, nbr_bond_, ev_bits_, brk_bits_, grp_bits_, ev_d_, ev_e_;
+    DevBuf flag_list_, nbr_key_, spal_, sdelta_, epal_;
+    DevBuf rec_, alive_, lambda_, pext_, bnd_, alive_n_, pstr_, thr_, thr2_, pvd_, mode_, slots_;
+    DevBuf tile_cand_, tile_plastic_, load_part_, fext_, freq_, cand_all_, cand_lo_, cand_cnt_, nbb_;
     DevBuf first_fail_, ctl_, sphere_;
     DevBuf cc_parent_, cc_root_, cc_size_, cc_rank_, cc_label_, cc_keys_, cc_sorted_, cc_count_, cc_temp_;
 
@@ -574,6 +705,7 @@ private:
         if (s.constraint_iterations == 0) throw Invalid("at least one constraint iteration");
         if (st.node_count != sch_.N || st.bond_count != sch_.B) throw Invalid("state does not match the schedule");
         if (size_t(st.max_degree) * st.node_count > 0xffffffffULL) throw Invalid("neighbour slots exceed 32-bit indices");
+        if (st.bond_count >= 0x80000000U) throw Invalid("at most 2^31 - 1 bonds");
         if (!(s.dt > 0.0) || !std::isfinite(s.dt)) throw Invalid("the substep must be finite and positive");
         if (s.sphere_enabled && (st.node_count + kWG - 1) / kWG > kMaxNodeTiles)
             throw Invalid("the striker pass holds at most 4,194,304 nodes");
@@ -652,7 +784,8 @@ private:
         L.blocks = sch_.blocks;
         L.node_tiles = node_tiles_;
         L.bond_tiles = bond_tiles_;
-        L.bitmap_words = bitmap_words_;
+        L.group_words = group_words_;
+        L.tensile_only = tensile_only_ ? 1U : 0U;
         L.u4 = u4_.as<R4<R>>();
         L.up4 = up4_.as<R4<R>>();
         L.v4 = v4_.as<R4<R>>();
@@ -665,31 +798,30 @@ private:
         L.approach = approach_.as<R>();
         L.engaged = engaged_.as<uint8_t>();
         L.cand = cand_.as<uint8_t>();
-        L.nbr_other = nbr_other_.as<uint32_t>();
-        L.nbr_rx = nbr_rx_.as<R>();
-        L.nbr_ry = nbr_ry_.as<R>();
-        L.nbr_rz = nbr_rz_.as<R>();
-        L.nbr_w = nbr_w_.as<R>();
-        L.nbr_alive = nbr_alive_.as<uint8_t>();
+        L.nbr_key = nbr_key_.as<uint32_t>();
+        L.spal = spal_.as<R4<R>>();
+        L.sdelta = sdelta_.as<int>();
+        L.nbr_bond = nbr_bond_.as<uint32_t>();
+        L.epal = epal_.as<EdgePal<R>>();
         L.rec = rec_.as<BondRec<R>>();
         L.alive = alive_.as<uint8_t>();
         L.lambda = lambda_.as<R>();
         L.pext = pext_.as<R>();
-        L.pstr = pstr_.as<R>();
+        L.bnd = bnd_.as<BondNode>();
+        L.alive_n = alive_n_.as<uint8_t>();
         L.thr = thr_.as<Thresholds<R>>();
+        L.thr2 = thr2_.as<Tensile<R>>();
         L.pvd = pvd_.as<R4<R>>();
         L.mode = mode_.as<uint8_t>();
+        L.pstr = pstr_.as<R>();
         L.slots = slots_.as<uint2>();
-        L.stamp = stamp_.as<uint32_t>();
-        L.removed = removed_.as<R>();
-        L.tile_cand = tile_cand_.as<uint32_t>();
-        L.tile_bits = tile_bits_.as<uint64_t>();
-        L.nbr_bond = nbr_bond_.as<uint32_t>();
-        L.tile_events = tile_events_.as<uint32_t>();
-        L.ev_j = ev_j_.as<uint32_t>();
+        L.ev_bits = ev_bits_.as<uint64_t>();
+        L.brk_bits = brk_bits_.as<uint64_t>();
+        L.grp_bits = grp_bits_.as<uint64_t>();
         L.ev_d = ev_d_.as<float>();
         L.ev_e = ev_e_.as<R>();
         L.flag_list = flag_list_.as<uint32_t>();
+        L.tile_cand = tile_cand_.as<uint32_t>();
         L.tile_plastic = tile_plastic_.as<double>();
         L.load_part = load_part_.as<double>();
         L.fext = fext_.as<R4<R>>();
@@ -749,7 +881,7 @@ private:
         f.alive.resize(B_);
         HIP_OK(hipMemcpy(f.alive.data(), alive_.p, B_, hipMemcpyDeviceToHost));
         f.damage.resize(B_);
-        for (uint32_t k = 0; k < B_; ++k) f.damage[k] = float(pvd[k].w);
+        for (uint32_t o = 0; o < B_; ++o) f.damage[node_j_host_[o]] = float(pvd[o].w);
         f.sphere = readSphere();
         frames_.push_back(std::move(f));
     }
@@ -837,16 +969,16 @@ private:
         if (sphere) BLX_LAUNCH(KPass1, 4, hipLaunchKernelGGL((k_pass<R, 1>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
         const uint2 *wg = wg_.as<uint2>();
         for (uint32_t it = 0; it < iterations_; ++it) {
-            const int first = it == 0;
+            const uint32_t flags = (it == 0 ? kSolveFirst : 0U) | (it + 1 < iterations_ ? kSolveKeepLambda : 0U);
             for (size_t k = 0; k < stages_.size(); ++k) {
                 const auto &st = stages_[k];
                 profBegin(KSolve, k < interior_stages_ ? 5 : 6);
                 if (plastic_)
                     hipLaunchKernelGGL((k_solve<R, DIRECT, true>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L, S,
-                                       wg, st.first, st.second, first);
+                                       wg, st.first, st.second, flags);
                 else
                     hipLaunchKernelGGL((k_solve<R, DIRECT, false>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L, S,
-                                       wg, st.first, st.second, first);
+                                       wg, st.first, st.second, flags);
                 profEnd();
             }
             if (S.plane_count > 0 && it + 1 < iterations_)

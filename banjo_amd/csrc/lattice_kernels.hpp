@@ -78,6 +78,8 @@ struct DevCtl {
 // tiles, which keeps the node data that neighbouring tiles share in one L2.
 // Launch xcdGrid(tiles) workgroups; a result >= tiles is an idle workgroup.
 constexpr u32 kXcds = 8;
+constexpr u32 kSolveFirst = 1U;      // k_solve: first constraint iteration, the multiplier starts at 0
+constexpr u32 kSolveKeepLambda = 2U; // k_solve: a later iteration reads the multiplier
 __host__ __device__ __forceinline__ u32 xcdGrid(u32 tiles) { return (tiles + kXcds - 1) / kXcds * kXcds; }
 __device__ __forceinline__ u32 xcdTile(u32 w, u32 tiles) {
     const u32 per = (tiles + kXcds - 1) / kXcds;
@@ -85,7 +87,8 @@ __device__ __forceinline__ u32 xcdTile(u32 w, u32 tiles) {
 }
 
 template <class R> struct DevLattice {
-    u32 N, B, D, blocks, node_tiles, bond_tiles, bitmap_words;
+    u32 N, B, D, blocks, node_tiles, bond_tiles, group_words;
+    u32 tensile_only; // every bond's compression and shear ramps are off: thr2 replaces thr
     R4<R> *u4;        // displacement, w = inv_mass
     R4<R> *up4;       // displacement at substep start
     R4<R> *v4;        // velocity
@@ -97,30 +100,37 @@ template <class R> struct DevLattice {
     R *approach;      // 2 planes of N
     u8 *engaged;      // 2 planes of N
     u8 *cand;         // N
-    const u32 *nbr_other; // D planes of N (k-major), kNone where no bond
-    const R *nbr_rx, *nbr_ry, *nbr_rz, *nbr_w;
-    u8 *nbr_alive;
+    // Neighbour slots, D planes of N (k-major): slot palette index | kDeadSlot, kNone where absent.
+    u32 *nbr_key;
+    const R4<R> *spal;   // slot palette: rest vector to the neighbour, weight
+    const int *sdelta;   // slot palette: neighbour index minus node index
+    const u32 *nbr_bond; // D planes of N: node-major index of the slot's bond (start sample)
+    const EdgePal<R> *epal;
+    // Bonds in schedule order (the colour sweep).
     const BondRec<R> *rec;
     u8 *alive;
-    R *lambda, *pext, *pstr;
+    R *lambda, *pext;
+    // Bonds in node-major order (the end sample).
+    const BondNode *bnd;
+    u8 *alive_n;      // alive again, in node-major order (written together with alive)
     const Thresholds<R> *thr;
+    const Tensile<R> *thr2;
     R4<R> *pvd;       // prev tensile, compressive, shear; damage
     u8 *mode;
+    R *pstr;
     const uint2 *slots;
-    u32 *stamp;       // substep (+1, low 32 bits) a bond broke in
-    R *removed;       // stored energy of a bond at its break
-    const u32 *nbr_bond;  // D planes of N: schedule index of the slot's bond (start sample)
-    u32 *tile_cand;   // candidates per node tile
-    u64 *tile_bits;   // bond tiles with events this substep
-    u32 *tile_events; // events per bond tile
-    u32 *ev_j;        // per tile, its events in bond order: schedule index, bit 31 = broke
+    // Substep events by schedule index: a bit per bond, a summary bit per 64 words.
+    u64 *ev_bits;     // the bond broke, or its damage may raise the running maximum
+    u64 *brk_bits;    // the bond broke
+    u64 *grp_bits;    // words of ev_bits that hold a set bit
     float *ev_d;      // the event bond's damage as the float the fold compares
     R *ev_e;          // stored energy removed by a break
-    u32 *flag_list;   // scratch: flagged tiles in order
+    u32 *flag_list;   // scratch: flagged groups in order
+    u32 *tile_cand;   // candidates per node tile
     u32 *dirty_list;  // nodes a break made dirty, each once per substep (unordered)
     u32 *dirty_stamp; // substep stamp of a node's entry in dirty_list
     u32 *rank_stamp;  // break round a node was last counted rank-deficient in
-    double *tile_plastic; // plastic work per bond tile
+    double *tile_plastic; // plastic work per end-sample tile
     double *load_part;    // ledger partial sums per node tile (13 each)
     R4<R> *fext;      // applied external force in backend precision, w = 1 where loaded
     const double *freq; // requested external force (3N double)
@@ -152,12 +162,13 @@ template <class R> __device__ void rinvNode(const DevLattice<R> &L, u32 i, u32 *
     R cov[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     u32 live = 0;
     for (u32 k = 0; k < D; ++k) {
-        const size_t slot = size_t(k) * N + i;
-        if (L.nbr_other[slot] == kNone) break;
-        if (!L.nbr_alive[slot]) continue;
-        const R w = L.nbr_w[slot];
+        const u32 key = L.nbr_key[size_t(k) * N + i];
+        if (key == kNone) break;
+        if (key & kDeadSlot) continue;
+        const R4<R> rw = L.spal[key];
+        const R w = rw.w;
         if (w <= R(0)) continue;
-        const R rr[3] = {L.nbr_rx[slot], L.nbr_ry[slot], L.nbr_rz[slot]};
+        const R rr[3] = {rw.x, rw.y, rw.z};
         for (int r = 0; r < 3; ++r)
             for (int q = 0; q < 3; ++q) cov[r * 3 + q] += w * rr[r] * rr[q];
         ++live;
@@ -192,28 +203,28 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
     // which stay in slot order.
     constexpr u32 kChunk = 8;
     for (u32 k0 = 0; k0 < D; k0 += kChunk) {
+        u32 key[kChunk];
+        for (u32 q = 0; q < kChunk; ++q) key[q] = k0 + q < D ? L.nbr_key[size_t(k0 + q) * N + i] : kNone;
+        R4<R> rw[kChunk];
         u32 other[kChunk];
-        bool live[kChunk];
         for (u32 q = 0; q < kChunk; ++q) {
-            const u32 k = k0 + q;
-            const size_t slot = size_t(k) * N + i;
-            other[q] = k < D ? L.nbr_other[slot] : kNone;
-            live[q] = other[q] != kNone && L.nbr_alive[slot];
+            const bool live = !(key[q] & kDeadSlot);
+            rw[q] = live ? L.spal[key[q]] : R4<R>{R(0), R(0), R(0), R(0)};
+            other[q] = live ? u32(int(i) + L.sdelta[key[q]]) : i;
         }
         R4<R> xo4[kChunk];
-        for (u32 q = 0; q < kChunk; ++q) xo4[q] = live[q] ? L.u4[other[q]] : R4<R>{R(0), R(0), R(0), R(0)};
+        for (u32 q = 0; q < kChunk; ++q) xo4[q] = !(key[q] & kDeadSlot) ? L.u4[other[q]] : R4<R>{R(0), R(0), R(0), R(0)};
         for (u32 q = 0; q < kChunk; ++q) {
-            if (!live[q]) continue;
-            const size_t slot = size_t(k0 + q) * N + i;
-            const R w = L.nbr_w[slot];
-            const V3<R> rest{L.nbr_rx[slot], L.nbr_ry[slot], L.nbr_rz[slot]};
+            if (key[q] & kDeadSlot) continue;
+            const R w = rw[q].w;
+            const V3<R> rest = xyz(rw[q]);
             const V3<R> xo = nodePos<R, DIRECT>(L, other[q], xo4[q]);
             const V3<R> cur = DIRECT ? (xo - xi) - rest : xo - xi;
             const R cc[3] = {cur.x, cur.y, cur.z}, rr[3] = {rest.x, rest.y, rest.z};
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) acc[r * 3 + c] += w * cc[r] * rr[c];
         }
-        if (other[kChunk - 1] == kNone) break;
+        if (key[kChunk - 1] == kNone) break;
     }
     R inv[9], un[3], e[6];
     for (int k = 0; k < 9; ++k) inv[k] = L.rinv[size_t(k) * N + i];
@@ -261,33 +272,35 @@ template <class R> __global__ __launch_bounds__(kWG) void k_rinv(DevLattice<R> L
 // Bonds: the strain sample (LatticePhysics.hpp:924-959).
 
 template <class R, bool DIRECT>
-__device__ __forceinline__ V3<R> bondVec(const DevLattice<R> &L, const BondRec<R> &r, const R4<R> &ua, const R4<R> &ub) {
-    if (DIRECT) return (xyz(L.x04[r.b]) + xyz(ub)) - (xyz(L.x04[r.a]) + xyz(ua));
-    return V3<R>{r.ex, r.ey, r.ez} + (xyz(ub) - xyz(ua));
+__device__ __forceinline__ V3<R> bondVec(const DevLattice<R> &L, u32 a, u32 b, const R4<R> &e, const R4<R> &ua,
+                                         const R4<R> &ub) {
+    if (DIRECT) return (xyz(L.x04[b]) + xyz(ub)) - (xyz(L.x04[a]) + xyz(ua));
+    return xyz(e) + (xyz(ub) - xyz(ua));
 }
 
+// The strain sample of one bond (LatticePhysics.hpp:924-959). The unit rest
+// direction comes from the edge palette, where it was computed as
+// e * (1 / |e|) by the same expression.
 template <class R, bool DIRECT>
-__device__ void sampleBond(const DevLattice<R> &L, const BondRec<R> &r, const R4<R> &ua, const R4<R> &ub, R &t, R &c,
-                           R &s) {
-    const V3<R> delta = bondVec<R, DIRECT>(L, r, ua, ub);
+__device__ void sampleBond(const DevLattice<R> &L, u32 a, u32 b, u32 p, const EdgePal<R> &g, const R4<R> &ua,
+                           const R4<R> &ub, R &t, R &c, R &s) {
+    const V3<R> delta = bondVec<R, DIRECT>(L, a, b, g.e, ua, ub);
     const R len = length(delta);
-    const R rest = r.L;
-    const V3<R> e{r.ex, r.ey, r.ez};
+    const R rest = g.e.w;
     R stretch;
     if (DIRECT) {
         stretch = len / rest - R(1);
     } else {
         const V3<R> du = xyz(ub) - xyz(ua);
-        const R l2 = r.rsm + R(2) * dot(e, du) + dot(du, du);
+        const R l2 = g.d.x + R(2) * dot(xyz(g.e), du) + dot(du, du);
         stretch = l2 / ((len + rest) * rest);
     }
     t = stretch;
     c = -stretch;
     s = R(0);
-    const R rl = length(e);
-    if (rl > R(1.0e-9)) {
-        const V3<R> dir = e * (R(1) / rl);
-        const u32 ends[2] = {r.a, r.b};
+    if (!(p & kNoDir)) {
+        const V3<R> dir{g.d.y, g.d.z, g.d.w};
+        const u32 ends[2] = {a, b};
         for (int q = 0; q < 2; ++q) {
             const R4<R> sb = L.sb[ends[q]];
             if (sb.z == R(0)) continue;
@@ -301,6 +314,21 @@ __device__ void sampleBond(const DevLattice<R> &L, const BondRec<R> &r, const R4
     }
 }
 
+// The edge palette's derived terms: |e| and the unit direction
+// (LatticePhysics.hpp:944-946), one entry per thread at upload.
+template <class R> __global__ void k_edge_palette(EdgePal<R> *pal, u32 *no_dir, u32 n) {
+    for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        EdgePal<R> g = pal[k];
+        const V3<R> e = xyz(g.e);
+        const R rl = length(e);
+        const bool has = rl > R(1.0e-9);
+        const V3<R> dir = has ? e * (R(1) / rl) : V3<R>{R(0), R(0), R(0)};
+        g.d = R4<R>{g.d.x, dir.x, dir.y, dir.z};
+        pal[k] = g;
+        no_dir[k] = has ? 0U : 1U;
+    }
+}
+
 // The start sample of the live bonds at dirty nodes (LatticePhysics.hpp:1118-1126).
 // A bond with no dirty end samples exactly what its end sample stored, so
 // only the bonds in dirty nodes' slots are resampled; a bond with both ends
@@ -308,18 +336,18 @@ __device__ void sampleBond(const DevLattice<R> &L, const BondRec<R> &r, const R4
 template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_start(DevLattice<R> L) {
     if (L.ctl->stop || !L.ctl->dirty_start) return;
     const u64 pairs = u64(L.ctl->dirty_count) * L.D;
-    for (u64 p = u64(blockIdx.x) * kWG + threadIdx.x; p < pairs; p += u64(gridDim.x) * kWG) {
-        const u32 i = L.dirty_list[p / L.D], k = u32(p % L.D);
-        const u32 j = L.nbr_bond[size_t(k) * L.N + i];
-        if (j == kNone || !L.alive[j]) continue;
-        const BondRec<R> r = L.rec[j];
+    for (u64 q = u64(blockIdx.x) * kWG + threadIdx.x; q < pairs; q += u64(gridDim.x) * kWG) {
+        const u32 i = L.dirty_list[q / L.D], k = u32(q % L.D);
+        const u32 o = L.nbr_bond[size_t(k) * L.N + i];
+        if (o == kNone || !L.alive_n[o]) continue;
+        const BondNode r = L.bnd[o];
         R t, c, s;
-        sampleBond<R, DIRECT>(L, r, L.u4[r.a], L.u4[r.b], t, c, s);
-        R4<R> pv = L.pvd[j];
+        sampleBond<R, DIRECT>(L, r.a, r.b, r.p, L.epal[r.p & ~kNoDir], L.u4[r.a], L.u4[r.b], t, c, s);
+        R4<R> pv = L.pvd[o];
         pv.x = maxR(R(0), t);
         pv.y = maxR(R(0), c);
         pv.z = maxR(R(0), s);
-        L.pvd[j] = pv;
+        L.pvd[o] = pv;
     }
 }
 
@@ -751,13 +779,14 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R
 
 template <class R, bool DIRECT, bool PLASTIC>
 __global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const uint2 *wg_range, u32 wg_base,
-                                                u32 wg_count, int first) {
+                                                u32 wg_count, u32 flags) {
     if (L.ctl->stop) return;
     const u32 w = xcdTile(blockIdx.x, wg_count);
     if (w >= wg_count) return;
     const uint2 rg = wg_range[wg_base + w];
     const u32 j = rg.x + threadIdx.x;
     if (j >= rg.y) return;
+    const bool first = flags & kSolveFirst;
     // The record is loaded with the alive flag rather than after it, so the
     // node gathers wait on one memory round trip, not two.
     const BondRec<R> r = L.rec[j];
@@ -765,26 +794,29 @@ __global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const
     const R lam = first ? R(0) : L.lambda[j];
     if (!live) return;
     const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
+    const EdgePal<R> &g = L.epal[r.p & ~kNoDir];
+    const R4<R> ge = g.e;
     const V3<R> du = xyz(ub) - xyz(ua);
-    const V3<R> e{r.ex, r.ey, r.ez};
+    const V3<R> e = xyz(ge);
     const V3<R> delta = DIRECT ? (xyz(L.x04[r.b]) + xyz(ub)) - (xyz(L.x04[r.a]) + xyz(ua)) : e + du;
     const R len = length(delta);
     if (len <= R(1.0e-12)) return;
-    const R orig = r.L;
+    const R orig = ge.w;
     R C;
     if (PLASTIC) {
         const R plastic = L.pext[j];
         const R rest = orig + plastic;
-        C = DIRECT ? len - rest : (r.rsm + R(2) * dot(e, du) + dot(du, du) - plastic * (rest + orig)) / (len + rest);
+        C = DIRECT ? len - rest : (g.d.x + R(2) * dot(e, du) + dot(du, du) - plastic * (rest + orig)) / (len + rest);
     } else {
         // plastic == 0: rest == orig and the subtracted term is an exact +0.
-        C = DIRECT ? len - orig : (r.rsm + R(2) * dot(e, du) + dot(du, du)) / (len + orig);
+        C = DIRECT ? len - orig : (g.d.x + R(2) * dot(e, du) + dot(du, du)) / (len + orig);
     }
     const V3<R> dir = delta / len;
     const R wa = ua.w, wb = ub.w;
     const R alpha = r.c / (S.dt * S.dt);
     const R dl = (-C - alpha * lam) / (wa + wb + alpha);
-    L.lambda[j] = lam + dl;
+    // The multiplier is read only by a later iteration of the same substep.
+    if (flags & kSolveKeepLambda) L.lambda[j] = lam + dl;
     L.u4[r.a] = with(xyz(ua) - wa * dl * dir, ua.w);
     L.u4[r.b] = with(xyz(ub) + wb * dl * dir, ub.w);
 }
@@ -800,7 +832,7 @@ __global__ __launch_bounds__(kWG) void k_damp(DevLattice<R> L, Step<R> S, const 
     const BondRec<R> r = L.rec[j];
     const R ma = L.x04[r.a].w, mb = L.x04[r.b].w;
     if (ma <= R(0) || mb <= R(0)) return;
-    const V3<R> n = normalized(bondVec<R, DIRECT>(L, r, L.u4[r.a], L.u4[r.b]));
+    const V3<R> n = normalized(bondVec<R, DIRECT>(L, r.a, r.b, L.epal[r.p & ~kNoDir].e, L.u4[r.a], L.u4[r.b]));
     const V3<R> va = xyz(L.v4[r.a]), vb = xyz(L.v4[r.b]);
     const R rs = dot(vb - va, n);
     const R im = R(1) / ma + R(1) / mb;
@@ -880,8 +912,11 @@ template <class R> __global__ __launch_bounds__(kWG) void k_support_velocity(Dev
 
 // ---------------------------------------------------------------------------
 // End-of-substep sample, damage and break (LatticePhysics.hpp:1068-1114).
-// One bond tile per workgroup iteration; a tile with a break, or with a damage
-// that could raise the running maximum, sets its bit for the ordered folds.
+// The bonds are walked in node-major order, so a workgroup's bonds share their
+// nodes and the node gathers hit in cache (schedule order is colour-major and
+// would sweep all node data once per colour). A bond whose break or damage
+// could change the ordered folds sets its bit in ev_bits, by schedule index,
+// so k_finish reads the events back in bond order.
 
 template <class R, bool DIRECT, bool PLASTIC>
 __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
@@ -889,7 +924,6 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
     if (ctl->stop) return;
     __shared__ float s_max[4][kWG / 64];
     __shared__ double s_pw[kWG];
-    __shared__ u32 s_wave_events[kWG / 64];
     const double m0 = ctl->max_damage;
     const u32 stamp = u32(ctl->step) + 1U;
     const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
@@ -901,51 +935,58 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
     const u32 tile = xcdTile(blockIdx.x, L.bond_tiles);
     if (tile >= L.bond_tiles) return;
     {
-        const u32 j = tile * kWG + threadIdx.x;
-        bool event = false, broke = false;
+        const u32 o = tile * kWG + threadIdx.x;
         u32 claim_a = kNone, claim_b = kNone;
-        float event_damage = 0.f;
-        R event_energy = R(0);
         double pw = 0.0;
         // The bond's streamed data is loaded together with its alive flag,
         // so the node gathers wait on one memory round trip.
-        const bool in_range = j < L.B;
-        BondRec<R> r{};
+        const bool in_range = o < L.B;
+        BondNode r{};
         R4<R> pv{};
         Thresholds<R> th{};
         u8 live = 0;
         if (in_range) {
-            r = L.rec[j];
-            pv = L.pvd[j];
-            th = L.thr[j];
-            live = L.alive[j];
+            r = L.bnd[o];
+            pv = L.pvd[o];
+            live = L.alive_n[o];
+            if (L.tensile_only) {
+                const Tensile<R> t2 = L.thr2[o];
+                th.ds = t2.ds;
+                th.de = t2.de;
+            } else {
+                th = L.thr[o];
+            }
         }
         if (live) {
+            const u32 j = r.j;
             const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
+            const EdgePal<R> g = L.epal[r.p & ~kNoDir];
+            const R restL = g.e.w;
             R plastic = R(0);
             if (PLASTIC) {
                 // bondPlasticReturn (LatticePhysics.hpp:1018-1039).
                 plastic = L.pext[j];
-                R ps = L.pstr[j];
-                const R elastic = length(bondVec<R, DIRECT>(L, r, ua, ub)) - r.L - plastic;
+                R ps = L.pstr[o];
+                const R elastic = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
                 const R mag = absR(elastic);
                 const R h = S.hardening;
-                const R ye = S.yield * r.L + h * ps;
+                const R ye = S.yield * restL + h * ps;
                 if (mag > ye) {
                     const R inc = (mag - ye) / (R(1) + h);
                     plastic = plastic + (elastic < R(0) ? -inc : inc);
                     ps = ps + inc;
                     L.pext[j] = plastic;
-                    L.pstr[j] = ps;
-                    if (r.c > R(0)) {
+                    L.pstr[o] = ps;
+                    const R c = L.rec[j].c;
+                    if (c > R(0)) {
                         const R yend = ye + h * inc;
-                        pw = double(R(0.5) * (ye + yend) * inc / r.c);
+                        pw = double(R(0.5) * (ye + yend) * inc / c);
                     }
                 }
-                mp = fmaxf(mp, float(absR(plastic) / r.L));
+                mp = fmaxf(mp, float(absR(plastic) / restL));
             }
             R t, c, s;
-            sampleBond<R, DIRECT>(L, r, ua, ub, t, c, s);
+            sampleBond<R, DIRECT>(L, r.a, r.b, r.p, g, ua, ub, t, c, s);
             const R pt = maxR(R(0), maxR(pv.x, t)), pc = maxR(R(0), maxR(pv.y, c)), pss = maxR(R(0), maxR(pv.z, s));
             mt = fmaxf(mt, float(pt));
             mc = fmaxf(mc, float(pc));
@@ -953,39 +994,52 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
             pv.x = maxR(R(0), t);
             pv.y = maxR(R(0), c);
             pv.z = maxR(R(0), s);
-            const R td = progress(pt, th.ds, th.de), cd = progress(pc, th.cs, th.ce), sd = progress(pss, th.ss, th.se);
+            // With every compression and shear start non-finite their ramps
+            // are an exact 0 (damageProgress), which never wins below.
+            const R td = progress(pt, th.ds, th.de);
+            const R cd = L.tensile_only ? R(0) : progress(pc, th.cs, th.ce);
+            const R sd = L.tensile_only ? R(0) : progress(pss, th.ss, th.se);
             R dmg = td;
             u8 md = 1;
             if (cd > dmg) { dmg = cd; md = 2; }
             if (sd > dmg) { dmg = sd; md = 3; }
             if (dmg > pv.w) {
                 pv.w = dmg;
-                L.mode[j] = md;
+                L.mode[o] = md;
             }
-            L.pvd[j] = pv;
-            event_damage = float(pv.w);
-            event = double(event_damage) > m0 + 1.0e-6;
-            if (pv.w >= R(1)) {
-                const R ext = length(bondVec<R, DIRECT>(L, r, ua, ub)) - r.L - plastic;
-                event_energy = r.c <= R(0) ? R(0) : R(0.5) * ext * ext / r.c;
-                L.removed[j] = event_energy;
-                broke = true;
+            L.pvd[o] = pv;
+            const float event_damage = float(pv.w);
+            bool event = double(event_damage) > m0 + 1.0e-6;
+            R event_energy = R(0);
+            const bool broke = pv.w >= R(1);
+            if (broke) {
+                const R ext = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
+                const R cc = L.rec[j].c;
+                event_energy = cc <= R(0) ? R(0) : R(0.5) * ext * ext / cc;
                 L.alive[j] = 0;
-                const uint2 sl = L.slots[j];
-                L.nbr_alive[sl.x] = 0;
-                L.nbr_alive[sl.y] = 0;
-                L.stamp[j] = stamp;
+                L.alive_n[o] = 0;
+                const uint2 sl = L.slots[o];
+                L.nbr_key[sl.x] |= kDeadSlot;
+                L.nbr_key[sl.y] |= kDeadSlot;
                 claim_a = atomicExch(&L.dirty_stamp[r.a], stamp) != stamp ? r.a : kNone;
                 claim_b = atomicExch(&L.dirty_stamp[r.b], stamp) != stamp ? r.b : kNone;
                 event = true;
+            }
+            if (event) {
+                L.ev_d[j] = event_damage;
+                L.ev_e[j] = event_energy;
+                const unsigned long long bit = 1ULL << (j & 63U);
+                atomicOr((unsigned long long *)&L.ev_bits[j >> 6], bit);
+                if (broke) atomicOr((unsigned long long *)&L.brk_bits[j >> 6], bit);
+                atomicOr((unsigned long long *)&L.grp_bits[j >> 12], 1ULL << ((j >> 6) & 63U));
             }
         }
         {
             const u32 mine = (claim_a != kNone) + (claim_b != kNone);
             u32 x = mine;
-            for (u32 o = 1; o < 64; o <<= 1) {
-                const u32 y = __shfl_up(x, o);
-                if (lane >= o) x += y;
+            for (u32 q = 1; q < 64; q <<= 1) {
+                const u32 y = __shfl_up(x, q);
+                if (lane >= q) x += y;
             }
             const u32 wave_total = __shfl(x, 63);
             u32 base = 0;
@@ -995,27 +1049,6 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
             if (claim_a != kNone) L.dirty_list[at++] = claim_a;
             if (claim_b != kNone) L.dirty_list[at] = claim_b;
         }
-        // Ordered compaction of the tile's events: wave ballots, then the
-        // waves' counts in order, so the list is in bond (schedule) order.
-        const u64 ballot = __ballot(event);
-        if (lane == 0) s_wave_events[wave] = u32(__popcll(ballot));
-        __syncthreads();
-        u32 before = 0, total = 0;
-        for (u32 w = 0; w < kWG / 64; ++w) {
-            before += w < wave ? s_wave_events[w] : 0U;
-            total += s_wave_events[w];
-        }
-        if (event) {
-            const u32 pos = tile * kWG + before + u32(__popcll(ballot & ((1ULL << lane) - 1ULL)));
-            L.ev_j[pos] = j | (broke ? 0x80000000U : 0U);
-            L.ev_d[pos] = event_damage;
-            L.ev_e[pos] = event_energy;
-        }
-        if (threadIdx.x == 0 && total) {
-            L.tile_events[tile] = total;
-            atomicOr((unsigned long long *)&L.tile_bits[tile >> 6], 1ULL << (tile & 63U));
-        }
-        __syncthreads();
         if (PLASTIC) {
             s_pw[threadIdx.x] = pw;
             __syncthreads();
@@ -1028,11 +1061,11 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
         }
     }
     // Running maxima of non-negative floats: integer atomicMax on the bits.
-    for (int o = 32; o > 0; o >>= 1) {
-        mt = fmaxf(mt, __shfl_xor(mt, o));
-        mc = fmaxf(mc, __shfl_xor(mc, o));
-        ms = fmaxf(ms, __shfl_xor(ms, o));
-        mp = fmaxf(mp, __shfl_xor(mp, o));
+    for (int q = 32; q > 0; q >>= 1) {
+        mt = fmaxf(mt, __shfl_xor(mt, q));
+        mc = fmaxf(mc, __shfl_xor(mc, q));
+        ms = fmaxf(ms, __shfl_xor(ms, q));
+        mp = fmaxf(mp, __shfl_xor(mp, q));
     }
     if (lane == 0) {
         s_max[0][wave] = mt;
@@ -1119,13 +1152,28 @@ __device__ double fixedOrderSum(const double *part, u32 count, u32 stride, u32 o
 
 constexpr u32 kEventBatch = 4096;
 
+// Position of the k-th (from 0) set bit of m; k < popcount(m).
+__device__ __forceinline__ u32 selectBit(u64 m, u32 k) {
+    u32 pos = 0;
+    for (u32 half = 32; half > 0; half >>= 1) {
+        const u32 c = u32(__popcll(m & ((1ULL << half) - 1ULL)));
+        if (k >= c) {
+            k -= c;
+            m >>= half;
+            pos += half;
+        }
+    }
+    return pos;
+}
+
 template <class R, bool LOADS, bool PLASTIC>
 __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step<R> S) {
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
     __shared__ double s_red[kFinishThreads];
     __shared__ u32 s_waves[kFinishThreads / 64];
-    __shared__ u32 s_tile[kFinishThreads], s_incl[kFinishThreads];
+    __shared__ u32 s_word[kFinishThreads], s_incl[kFinishThreads];
+    __shared__ u64 s_evw[kFinishThreads], s_bkw[kFinishThreads];
     __shared__ u32 s_j[kEventBatch];
     __shared__ float s_d[kEventBatch];
     __shared__ R s_e[kEventBatch];
@@ -1139,34 +1187,46 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step
     const bool first_round = ctl->broken_bonds == 0;
     u64 ff = ctl->first_failure_count;
     const double flat = ctl->flat_fraction;
-    for (u32 wbase = 0; wbase < L.bitmap_words; wbase += kFinishThreads) {
-        // Flagged tiles of these bitmap words, in order, into flag_list.
-        const u32 w = wbase + tid;
+    for (u32 gbase = 0; gbase < L.group_words; gbase += kFinishThreads) {
+        // Event words flagged in these summary words, in order, into flag_list.
+        const u32 g = gbase + tid;
         u64 bits = 0;
-        if (w < L.bitmap_words) {
-            bits = L.tile_bits[w];
-            if (bits) L.tile_bits[w] = 0;
+        if (g < L.group_words) {
+            bits = L.grp_bits[g];
+            if (bits) L.grp_bits[g] = 0;
         }
         u32 nflag = 0;
         u32 at = blockScan1024(u32(__popcll(bits)), s_waves, &nflag);
         while (bits) {
-            L.flag_list[at++] = w * 64U + u32(__ffsll((long long)bits) - 1);
+            L.flag_list[at++] = g * 64U + u32(__ffsll((long long)bits) - 1);
             bits &= bits - 1;
         }
         __syncthreads();
         for (u32 fb = 0; fb < nflag;) {
-            // As many whole tiles as fit in the event batch.
+            // As many whole event words as fit in the event batch; a word holds
+            // at most 64 events, so at least one always fits.
             const u32 mcount = min(u32(kFinishThreads), nflag - fb);
-            const u32 t = tid < mcount ? L.flag_list[fb + tid] : 0U;
-            const u32 c = tid < mcount ? L.tile_events[t] : 0U;
+            const u32 wd = tid < mcount ? L.flag_list[fb + tid] : 0U;
+            u64 evw = 0, bkw = 0;
+            if (tid < mcount) {
+                evw = L.ev_bits[wd];
+                bkw = L.brk_bits[wd];
+            }
+            const u32 c = u32(__popcll(evw));
             u32 all = 0;
             const u32 incl = blockScan1024(c, s_waves, &all) + c;
-            s_tile[tid] = t;
+            s_word[tid] = wd;
             s_incl[tid] = incl;
+            s_evw[tid] = evw;
+            s_bkw[tid] = bkw;
             __syncthreads();
             if (tid < mcount && incl <= kEventBatch && (tid + 1 == mcount || s_incl[tid + 1] > kEventBatch)) s_take = tid + 1;
             __syncthreads();
             const u32 take = s_take;
+            if (tid < take) {
+                L.ev_bits[wd] = 0;
+                L.brk_bits[wd] = 0;
+            }
             const u32 nev = s_incl[take - 1];
             for (u32 e = tid; e < nev; e += kFinishThreads) {
                 u32 lo = 0, hi = take;
@@ -1175,10 +1235,11 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step
                     if (s_incl[mid] <= e) lo = mid + 1;
                     else hi = mid;
                 }
-                const u32 slot = s_tile[lo] * kWG + (e - (lo ? s_incl[lo - 1] : 0U));
-                s_j[e] = L.ev_j[slot];
-                s_d[e] = L.ev_d[slot];
-                s_e[e] = L.ev_e[slot];
+                const u32 pos = selectBit(s_evw[lo], e - (lo ? s_incl[lo - 1] : 0U));
+                const u32 j = s_word[lo] * 64U + pos;
+                s_j[e] = j | (((s_bkw[lo] >> pos) & 1ULL) ? 0x80000000U : 0U);
+                s_d[e] = L.ev_d[j];
+                s_e[e] = L.ev_e[j];
             }
             __syncthreads();
             if (tid < 64) {
