@@ -609,7 +609,13 @@ template <class R, int PASS>
 __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R> S) {
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
-    __shared__ u32 s_prefix[kMaxNodeTiles];
+    // Pass 1's tile prefix is dead once the candidate lists are staged, so the
+    // candidates' reference positions reuse its memory.
+    constexpr size_t kPrefixBytes = PASS == 1 ? kMaxNodeTiles * sizeof(u32) : 0;
+    constexpr size_t kX0Bytes = kCap * sizeof(R4<R>);
+    __shared__ __attribute__((aligned(16))) unsigned char s_raw[kPrefixBytes > kX0Bytes ? kPrefixBytes : kX0Bytes];
+    u32 *s_prefix = reinterpret_cast<u32 *>(s_raw);
+    R4<R> *s_x0 = reinterpret_cast<R4<R> *>(s_raw);
     __shared__ u32 s_waves[kFinishThreads / 64];
     __shared__ u32 s_ids[kCap];
     __shared__ R4<R> s_u[kCap], s_v[kCap];
@@ -683,6 +689,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R
             s_ids[k] = i;
             s_u[k] = L.u4[i];
             s_v[k] = L.v4[i];
+            s_x0[k] = L.x04[i];
         }
         __syncthreads();
         if (tid < 64) {
@@ -693,7 +700,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R
                 V3<R> x0{R(0), R(0), R(0)}, mu{R(0), R(0), R(0)}, mv{R(0), R(0), R(0)};
                 R mass = R(0);
                 if (have) {
-                    const R4<R> x = L.x04[s_ids[k]];
+                    const R4<R> x = s_x0[k];
                     x0 = xyz(x);
                     mass = x.w;
                     mu = xyz(s_u[k]);
@@ -702,21 +709,36 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R
                 const Spec<R> q = speculate(S, sp, x0, mu);
                 const u32 span = min(64U, n - k0);
                 u32 next = k0 + span;
-                // A candidate that fails the contact test (LatticePhysics.hpp:1341)
-                // changes nothing, and the test reads only its own terms and the
-                // sphere's velocity and spin: every lane evaluates it at the
-                // current sphere state, and the chain jumps to the first lane
-                // that passes. After a contact the state may have changed, so the
-                // remaining lanes are evaluated again.
+                // The impulse denominator of contactNode, a per-candidate constant.
+                const R den = R(1) / mass + R(1) / sp.mass + R(1) / sp.inertia * q.crn;
+                // A candidate changes nothing but the maximum penetration unless
+                // it passes the contact test (LatticePhysics.hpp:1341) and then
+                // takes an impulse or, in pass 2, a position correction. Both
+                // read only its own terms and the sphere's velocity and spin, so
+                // every lane evaluates them at the current sphere state and the
+                // chain jumps to the first lane that acts; the lanes before it
+                // fold their penetration into the running maximum in order. An
+                // impulse changes the sphere state, so the remaining lanes are
+                // evaluated again.
                 u32 pos = 0;
                 while (pos < span) {
                     const V3<R> rel = mv - sp.velocity - cross(sp.omega, q.r);
                     const R vn = dot(rel, q.n);
-                    const bool passes = lane >= pos && lane < span && !(mass <= R(0)) &&
-                                        !(q.gap > maxR(S.c_margin, -vn * S.dt));
-                    const u64 m = __ballot(passes);
-                    if (m == 0) break;
-                    const u32 w = u32(__ffsll((long long)m) - 1);
+                    const bool touch = lane >= pos && lane < span && !(mass <= R(0)) &&
+                                       !(q.gap > maxR(S.c_margin, -vn * S.dt));
+                    const R rest = -vn > S.c_rest_thr ? S.c_rest : R(0);
+                    const R dvn = q.gap > S.c_margin ? -q.gap / S.dt : -rest * minR(vn, R(0));
+                    const R ni = maxR(R(0), (dvn - vn) / den);
+                    const bool acts = touch && (ni > R(0) || (PASS == 2 && q.gap < -S.c_margin));
+                    const u64 am = __ballot(acts);
+                    const u32 w = am ? u32(__ffsll((long long)am) - 1) : span;
+                    u64 pen = __ballot(touch && lane < w && -q.gap > R(acc.max_pen));
+                    while (pen) {
+                        const u32 l = u32(__ffsll((long long)pen) - 1);
+                        acc.max_pen = double(laneValue(-q.gap, l));
+                        pen = __ballot(touch && lane > l && lane < w && -q.gap > R(acc.max_pen));
+                    }
+                    if (am == 0) break;
                     Spec<R> qw;
                     qw.r = laneValue(q.r, w);
                     qw.n = laneValue(q.n, w);
@@ -1242,6 +1264,8 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step
                 s_e[e] = L.ev_e[j];
             }
             __syncthreads();
+            // Wave 0 folds the staged batch in bond order, 64 events at a time;
+            // the sequential steps broadcast one lane's values with readlane.
             if (tid < 64) {
                 for (u32 c0 = 0; c0 < nev; c0 += 64) {
                     const u32 e = c0 + lane;
@@ -1252,10 +1276,10 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step
                     // max_damage rises only by more than 1e-6 (CpuLatticeBackend.cpp:462-465).
                     u64 cand = __ballot(ok && double(dmg) > m + 1.0e-6);
                     while (cand) {
-                        const int l = __ffsll((long long)cand) - 1;
-                        m = double(__shfl(dmg, l));
+                        const u32 l = u32(__ffsll((long long)cand) - 1);
+                        m = double(laneValue(dmg, l));
                         damage_gain = true;
-                        cand = __ballot(ok && int(lane) > l && double(dmg) > m + 1.0e-6);
+                        cand = __ballot(ok && lane > l && double(dmg) > m + 1.0e-6);
                     }
                     // Removed energy summed in bond order (CpuLatticeBackend.cpp:466-477).
                     const u64 bmask = __ballot(ok && (jr >> 31));
@@ -1265,9 +1289,9 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step
                     broken_now += u32(__popcll(bmask));
                     u64 left = bmask;
                     while (left) {
-                        const int l = __ffsll((long long)left) - 1;
+                        const u32 l = u32(__ffsll((long long)left) - 1);
                         left &= left - 1;
-                        const double el = __shfl(en, l);
+                        const double el = laneValue(en, l);
                         removed += el;
                         if (el > flat * (removed > 1.0e-12 ? removed : 1.0e-12)) energy_gain = true;
                     }
