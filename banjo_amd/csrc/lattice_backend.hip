@@ -152,8 +152,12 @@ public:
         HIP_OK(hipSetDevice(device_));
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&early_, hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&ev_bond_end_, hipEventDisableTiming));
         HIP_OK(hipEventCreateWithFlags(&ev_rinv_, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&ev_pass1_, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&ev_velocity_, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&ev_early_, hipEventDisableTiming));
         HIP_OK(hipEventCreate(&ev0_));
         HIP_OK(hipEventCreate(&ev1_));
         hipDeviceProp_t prop{};
@@ -170,6 +174,10 @@ public:
         if (side_) (void)hipStreamSynchronize(side_);
         if (ev_bond_end_) (void)hipEventDestroy(ev_bond_end_);
         if (ev_rinv_) (void)hipEventDestroy(ev_rinv_);
+        if (early_) (void)hipStreamSynchronize(early_);
+        for (hipEvent_t e : {ev_pass1_, ev_velocity_, ev_early_})
+            if (e) (void)hipEventDestroy(e);
+        if (early_) (void)hipStreamDestroy(early_);
         if (side_) (void)hipStreamDestroy(side_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
@@ -367,6 +375,8 @@ public:
         zero<double>(freq_, 3 * size_t(N));
         zero<uint32_t>(cand_all_, std::max<size_t>(N, size_t(sch_.blocks) * kCap));
         zero<uint32_t>(rank_stamp_, N);
+        zero<uint32_t>(affected_, N);
+        zero<uint32_t>(affected_list_, N);
         zero<uint32_t>(cand_tile_, size_t(node_tiles_) * kWG);
         {
             std::vector<uint32_t> all(N);
@@ -654,8 +664,12 @@ private:
 
     int device_;
     Schedule sch_;
-    hipStream_t stream_ = nullptr, side_ = nullptr;
-    hipEvent_t ev_bond_end_ = nullptr, ev_rinv_ = nullptr;
+    // stream_ carries the substep; side_ the inverse rest covariances that
+    // overlap the folds; early_ the end sample of the nodes and bonds striker
+    // pass 2 cannot reach, which overlaps pass 2.
+    hipStream_t stream_ = nullptr, side_ = nullptr, early_ = nullptr;
+    hipEvent_t ev_bond_end_ = nullptr, ev_rinv_ = nullptr, ev_pass1_ = nullptr, ev_velocity_ = nullptr,
+               ev_early_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t cus_ = 256;
     uint32_t N_ = 0, B_ = 0, D_ = 0, iterations_ = 1;
@@ -676,6 +690,7 @@ private:
     size_t interior_stages_ = 0;
     DevBuf wg_;
     DevBuf x04_, u4_, up4_, v4_, rinv_, unm_, sa_, sb_, valid_, approach_, engaged_, cand_;
+    DevBuf affected_, affected_list_;
     DevBuf rank_stamp_, dirty_list_, dirty_stamp_, cand_tile_, nbr_bond_, ev_bits_, brk_bits_, grp_bits_, ev_d_, ev_e_;
     DevBuf flag_list_, nbr_key_, spal_, sdelta_, epal_;
     DevBuf rec_, alive_, lambda_, pext_, bnd_, alive_n_, pstr_, thr_, thr2_, pvd_, mode_, slots_;
@@ -831,6 +846,8 @@ private:
         L.dirty_list = dirty_list_.as<uint32_t>();
         L.dirty_stamp = dirty_stamp_.as<uint32_t>();
         L.rank_stamp = rank_stamp_.as<uint32_t>();
+        L.affected = affected_.as<uint32_t>();
+        L.affected_list = affected_list_.as<uint32_t>();
         L.cand_lo = cand_lo_.as<uint32_t>();
         L.cand_cnt = cand_cnt_.as<uint32_t>();
         L.nbb = nbb_.as<uint32_t>();
@@ -889,11 +906,13 @@ private:
     // Kernel kinds for the optional per-launch event timing, each with the
     // reference phase id it belongs to (FastLattice.cpp:22-26).
     enum Kernel : uint32_t { KStrainStart, KBondStart, KPredict, KPass1, KSolve, KProject, KVelocity, KDamp,
-                             KSupportVelocity, KPass2, KStrain, KBondEnd, KFinish, KRinv, KCount };
+                             KSupportVelocity, KPass2, KStrain, KBondEnd, KFinish, KRinv, KMark, KStrainLate,
+                             KBondEndLate, KCount };
     static const char *kernelName(uint32_t k) {
         static const char *names[KCount] = {"k_strain_start", "k_bond_start", "k_predict", "k_pass1", "k_solve",
                                             "k_project", "k_velocity", "k_damp", "k_support_velocity", "k_pass2",
-                                            "k_strain", "k_bond_end", "k_finish", "k_rinv"};
+                                            "k_strain", "k_bond_end", "k_finish", "k_rinv", "k_mark_affected",
+                                            "k_strain_late", "k_bond_end_late"};
         return k < KCount ? names[k] : "unknown";
     }
     struct ProfRecord { uint32_t kernel, phase; hipEvent_t a, b; };
@@ -919,6 +938,9 @@ private:
     void profEnd(hipStream_t on = nullptr) {
         if (!prof_) return;
         HIP_OK(hipEventRecord(records_.back().b, on ? on : stream_));
+    }
+    void profAddLaunches(uint32_t kernel, uint32_t extra) {
+        if (prof_) kernel_launches_[kernel] += extra;
     }
     void profCollect() {
         for (const ProfRecord &r : records_) {
@@ -954,10 +976,12 @@ private:
     template <bool DIRECT> void enqueueT() {
         const DevLattice<R> L = L_;
         const Step<R> S = S_;
-        const uint32_t gn = gridFor(N_), gb = gridFor(B_);
+        const uint32_t gn = gridFor(N_);
         const bool sphere = S.sphere_enabled != 0;
+        // With a striker, the end sample of what pass 2 cannot reach overlaps pass 2.
+        const bool split = sphere && !plastic_;
         const uint32_t gnt = xcdGrid(node_tiles_), gbt = xcdGrid(bond_tiles_);
-        BLX_LAUNCH(KStrainStart, 1, hipLaunchKernelGGL((k_strain<R, DIRECT, true>), dim3(gn), dim3(kWG), 0, stream_, L));
+        BLX_LAUNCH(KStrainStart, 1, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainDirty>), dim3(gn), dim3(kWG), 0, stream_, L));
         BLX_LAUNCH(KBondStart, 2, hipLaunchKernelGGL((k_bond_start<R, DIRECT>), dim3(gn), dim3(kWG), 0, stream_, L));
         const dim3 pg(gnt);
         profBegin(KPredict, 3);
@@ -967,19 +991,34 @@ private:
         else hipLaunchKernelGGL((k_predict<R, false, false, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
         profEnd();
         if (sphere) BLX_LAUNCH(KPass1, 4, hipLaunchKernelGGL((k_pass<R, 1>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
+        if (split) {
+            HIP_OK(hipEventRecord(ev_pass1_, stream_));
+            HIP_OK(hipStreamWaitEvent(early_, ev_pass1_, 0));
+            const uint32_t marks = std::min<uint32_t>(cus_ * 4U, (sch_.blocks * kCap * (D_ + 1) + kWG - 1) / kWG);
+            profBegin(KMark, 4, early_);
+            hipLaunchKernelGGL((k_mark_affected<R>), dim3(std::max(1U, marks)), dim3(kWG), 0, early_, L);
+            profEnd(early_);
+        }
         const uint2 *wg = wg_.as<uint2>();
         for (uint32_t it = 0; it < iterations_; ++it) {
             const uint32_t flags = (it == 0 ? kSolveFirst : 0U) | (it + 1 < iterations_ ? kSolveKeepLambda : 0U);
-            for (size_t k = 0; k < stages_.size(); ++k) {
-                const auto &st = stages_[k];
-                profBegin(KSolve, k < interior_stages_ ? 5 : 6);
-                if (plastic_)
-                    hipLaunchKernelGGL((k_solve<R, DIRECT, true>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L, S,
-                                       wg, st.first, st.second, flags);
-                else
-                    hipLaunchKernelGGL((k_solve<R, DIRECT, false>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L, S,
-                                       wg, st.first, st.second, flags);
+            // One timing bracket over the back-to-back stage launches of the
+            // interior phase and one over the boundary phases.
+            for (int phase = 0; phase < 2; ++phase) {
+                const size_t lo = phase == 0 ? 0 : interior_stages_, hi = phase == 0 ? interior_stages_ : stages_.size();
+                if (lo == hi) continue;
+                profBegin(KSolve, phase == 0 ? 5 : 6);
+                for (size_t k = lo; k < hi; ++k) {
+                    const auto &st = stages_[k];
+                    if (plastic_)
+                        hipLaunchKernelGGL((k_solve<R, DIRECT, true>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L,
+                                           S, wg, st.first, st.second, flags);
+                    else
+                        hipLaunchKernelGGL((k_solve<R, DIRECT, false>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L,
+                                           S, wg, st.first, st.second, flags);
+                }
                 profEnd();
+                profAddLaunches(KSolve, uint32_t(hi - lo) - 1U);
             }
             if (S.plane_count > 0 && it + 1 < iterations_)
                 BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), dim3(gn), dim3(kWG), 0, stream_, L, S));
@@ -993,12 +1032,29 @@ private:
                 BLX_LAUNCH(KDamp, 9, hipLaunchKernelGGL((k_damp<R, DIRECT>), dim3(st.second), dim3(kWG), 0, stream_, L, S, wg, st.first));
             BLX_LAUNCH(KSupportVelocity, 9, hipLaunchKernelGGL((k_support_velocity<R>), dim3(gn), dim3(kWG), 0, stream_, L, S));
         }
-        if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
-        BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, false>), dim3(gnt), dim3(kWG), 0, stream_, L));
-        profBegin(KBondEnd, 12);
-        if (plastic_) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true>), dim3(gbt), dim3(kWG), 0, stream_, L, S);
-        else hipLaunchKernelGGL((k_bond_end<R, DIRECT, false>), dim3(gbt), dim3(kWG), 0, stream_, L, S);
-        profEnd();
+        if (split) {
+            HIP_OK(hipEventRecord(ev_velocity_, stream_));
+            HIP_OK(hipStreamWaitEvent(early_, ev_velocity_, 0));
+            profBegin(KStrain, 11, early_);
+            hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), dim3(gnt), dim3(kWG), 0, early_, L);
+            profEnd(early_);
+            profBegin(KBondEnd, 12, early_);
+            hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), dim3(gbt), dim3(kWG), 0, early_, L, S);
+            profEnd(early_);
+            HIP_OK(hipEventRecord(ev_early_, early_));
+            BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
+            const uint32_t late = std::min<uint32_t>(cus_ * 4U, gridFor(N_));
+            BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), dim3(late), dim3(kWG), 0, stream_, L));
+            HIP_OK(hipStreamWaitEvent(stream_, ev_early_, 0));
+            BLX_LAUNCH(KBondEndLate, 12, hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), dim3(late), dim3(kWG), 0, stream_, L, S));
+        } else {
+            if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
+            BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAll>), dim3(gnt), dim3(kWG), 0, stream_, L));
+            profBegin(KBondEnd, 12);
+            if (plastic_) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true, kBondAll>), dim3(gbt), dim3(kWG), 0, stream_, L, S);
+            else hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAll>), dim3(gbt), dim3(kWG), 0, stream_, L, S);
+            profEnd();
+        }
         // The dirty nodes' inverse rest covariances overlap the ordered folds.
         HIP_OK(hipEventRecord(ev_bond_end_, stream_));
         HIP_OK(hipStreamWaitEvent(side_, ev_bond_end_, 0));
@@ -1151,9 +1207,8 @@ int blx_hbm_copy_probe(int device, uint64_t bytes, uint32_t repeats, double *sec
         src.alloc(n * 16);
         dst.alloc(n * 16);
         HIP_OK(hipMemset(src.p, 1, n * 16));
-        hipDeviceProp_t prop{};
-        HIP_OK(hipGetDeviceProperties(&prop, device));
-        const size_t grid = std::min<size_t>((n + kWG - 1) / kWG, size_t(prop.multiProcessorCount) * 32);
+        const size_t grid = (n + kWG - 1) / kWG;
+        if (grid > 0x7fffffffULL) throw Invalid("copy probe larger than one launch");
         hipEvent_t a, b;
         HIP_OK(hipEventCreate(&a));
         HIP_OK(hipEventCreate(&b));

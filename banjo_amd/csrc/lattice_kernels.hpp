@@ -68,6 +68,8 @@ struct DevCtl {
     u32 rinv_stamp;      // break round the dirty list belongs to
     double plastic_work;
     u64 first_failure_count;
+    u32 affected_count;  // entries in affected_list
+    u32 pad0;
     // RunControl of the current run() call.
     u64 quiet, min_steps, no_failure, energy_flat, calm;
     double flat_fraction, calm_margin, removable;
@@ -130,6 +132,8 @@ template <class R> struct DevLattice {
     u32 *dirty_list;  // nodes a break made dirty, each once per substep (unordered)
     u32 *dirty_stamp; // substep stamp of a node's entry in dirty_list
     u32 *rank_stamp;  // break round a node was last counted rank-deficient in
+    u32 *affected;    // substep stamp: striker pass 2 can move the node or a neighbour
+    u32 *affected_list; // the affected nodes of this substep (unordered)
     double *tile_plastic; // plastic work per end-sample tile
     double *load_part;    // ledger partial sums per node tile (13 each)
     R4<R> *fext;      // applied external force in backend precision, w = 1 where loaded
@@ -204,7 +208,7 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
     constexpr u32 kChunk = 8;
     for (u32 k0 = 0; k0 < D; k0 += kChunk) {
         u32 key[kChunk];
-        for (u32 q = 0; q < kChunk; ++q) key[q] = k0 + q < D ? L.nbr_key[size_t(k0 + q) * N + i] : kNone;
+        for (u32 q = 0; q < kChunk; ++q) key[q] = k0 + q < D ? ldStream(&L.nbr_key[size_t(k0 + q) * N + i]) : kNone;
         R4<R> rw[kChunk];
         u32 other[kChunk];
         for (u32 q = 0; q < kChunk; ++q) {
@@ -234,23 +238,62 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
     L.sb[i] = R4<R>{e[4], e[5], R(1), R(0)};
 }
 
-// GATED is the start sample after a failure round (CpuLatticeBackend.cpp:363-370):
+// Which nodes a k_strain launch covers.
+enum StrainCover : int {
+    kStrainAll = 0,      // every node
+    kStrainDirty = 1,    // the start sample after a failure round
+    kStrainUnaffected = 2, // the end sample of nodes striker pass 2 cannot reach
+    kStrainAffected = 3  // the end sample of the rest
+};
+
+// kStrainDirty is the start sample after a failure round (CpuLatticeBackend.cpp:363-370):
 // only nodes a break made dirty can have a different strain from the one the
 // previous substep ended with (their neighbours and positions are unchanged),
 // so only they are recomputed, and they stay marked for k_bond_start.
-template <class R, bool DIRECT, bool GATED>
+// The end sample splits in two when a striker is present: striker pass 2 moves
+// only its candidates, so the strain of a node with no candidate among itself
+// and its neighbours can be taken while pass 2 runs.
+template <class R, bool DIRECT, int COVER>
 __global__ __launch_bounds__(kWG) void k_strain(DevLattice<R> L) {
     if (L.ctl->stop) return;
-    if (GATED) {
-        if (!L.ctl->dirty_start) return;
-        const u32 count = L.ctl->dirty_count;
+    if (COVER == kStrainDirty || COVER == kStrainAffected) {
+        if (COVER == kStrainDirty && !L.ctl->dirty_start) return;
+        const u32 count = COVER == kStrainDirty ? L.ctl->dirty_count : L.ctl->affected_count;
+        const u32 *list = COVER == kStrainDirty ? L.dirty_list : L.affected_list;
         for (u32 k = blockIdx.x * kWG + threadIdx.x; k < count; k += gridDim.x * kWG)
-            strainNode<R, DIRECT>(L, L.dirty_list[k]);
+            strainNode<R, DIRECT>(L, list[k]);
         return;
     }
     const u32 tile = xcdTile(blockIdx.x, L.node_tiles);
     const u32 i = tile * kWG + threadIdx.x;
-    if (tile < L.node_tiles && i < L.N) strainNode<R, DIRECT>(L, i);
+    if (tile >= L.node_tiles || i >= L.N) return;
+    if (COVER == kStrainUnaffected && L.affected[i] == u32(L.ctl->step) + 1U) return;
+    strainNode<R, DIRECT>(L, i);
+}
+
+// The nodes striker pass 2 can reach this substep: its candidates (the
+// staged lists of pass 1) and their neighbours, each listed once. Runs on a
+// side stream while the colour sweep runs.
+template <class R> __global__ __launch_bounds__(kWG) void k_mark_affected(DevLattice<R> L) {
+    DevCtl *ctl = L.ctl;
+    if (ctl->stop) return;
+    const u32 stamp = u32(ctl->step) + 1U;
+    const u32 per = L.D + 1;
+    const u64 total = u64(L.blocks) * kCap * per;
+    for (u64 q = u64(blockIdx.x) * kWG + threadIdx.x; q < total; q += u64(gridDim.x) * kWG) {
+        const u32 b = u32(q / (u64(kCap) * per));
+        const u32 rem = u32(q % (u64(kCap) * per));
+        const u32 e = rem / per, k = rem % per;
+        if (e >= L.cand_cnt[b]) continue;
+        const u32 c = L.cand_all[size_t(b) * kCap + e];
+        u32 n = c;
+        if (k > 0) {
+            const u32 key = L.nbr_key[size_t(k - 1) * L.N + c];
+            if (key == kNone) continue;
+            n = u32(int(c) + L.sdelta[key & ~kDeadSlot]);
+        }
+        if (atomicExch(&L.affected[n], stamp) != stamp) L.affected_list[atomicAdd(&ctl->affected_count, 1U)] = n;
+    }
 }
 
 // Inverse rest covariances of the nodes the last breaks made dirty. It needs
@@ -373,6 +416,7 @@ __global__ __launch_bounds__(kWG) void k_predict(DevLattice<R> L, Step<R> S) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->dirty_start = 0;
         ctl->dirty_count = 0;
+        ctl->affected_count = 0;
     }
     const bool loads = LOADS && ctl->load_remaining > 0;
     const u32 tile = xcdTile(blockIdx.x, L.node_tiles);
@@ -811,8 +855,8 @@ __global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const
     const bool first = flags & kSolveFirst;
     // The record is loaded with the alive flag rather than after it, so the
     // node gathers wait on one memory round trip, not two.
-    const BondRec<R> r = L.rec[j];
-    const u8 live = L.alive[j];
+    const BondRec<R> r = ldStream(&L.rec[j]);
+    const u8 live = ldStream(&L.alive[j]);
     const R lam = first ? R(0) : L.lambda[j];
     if (!live) return;
     const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
@@ -839,8 +883,8 @@ __global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const
     const R dl = (-C - alpha * lam) / (wa + wb + alpha);
     // The multiplier is read only by a later iteration of the same substep.
     if (flags & kSolveKeepLambda) L.lambda[j] = lam + dl;
-    L.u4[r.a] = with(xyz(ua) - wa * dl * dir, ua.w);
-    L.u4[r.b] = with(xyz(ub) + wb * dl * dir, ub.w);
+    stStream(&L.u4[r.a], with(xyz(ua) - wa * dl * dir, ua.w));
+    stStream(&L.u4[r.b], with(xyz(ub) + wb * dl * dir, ub.w));
 }
 
 // Radial pair damping, one colour stage (LatticePhysics.hpp:1171-1184).
@@ -938,151 +982,133 @@ template <class R> __global__ __launch_bounds__(kWG) void k_support_velocity(Dev
 // nodes and the node gathers hit in cache (schedule order is colour-major and
 // would sweep all node data once per colour). A bond whose break or damage
 // could change the ordered folds sets its bit in ev_bits, by schedule index,
-// so k_finish reads the events back in bond order.
+// so k_finish reads the events back in bond order, whatever order the bonds
+// were sampled in.
 
-template <class R, bool DIRECT, bool PLASTIC>
-__global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
-    DevCtl *ctl = L.ctl;
-    if (ctl->stop) return;
-    __shared__ float s_max[4][kWG / 64];
-    __shared__ double s_pw[kWG];
-    const double m0 = ctl->max_damage;
-    const u32 stamp = u32(ctl->step) + 1U;
+// Which bonds a k_bond_end launch covers: all of them, or, while striker pass
+// 2 runs, those with neither end affected, and after it the rest.
+enum BondCover : int { kBondAll = 0, kBondUnaffected = 2, kBondAffected = 3 };
+
+// One bond's end sample, damage and break. Returns through the references the
+// bond's contributions to the running maxima, its plastic work and the nodes
+// it claims for the dirty list.
+template <class R, bool DIRECT, bool PLASTIC, int COVER>
+__device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step<R> &S, u32 o, double m0, u32 stamp,
+                                              float &mt, float &mc, float &ms, float &mp, double &pw, u32 &claim_a,
+                                              u32 &claim_b) {
+    // The bond's streamed data is loaded together with its alive flag, so the
+    // node gathers wait on one memory round trip.
+    const BondNode r = ldStream(&L.bnd[o]);
+    R4<R> pv = ldStream(&L.pvd[o]);
+    u8 live = ldStream(&L.alive_n[o]);
+    Thresholds<R> th{};
+    if (L.tensile_only) {
+        const Tensile<R> t2 = ldStream(&L.thr2[o]);
+        th.ds = t2.ds;
+        th.de = t2.de;
+    } else {
+        th = L.thr[o];
+    }
+    if (COVER == kBondUnaffected && live && (L.affected[r.a] == stamp || L.affected[r.b] == stamp)) live = 0;
+    if (!live) return;
+    const u32 j = r.j;
+    const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
+    const EdgePal<R> g = L.epal[r.p & ~kNoDir];
+    const R restL = g.e.w;
+    R plastic = R(0);
+    if (PLASTIC) {
+        // bondPlasticReturn (LatticePhysics.hpp:1018-1039).
+        plastic = L.pext[j];
+        R ps = L.pstr[o];
+        const R elastic = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
+        const R mag = absR(elastic);
+        const R h = S.hardening;
+        const R ye = S.yield * restL + h * ps;
+        if (mag > ye) {
+            const R inc = (mag - ye) / (R(1) + h);
+            plastic = plastic + (elastic < R(0) ? -inc : inc);
+            ps = ps + inc;
+            L.pext[j] = plastic;
+            L.pstr[o] = ps;
+            const R c = L.rec[j].c;
+            if (c > R(0)) {
+                const R yend = ye + h * inc;
+                pw = double(R(0.5) * (ye + yend) * inc / c);
+            }
+        }
+        mp = fmaxf(mp, float(absR(plastic) / restL));
+    }
+    R t, c, s;
+    sampleBond<R, DIRECT>(L, r.a, r.b, r.p, g, ua, ub, t, c, s);
+    const R pt = maxR(R(0), maxR(pv.x, t)), pc = maxR(R(0), maxR(pv.y, c)), pss = maxR(R(0), maxR(pv.z, s));
+    mt = fmaxf(mt, float(pt));
+    mc = fmaxf(mc, float(pc));
+    ms = fmaxf(ms, float(pss));
+    pv.x = maxR(R(0), t);
+    pv.y = maxR(R(0), c);
+    pv.z = maxR(R(0), s);
+    // With every compression and shear start non-finite their ramps are an
+    // exact 0 (damageProgress), which never wins below.
+    const R td = progress(pt, th.ds, th.de);
+    const R cd = L.tensile_only ? R(0) : progress(pc, th.cs, th.ce);
+    const R sd = L.tensile_only ? R(0) : progress(pss, th.ss, th.se);
+    R dmg = td;
+    u8 md = 1;
+    if (cd > dmg) { dmg = cd; md = 2; }
+    if (sd > dmg) { dmg = sd; md = 3; }
+    if (dmg > pv.w) {
+        pv.w = dmg;
+        L.mode[o] = md;
+    }
+    stStream(&L.pvd[o], pv);
+    const float event_damage = float(pv.w);
+    bool event = double(event_damage) > m0 + 1.0e-6;
+    R event_energy = R(0);
+    const bool broke = pv.w >= R(1);
+    if (broke) {
+        const R ext = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
+        const R cc = L.rec[j].c;
+        event_energy = cc <= R(0) ? R(0) : R(0.5) * ext * ext / cc;
+        L.alive[j] = 0;
+        L.alive_n[o] = 0;
+        const uint2 sl = L.slots[o];
+        L.nbr_key[sl.x] |= kDeadSlot;
+        L.nbr_key[sl.y] |= kDeadSlot;
+        claim_a = atomicExch(&L.dirty_stamp[r.a], stamp) != stamp ? r.a : kNone;
+        claim_b = atomicExch(&L.dirty_stamp[r.b], stamp) != stamp ? r.b : kNone;
+        event = true;
+    }
+    if (event) {
+        L.ev_d[j] = event_damage;
+        L.ev_e[j] = event_energy;
+        const unsigned long long bit = 1ULL << (j & 63U);
+        atomicOr((unsigned long long *)&L.ev_bits[j >> 6], bit);
+        if (broke) atomicOr((unsigned long long *)&L.brk_bits[j >> 6], bit);
+        atomicOr((unsigned long long *)&L.grp_bits[j >> 12], 1ULL << ((j >> 6) & 63U));
+    }
+}
+
+// Appends the wave's dirty-node claims to dirty_list (one atomic per wave).
+__device__ __forceinline__ void appendDirty(u32 *list, u32 *count, u32 claim_a, u32 claim_b) {
+    const u32 lane = threadIdx.x & 63U;
+    const u32 mine = (claim_a != kNone) + (claim_b != kNone);
+    u32 x = mine;
+    for (u32 q = 1; q < 64; q <<= 1) {
+        const u32 y = __shfl_up(x, q);
+        if (lane >= q) x += y;
+    }
+    const u32 wave_total = __shfl(x, 63);
+    u32 base = 0;
+    if (lane == 63 && wave_total) base = atomicAdd(count, wave_total);
+    base = __shfl(base, 63);
+    u32 at = base + x - mine;
+    if (claim_a != kNone) list[at++] = claim_a;
+    if (claim_b != kNone) list[at] = claim_b;
+}
+
+// Running maxima of non-negative floats: integer atomicMax on the bits.
+__device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, float ms, float mp, float (*s_max)[kWG / 64]) {
     const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->rank_slot = u32(ctl->step & 1U);
-        ctl->rinv_stamp = stamp;
-    }
-    float mt = 0.f, mc = 0.f, ms = 0.f, mp = 0.f;
-    const u32 tile = xcdTile(blockIdx.x, L.bond_tiles);
-    if (tile >= L.bond_tiles) return;
-    {
-        const u32 o = tile * kWG + threadIdx.x;
-        u32 claim_a = kNone, claim_b = kNone;
-        double pw = 0.0;
-        // The bond's streamed data is loaded together with its alive flag,
-        // so the node gathers wait on one memory round trip.
-        const bool in_range = o < L.B;
-        BondNode r{};
-        R4<R> pv{};
-        Thresholds<R> th{};
-        u8 live = 0;
-        if (in_range) {
-            r = L.bnd[o];
-            pv = L.pvd[o];
-            live = L.alive_n[o];
-            if (L.tensile_only) {
-                const Tensile<R> t2 = L.thr2[o];
-                th.ds = t2.ds;
-                th.de = t2.de;
-            } else {
-                th = L.thr[o];
-            }
-        }
-        if (live) {
-            const u32 j = r.j;
-            const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
-            const EdgePal<R> g = L.epal[r.p & ~kNoDir];
-            const R restL = g.e.w;
-            R plastic = R(0);
-            if (PLASTIC) {
-                // bondPlasticReturn (LatticePhysics.hpp:1018-1039).
-                plastic = L.pext[j];
-                R ps = L.pstr[o];
-                const R elastic = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
-                const R mag = absR(elastic);
-                const R h = S.hardening;
-                const R ye = S.yield * restL + h * ps;
-                if (mag > ye) {
-                    const R inc = (mag - ye) / (R(1) + h);
-                    plastic = plastic + (elastic < R(0) ? -inc : inc);
-                    ps = ps + inc;
-                    L.pext[j] = plastic;
-                    L.pstr[o] = ps;
-                    const R c = L.rec[j].c;
-                    if (c > R(0)) {
-                        const R yend = ye + h * inc;
-                        pw = double(R(0.5) * (ye + yend) * inc / c);
-                    }
-                }
-                mp = fmaxf(mp, float(absR(plastic) / restL));
-            }
-            R t, c, s;
-            sampleBond<R, DIRECT>(L, r.a, r.b, r.p, g, ua, ub, t, c, s);
-            const R pt = maxR(R(0), maxR(pv.x, t)), pc = maxR(R(0), maxR(pv.y, c)), pss = maxR(R(0), maxR(pv.z, s));
-            mt = fmaxf(mt, float(pt));
-            mc = fmaxf(mc, float(pc));
-            ms = fmaxf(ms, float(pss));
-            pv.x = maxR(R(0), t);
-            pv.y = maxR(R(0), c);
-            pv.z = maxR(R(0), s);
-            // With every compression and shear start non-finite their ramps
-            // are an exact 0 (damageProgress), which never wins below.
-            const R td = progress(pt, th.ds, th.de);
-            const R cd = L.tensile_only ? R(0) : progress(pc, th.cs, th.ce);
-            const R sd = L.tensile_only ? R(0) : progress(pss, th.ss, th.se);
-            R dmg = td;
-            u8 md = 1;
-            if (cd > dmg) { dmg = cd; md = 2; }
-            if (sd > dmg) { dmg = sd; md = 3; }
-            if (dmg > pv.w) {
-                pv.w = dmg;
-                L.mode[o] = md;
-            }
-            L.pvd[o] = pv;
-            const float event_damage = float(pv.w);
-            bool event = double(event_damage) > m0 + 1.0e-6;
-            R event_energy = R(0);
-            const bool broke = pv.w >= R(1);
-            if (broke) {
-                const R ext = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
-                const R cc = L.rec[j].c;
-                event_energy = cc <= R(0) ? R(0) : R(0.5) * ext * ext / cc;
-                L.alive[j] = 0;
-                L.alive_n[o] = 0;
-                const uint2 sl = L.slots[o];
-                L.nbr_key[sl.x] |= kDeadSlot;
-                L.nbr_key[sl.y] |= kDeadSlot;
-                claim_a = atomicExch(&L.dirty_stamp[r.a], stamp) != stamp ? r.a : kNone;
-                claim_b = atomicExch(&L.dirty_stamp[r.b], stamp) != stamp ? r.b : kNone;
-                event = true;
-            }
-            if (event) {
-                L.ev_d[j] = event_damage;
-                L.ev_e[j] = event_energy;
-                const unsigned long long bit = 1ULL << (j & 63U);
-                atomicOr((unsigned long long *)&L.ev_bits[j >> 6], bit);
-                if (broke) atomicOr((unsigned long long *)&L.brk_bits[j >> 6], bit);
-                atomicOr((unsigned long long *)&L.grp_bits[j >> 12], 1ULL << ((j >> 6) & 63U));
-            }
-        }
-        {
-            const u32 mine = (claim_a != kNone) + (claim_b != kNone);
-            u32 x = mine;
-            for (u32 q = 1; q < 64; q <<= 1) {
-                const u32 y = __shfl_up(x, q);
-                if (lane >= q) x += y;
-            }
-            const u32 wave_total = __shfl(x, 63);
-            u32 base = 0;
-            if (lane == 63 && wave_total) base = atomicAdd(&ctl->dirty_count, wave_total);
-            base = __shfl(base, 63);
-            u32 at = base + x - mine;
-            if (claim_a != kNone) L.dirty_list[at++] = claim_a;
-            if (claim_b != kNone) L.dirty_list[at] = claim_b;
-        }
-        if (PLASTIC) {
-            s_pw[threadIdx.x] = pw;
-            __syncthreads();
-            for (u32 w = kWG / 2; w > 0; w >>= 1) {
-                if (threadIdx.x < w) s_pw[threadIdx.x] = s_pw[threadIdx.x] + s_pw[threadIdx.x + w];
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) L.tile_plastic[tile] = s_pw[0];
-            __syncthreads();
-        }
-    }
-    // Running maxima of non-negative floats: integer atomicMax on the bits.
     for (int q = 32; q > 0; q >>= 1) {
         mt = fmaxf(mt, __shfl_xor(mt, q));
         mc = fmaxf(mc, __shfl_xor(mc, q));
@@ -1106,6 +1132,64 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
             atomicMax(dst, __float_as_uint(m));
         }
     }
+}
+
+template <class R, bool DIRECT, bool PLASTIC, int COVER>
+__global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
+    static_assert(!PLASTIC || COVER == kBondAll, "plastic work is summed per end-sample tile in a fixed order");
+    DevCtl *ctl = L.ctl;
+    if (ctl->stop) return;
+    __shared__ float s_max[4][kWG / 64];
+    __shared__ double s_pw[kWG];
+    const double m0 = ctl->max_damage;
+    const u32 stamp = u32(ctl->step) + 1U;
+    if (COVER != kBondAffected && blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->rank_slot = u32(ctl->step & 1U);
+        ctl->rinv_stamp = stamp;
+    }
+    float mt = 0.f, mc = 0.f, ms = 0.f, mp = 0.f;
+    if (COVER == kBondAffected) {
+        // Bonds with an affected end, each taken once: from its first end when
+        // that end is affected, else from its second.
+        const u64 pairs = u64(ctl->affected_count) * L.D;
+        for (u64 base = u64(blockIdx.x) * kWG; base < pairs; base += u64(gridDim.x) * kWG) {
+            const u64 q = base + threadIdx.x;
+            u32 claim_a = kNone, claim_b = kNone;
+            double pw = 0.0;
+            if (q < pairs) {
+                const u32 i = L.affected_list[q / L.D], k = u32(q % L.D);
+                const u32 o = L.nbr_bond[size_t(k) * L.N + i];
+                if (o != kNone) {
+                    const u32 a = L.bnd[o].a;
+                    if (a == i || L.affected[a] != stamp)
+                        endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, m0, stamp, mt, mc, ms, mp, pw, claim_a, claim_b);
+                }
+            }
+            appendDirty(L.dirty_list, &ctl->dirty_count, claim_a, claim_b);
+        }
+        publishMaxima(ctl, mt, mc, ms, mp, s_max);
+        return;
+    }
+    const u32 tile = xcdTile(blockIdx.x, L.bond_tiles);
+    if (tile >= L.bond_tiles) return;
+    {
+        const u32 o = tile * kWG + threadIdx.x;
+        u32 claim_a = kNone, claim_b = kNone;
+        double pw = 0.0;
+        if (o < L.B) endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, m0, stamp, mt, mc, ms, mp, pw, claim_a, claim_b);
+        appendDirty(L.dirty_list, &ctl->dirty_count, claim_a, claim_b);
+        if (PLASTIC) {
+            s_pw[threadIdx.x] = pw;
+            __syncthreads();
+            for (u32 w = kWG / 2; w > 0; w >>= 1) {
+                if (threadIdx.x < w) s_pw[threadIdx.x] = s_pw[threadIdx.x] + s_pw[threadIdx.x + w];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) L.tile_plastic[tile] = s_pw[0];
+            __syncthreads();
+        }
+    }
+    publishMaxima(ctl, mt, mc, ms, mp, s_max);
 }
 
 // ---------------------------------------------------------------------------
@@ -1440,8 +1524,11 @@ __global__ void k_cc_label(const u32 *root, const u32 *rank, u32 *label, u32 n) 
 
 // A plain streaming copy, 16 bytes per lane per iteration: the known-size
 // workload rocprofv3's FETCH_SIZE and WRITE_SIZE counters are scaled against.
+// Streaming copy, one 16-byte element per thread: the achievable-bandwidth
+// reference, and the known traffic that scales the HBM byte counters.
 __global__ __launch_bounds__(kWG) void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
-    for (size_t i = size_t(blockIdx.x) * kWG + threadIdx.x; i < n; i += size_t(gridDim.x) * kWG) dst[i] = src[i];
+    const size_t i = size_t(blockIdx.x) * kWG + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 } // namespace blx
