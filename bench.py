@@ -63,6 +63,25 @@ def algorithmic_bytes(N, B, stages, sphere=True, plastic=False, iterations=1):
     return per, step
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "impact128_hbm_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs of this workload (each scaled by its copy-probe calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    hits = [v for k, v in t.get("kernels", {}).items() if k == kernel or k.startswith(kernel + "<")]
+    if not hits:
+        return None
+    v = max(hits, key=lambda x: x.get("dispatches", 0))
+    return {"bytes": v["bytes"], "fetch_bytes": v.get("fetch_bytes"), "write_bytes": v.get("write_bytes"),
+            "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
+
+
 def make_workload(args):
     from banjo_amd import scenes
     sc = scenes.impact(128, 64, 128, substeps=args.steps * SUBSTEPS_PER_STEP, blocks=args.blocks,
@@ -205,9 +224,13 @@ def main():
     roofline = None
     if bytes_launch is not None:
         achieved = bytes_launch / avg
+        measured = measured_traffic(dominant)
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                    "traffic": measured["bytes"] if measured else None,
                     "bytes_per_launch": bytes_launch, "mean_launch_us": avg * 1e6}
+        if measured:
+            roofline["traffic_detail"] = measured
     per_bond_solve = step_bytes / B
     step_roofline = {"bytes_per_bond_solve": per_bond_solve, "achieved": per_bond_solve * rate / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": per_bond_solve * rate / HBM_PEAK}
