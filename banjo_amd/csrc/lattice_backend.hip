@@ -3,6 +3,13 @@
 // reference's LatticeBackend (src/fastlattice/FastLattice.hpp:255-369), whose
 // CUDA implementation the reference no longer carries
 // (src/fastlattice/CudaLatticeUnavailable.cpp:1-27).
+//
+// One engine advances a batch of independent lattices ("slots") that share
+// step settings: every phase of the substep is one launch over all running
+// slots, each workgroup finding its lattice in the launch's map. The plain
+// single-lattice backend is an engine with one slot. Each slot's results are
+// those of the same lattice run alone, bit for bit: lattices share no data,
+// and every slot keeps its own colour order, folds and exit rule.
 
 #include "banjo_lattice.h"
 #include "lattice_errors.hpp"
@@ -37,7 +44,7 @@ std::string &lastError() {
 
 namespace {
 
-// One device allocation, freed with the backend.
+// One device allocation, freed with its owner.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -55,6 +62,27 @@ struct DevBuf {
         bytes = n;
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// A slot's arrays carved from one allocation, which a later upload reuses
+// when it fits: admitting a lattice into a used slot allocates nothing.
+class Arena {
+public:
+    void reset() { total_ = 0; }
+    template <class T> size_t reserve(size_t count) {
+        total_ = (total_ + 255) & ~size_t(255);
+        const size_t at = total_;
+        total_ += std::max<size_t>(count, 1) * sizeof(T);
+        return at;
+    }
+    void commit() { buf_.alloc(total_); }
+    template <class T> T *at(size_t offset) const { return reinterpret_cast<T *>(static_cast<char *>(buf_.p) + offset); }
+    size_t bytes() const { return buf_.bytes; }
+    size_t used() const { return total_; }
+
+private:
+    DevBuf buf_;
+    size_t total_ = 0;
 };
 
 // Distinct keys, compared bit for bit, numbered in first-seen order: an
@@ -99,43 +127,29 @@ private:
     }
 };
 
-template <class T> void toDevice(DevBuf &b, const std::vector<T> &v) {
-    b.alloc(v.size() * sizeof(T));
-    if (!v.empty()) HIP_OK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+int deviceCount() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
 }
-
-template <class T> void zero(DevBuf &b, size_t count) {
-    b.alloc(count * sizeof(T));
-    HIP_OK(hipMemset(b.p, 0, count * sizeof(T)));
-}
-
-class BackendBase {
-public:
-    virtual ~BackendBase() = default;
-    virtual std::string name() const = 0;
-    virtual void upload(const blx_state_view &, const blx_settings &, const blx_sphere &) = 0;
-    virtual void setForces(const double *, uint64_t) = 0;
-    virtual void run(const blx_run_control &, blx_run_status &) = 0;
-    virtual void status(blx_run_status &) const = 0;
-    virtual void download(blx_state_view &, blx_sphere &) = 0;
-    virtual uint64_t firstFailure(uint32_t *, uint64_t) const = 0;
-    virtual uint32_t frameCount() const = 0;
-    virtual void takeFrame(uint32_t, uint64_t *, float *, uint8_t *, float *, blx_sphere *) = 0;
-    virtual void clearFrames() = 0;
-    virtual uint32_t labels(uint32_t *) = 0;
-    virtual void setProfiling(bool) = 0;
-    virtual uint32_t kernelStats(uint32_t, std::string &, double &, uint64_t &) const = 0;
-};
 
 struct Schedule {
     uint32_t N = 0, B = 0, blocks = 0, colors = 0;
     std::vector<uint32_t> nbb, range_begin, range_end;
 };
 
-int deviceCount() {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
-    return n;
+Schedule scheduleOf(const blx_schedule_view &sv) {
+    Schedule s;
+    s.N = sv.node_count;
+    s.B = sv.bond_count;
+    s.blocks = sv.block_count;
+    s.colors = sv.color_count;
+    if (s.blocks == 0) throw Invalid("a schedule has at least one block");
+    s.nbb.assign(sv.node_block_begin, sv.node_block_begin + sv.block_count + 1);
+    const size_t ranges = size_t(sv.block_count) * 2 * std::max(sv.color_count, 1U);
+    s.range_begin.assign(sv.range_begin, sv.range_begin + ranges);
+    s.range_end.assign(sv.range_end, sv.range_end + ranges);
+    return s;
 }
 
 template <class R> struct HostFrame {
@@ -146,77 +160,175 @@ template <class R> struct HostFrame {
     Sphere<R> sphere;
 };
 
-template <class R> class Backend final : public BackendBase {
+template <class R> void sphereOut(const Sphere<R> &s, blx_sphere &o) {
+    o.center[0] = double(s.center.x);
+    o.center[1] = double(s.center.y);
+    o.center[2] = double(s.center.z);
+    o.velocity[0] = double(s.velocity.x);
+    o.velocity[1] = double(s.velocity.y);
+    o.velocity[2] = double(s.velocity.z);
+    o.angular_velocity[0] = double(s.omega.x);
+    o.angular_velocity[1] = double(s.omega.y);
+    o.angular_velocity[2] = double(s.omega.z);
+    o.radius = double(s.radius);
+    o.mass = double(s.mass);
+    o.inertia = double(s.inertia);
+}
+
+template <class R> Step<R> convertStep(const blx_settings &s) {
+    Step<R> o{};
+    o.dt = R(s.dt);
+    o.gravity = {R(s.gravity[0]), R(s.gravity[1]), R(s.gravity[2])};
+    o.damping = R(s.damping_fraction);
+    o.yield = R(s.plastic_yield_stretch);
+    o.hardening = R(s.plastic_hardening);
+    o.c_sf = R(s.contact.static_friction);
+    o.c_df = R(s.contact.dynamic_friction);
+    o.c_rest = R(s.contact.restitution);
+    o.c_rest_thr = R(s.contact.restitution_speed_threshold);
+    o.c_node_radius = R(s.contact.node_contact_radius);
+    o.c_margin = R(s.contact.contact_margin);
+    o.c_slack = R(s.contact.prefilter_slack);
+    o.plane_count = s.plane_count;
+    o.sphere_enabled = s.sphere_enabled;
+    for (int p = 0; p < 2; ++p) {
+        const blx_support_plane &in = s.planes[p];
+        Plane<R> &pl = o.planes[p];
+        pl.point = {R(in.point[0]), R(in.point[1]), R(in.point[2])};
+        pl.normal = {R(in.normal[0]), R(in.normal[1]), R(in.normal[2])};
+        pl.tangent = {R(in.tangent[0]), R(in.tangent[1]), R(in.tangent[2])};
+        pl.bitangent = {R(in.bitangent[0]), R(in.bitangent[1]), R(in.bitangent[2])};
+        pl.restitution = R(in.restitution);
+        pl.static_friction = R(in.static_friction);
+        pl.dynamic_friction = R(in.dynamic_friction);
+        pl.node_radius = R(in.node_radius);
+        pl.reach_capped = in.reach_capped;
+        pl.footprint_count = in.footprint_count;
+        for (int f = 0; f < 4; ++f) {
+            pl.fp[f][0] = R(in.footprints[f].center_t);
+            pl.fp[f][1] = R(in.footprints[f].center_b);
+            pl.fp[f][2] = R(in.footprints[f].half_t);
+            pl.fp[f][3] = R(in.footprints[f].half_b);
+        }
+    }
+    return o;
+}
+
+void validateSettings(const blx_settings &s) {
+    if (s.bond_integrator == BLX_BOND_VELOCITY_VERLET)
+        throw Invalid("the Verlet bond integrator is the serial double CPU reference only");
+    if (s.bond_integrator != BLX_BOND_XPBD) throw Invalid("unknown bond integrator");
+    if (s.node_contact.mode != BLX_NODE_CONTACT_OFF) throw Invalid("node-node contact is not available on the HIP backend");
+    if (s.plate_half_thickness != 0.0) throw Invalid("plate bending is not available on the HIP backend");
+    if (s.audit_energy) throw Invalid("the kinetic energy audit is not available on the HIP backend");
+    if (s.plane_count > BLX_MAX_SUPPORT_PLANES) throw Invalid("at most two support planes");
+    for (uint32_t p = 0; p < s.plane_count; ++p)
+        if (s.planes[p].footprint_count > BLX_MAX_FOOTPRINTS) throw Invalid("at most four footprints per plane");
+    if (s.constraint_iterations == 0) throw Invalid("at least one constraint iteration");
+    if (!(s.dt > 0.0) || !std::isfinite(s.dt)) throw Invalid("the substep must be finite and positive");
+}
+
+// The settings every slot of a batch shares, compared as the values the
+// kernels see.
+struct Shared {
+    bool set = false;
+    bool direct = false, plastic = false;
+    uint32_t iterations = 1;
+    std::vector<unsigned char> step_bytes;
+};
+
+// ---------------------------------------------------------------------------
+// One lattice of a batch: its device arrays (one arena), its colour stages,
+// its run status and frames.
+
+template <class R> class Slot {
 public:
-    Backend(int device, Schedule s) : device_(device), sch_(std::move(s)) {
-        HIP_OK(hipSetDevice(device_));
-        HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
-        HIP_OK(hipStreamCreateWithFlags(&early_, hipStreamNonBlocking));
-        HIP_OK(hipEventCreateWithFlags(&ev_bond_end_, hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&ev_rinv_, hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&ev_pass1_, hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&ev_velocity_, hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&ev_early_, hipEventDisableTiming));
-        HIP_OK(hipEventCreate(&ev0_));
-        HIP_OK(hipEventCreate(&ev1_));
-        hipDeviceProp_t prop{};
-        HIP_OK(hipGetDeviceProperties(&prop, device_));
-        cus_ = uint32_t(prop.multiProcessorCount);
-        buildStages();
-    }
-    ~Backend() override {
-        (void)hipSetDevice(device_);
-        if (stream_) (void)hipStreamSynchronize(stream_);
-        if (ev0_) (void)hipEventDestroy(ev0_);
-        if (ev1_) (void)hipEventDestroy(ev1_);
-        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
-        if (side_) (void)hipStreamSynchronize(side_);
-        if (ev_bond_end_) (void)hipEventDestroy(ev_bond_end_);
-        if (ev_rinv_) (void)hipEventDestroy(ev_rinv_);
-        if (early_) (void)hipStreamSynchronize(early_);
-        for (hipEvent_t e : {ev_pass1_, ev_velocity_, ev_early_})
-            if (e) (void)hipEventDestroy(e);
-        if (early_) (void)hipStreamDestroy(early_);
-        if (side_) (void)hipStreamDestroy(side_);
-        if (stream_) (void)hipStreamDestroy(stream_);
+    Schedule sch;
+    bool scheduled = false;
+    bool uploaded = false;
+    bool armed = false;  // takes part in the next advance
+    bool loads_set = false;
+    uint32_t N = 0, B = 0, D = 0, node_tiles = 0, bond_tiles = 0, group_words = 0;
+    uint32_t slot_palette = 0, edge_palette = 0;
+    bool tensile_only = false;
+    DevLattice<R> L{};
+    DevCtl ctl_host{};
+    blx_run_status status{};
+    std::vector<HostFrame<R>> frames;
+    std::vector<R> mass_host, inv_mass_host;
+    std::vector<uint32_t> node_j_host;  // node-major bond position -> schedule index
+    // Colour stages in sweep order (CpuLatticeBackend.cpp:395-404): interior
+    // bonds of every block colour by colour, then the boundary bonds of even
+    // blocks, then of odd blocks. Interior bonds of different blocks share no
+    // node, nor do boundary bonds of blocks of one parity, so the ranges of
+    // one colour across those blocks form one node-disjoint stage. Each stage
+    // is a list of bond ranges of at most one workgroup each.
+    std::vector<std::vector<uint2>> stages;
+    size_t interior_stages = 0;
+    Arena arena;
+    DevBuf cc_parent, cc_root, cc_size, cc_rank, cc_label, cc_keys, cc_sorted, cc_count, cc_temp;
+
+    void setSchedule(Schedule s) {
+        sch = std::move(s);
+        stages.clear();
+        const uint32_t cc = std::max(sch.colors, 1U);
+        auto add = [&](uint32_t boundary, uint32_t parity_first, uint32_t step) {
+            for (uint32_t c = 0; c < sch.colors; ++c) {
+                std::vector<uint2> st;
+                for (uint32_t b = parity_first; b < sch.blocks; b += step) {
+                    const size_t key = (size_t(b) * 2 + boundary) * cc + c;
+                    const uint32_t lo = sch.range_begin[key], hi = sch.range_end[key];
+                    for (uint32_t j = lo; j < hi; j += kWG) st.push_back(make_uint2(j, std::min(hi, j + kWG)));
+                }
+                if (!st.empty()) stages.push_back(std::move(st));
+            }
+        };
+        add(0, 0, 1);
+        interior_stages = stages.size();
+        add(1, 0, 2);
+        add(1, 1, 2);
+        scheduled = true;
+        uploaded = false;
+        armed = false;
     }
 
-    std::string name() const override { return std::string("hip-gfx950-") + (sizeof(R) == 4 ? "float" : "double"); }
-
-    void upload(const blx_state_view &st, const blx_settings &s, const blx_sphere &sp) override {
-        HIP_OK(hipSetDevice(device_));
-        validate(st, s);
-        const uint32_t N = st.node_count, B = st.bond_count, D = st.max_degree;
-        N_ = N;
-        B_ = B;
-        D_ = D;
-        direct_ = s.direct_arithmetic != 0;
-        plastic_ = s.plastic_yield_stretch > 0.0;
-        S_ = convertStep(s);
-        iterations_ = s.constraint_iterations;
+    // Builds every device array of this lattice on `stream` (and waits for
+    // the copies); returns the initial control block and sphere for the
+    // engine to place at ctl_dev and sphere_dev.
+    void upload(const blx_state_view &st, const Step<R> &S, const blx_sphere &sp, hipStream_t stream, DevCtl *ctl_dev,
+                Sphere<R> *sphere_dev, DevCtl &ctl0, Sphere<R> &sph0) {
+        if (st.node_count != sch.N || st.bond_count != sch.B) throw Invalid("state does not match the schedule");
+        if (size_t(st.max_degree) * st.node_count > 0xffffffffULL) throw Invalid("neighbour slots exceed 32-bit indices");
+        if (st.bond_count >= 0x80000000U) throw Invalid("at most 2^31 - 1 bonds");
+        if (S.sphere_enabled && (st.node_count + kWG - 1) / kWG > kMaxNodeTiles)
+            throw Invalid("the striker pass holds at most 4,194,304 nodes");
+        N = st.node_count;
+        B = st.bond_count;
+        D = st.max_degree;
         const size_t P = size_t(D) * N;
+        node_tiles = (N + kWG - 1) / kWG;
+        bond_tiles = (B + kWG - 1) / kWG;
+        const uint32_t event_words = (B + 63) / 64;
+        group_words = (event_words + 63) / 64;
+
+        // Host arrays in device layout.
         std::vector<R4<R>> x04(N), u4(N), up4(N), v4(N);
         for (uint32_t i = 0; i < N; ++i) {
-            x04[i] = {R(st.x0[3 * size_t(i)]), R(st.x0[3 * size_t(i) + 1]), R(st.x0[3 * size_t(i) + 2]), R(st.mass[i])};
-            u4[i] = {R(st.u[3 * size_t(i)]), R(st.u[3 * size_t(i) + 1]), R(st.u[3 * size_t(i) + 2]), R(st.inv_mass[i])};
-            up4[i] = {R(st.u_prev[3 * size_t(i)]), R(st.u_prev[3 * size_t(i) + 1]), R(st.u_prev[3 * size_t(i) + 2]), R(0)};
-            v4[i] = {R(st.v[3 * size_t(i)]), R(st.v[3 * size_t(i) + 1]), R(st.v[3 * size_t(i) + 2]), R(0)};
+            const size_t k = 3 * size_t(i);
+            x04[i] = {R(st.x0[k]), R(st.x0[k + 1]), R(st.x0[k + 2]), R(st.mass[i])};
+            u4[i] = {R(st.u[k]), R(st.u[k + 1]), R(st.u[k + 2]), R(st.inv_mass[i])};
+            up4[i] = {R(st.u_prev[k]), R(st.u_prev[k + 1]), R(st.u_prev[k + 2]), R(0)};
+            v4[i] = {R(st.v[k]), R(st.v[k + 1]), R(st.v[k + 2]), R(0)};
         }
-        mass_host_.resize(N);
-        inv_mass_host_.resize(N);
+        mass_host.resize(N);
+        inv_mass_host.resize(N);
         for (uint32_t i = 0; i < N; ++i) {
-            mass_host_[i] = R(st.mass[i]);
-            inv_mass_host_[i] = R(st.inv_mass[i]);
+            mass_host[i] = R(st.mass[i]);
+            inv_mass_host[i] = R(st.inv_mass[i]);
         }
-        toDevice(x04_, x04);
-        toDevice(u4_, u4);
-        toDevice(up4_, up4);
-        toDevice(v4_, v4);
-        // Bonds in node-major order for the end sample: a counting sort by
-        // the first end, stable in schedule index.
-        std::vector<uint32_t> node_of(B);  // node-major position -> schedule index
-        std::vector<uint32_t> pos_of(B);   // schedule index -> node-major position
+        // Bonds in node-major order for the end sample: a counting sort by the
+        // first end, stable in schedule index.
+        std::vector<uint32_t> node_of(B), pos_of(B);
         {
             std::vector<uint32_t> start(size_t(N) + 1, 0U);
             for (uint32_t k = 0; k < B; ++k) {
@@ -230,15 +342,18 @@ public:
                 pos_of[k] = o;
             }
         }
-        node_j_host_ = node_of;
+        node_j_host = node_of;
         // Neighbour slots as slot palette indices.
+        std::vector<uint32_t> key(P), nbond(P);
+        std::vector<R4<R>> spal;
+        std::vector<int32_t> sdelta;
         {
-            Dedup<4 * sizeof(R) / 4 + 1> table;
-            std::vector<uint32_t> key(P), bond(P);
+            constexpr size_t W = 4 * sizeof(R) / 4 + 1;
+            Dedup<W> table;
             for (size_t k = 0; k < P; ++k) {
                 if (st.nbr_bond[k] == BLX_NO_BOND) {
                     key[k] = kNone;
-                    bond[k] = kNone;
+                    nbond[k] = kNone;
                     continue;
                 }
                 if (st.nbr_bond[k] >= B || st.nbr_other[k] >= N) throw Invalid("neighbour slot out of range");
@@ -246,73 +361,155 @@ public:
                 const R rw[4] = {R(st.nbr_rest[3 * k]), R(st.nbr_rest[3 * k + 1]), R(st.nbr_rest[3 * k + 2]),
                                  R(st.nbr_weight[k])};
                 const int32_t delta = int32_t(int64_t(st.nbr_other[k]) - int64_t(i));
-                typename Dedup<4 * sizeof(R) / 4 + 1>::Key kk{};
+                typename Dedup<W>::Key kk{};
                 std::memcpy(kk.data(), rw, sizeof(rw));
                 std::memcpy(kk.data() + 4 * sizeof(R) / 4, &delta, 4);
                 const uint32_t id = table.index(kk);
                 if (id >= kDeadSlot) throw Overflow("more than 2^31 distinct neighbour slots");
                 key[k] = id | (st.nbr_alive[k] ? 0U : kDeadSlot);
-                bond[k] = pos_of[st.nbr_bond[k]];
+                nbond[k] = pos_of[st.nbr_bond[k]];
             }
             const auto &keys = table.keys();
-            std::vector<R4<R>> pal(keys.size());
-            std::vector<int32_t> dl(keys.size());
+            spal.resize(keys.size());
+            sdelta.resize(keys.size());
             for (size_t q = 0; q < keys.size(); ++q) {
-                std::memcpy(&pal[q], keys[q].data(), 4 * sizeof(R));
-                std::memcpy(&dl[q], keys[q].data() + 4 * sizeof(R) / 4, 4);
+                std::memcpy(&spal[q], keys[q].data(), 4 * sizeof(R));
+                std::memcpy(&sdelta[q], keys[q].data() + 4 * sizeof(R) / 4, 4);
             }
-            slot_palette_ = uint32_t(keys.size());
-            toDevice(nbr_key_, key);
-            toDevice(nbr_bond_, bond);
-            toDevice(spal_, pal);
-            toDevice(sdelta_, dl);
+            slot_palette = uint32_t(keys.size());
         }
+        // Edge palette: the distinct (rest edge, rest length, |e|^2 - L^2) in
+        // backend precision; the direction terms are filled on the device.
+        std::vector<uint32_t> pal_of(B);
+        std::vector<EdgePal<R>> epal;
         {
-            // Edge palette: the distinct (rest edge, rest length, |e|^2 - L^2) in
-            // backend precision; the direction terms are filled on the device.
-            Dedup<5 * sizeof(R) / 4> table;
-            std::vector<uint32_t> pal_of(B);
+            constexpr size_t W = 5 * sizeof(R) / 4;
+            Dedup<W> table;
             for (uint32_t k = 0; k < B; ++k) {
                 const R v[5] = {R(st.rest_edge[3 * size_t(k)]), R(st.rest_edge[3 * size_t(k) + 1]),
                                 R(st.rest_edge[3 * size_t(k) + 2]), R(st.rest_length[k]), R(st.rest_length_sq_minus[k])};
-                typename Dedup<5 * sizeof(R) / 4>::Key kk{};
+                typename Dedup<W>::Key kk{};
                 std::memcpy(kk.data(), v, sizeof(v));
                 pal_of[k] = table.index(kk);
                 if (pal_of[k] >= kNoDir) throw Overflow("more than 2^31 distinct bond edges");
             }
             const auto &keys = table.keys();
-            const uint32_t np = uint32_t(keys.size());
-            edge_palette_ = np;
-            std::vector<EdgePal<R>> pal(np);
-            for (uint32_t q = 0; q < np; ++q) {
+            epal.resize(keys.size());
+            for (size_t q = 0; q < keys.size(); ++q) {
                 R v[5];
                 std::memcpy(v, keys[q].data(), sizeof(v));
-                pal[q].e = {v[0], v[1], v[2], v[3]};
-                pal[q].d = {v[4], R(0), R(0), R(0)};
+                epal[q].e = {v[0], v[1], v[2], v[3]};
+                epal[q].d = {v[4], R(0), R(0), R(0)};
             }
-            toDevice(epal_, pal);
-            DevBuf nodir;
-            nodir.alloc(size_t(np) * 4);
-            hipLaunchKernelGGL((k_edge_palette<R>), dim3(gridFor(np)), dim3(kWG), 0, stream_, epal_.as<EdgePal<R>>(),
-                               nodir.as<uint32_t>(), np);
-            HIP_OK(hipGetLastError());
-            HIP_OK(hipStreamSynchronize(stream_));
-            std::vector<uint32_t> no_dir(np);
-            HIP_OK(hipMemcpy(no_dir.data(), nodir.p, size_t(np) * 4, hipMemcpyDeviceToHost));
-            for (uint32_t k = 0; k < B; ++k) pal_of[k] |= no_dir[pal_of[k]] ? kNoDir : 0U;
+            edge_palette = uint32_t(keys.size());
+        }
+        bool tensile = true;
+        for (uint32_t k = 0; k < B && tensile; ++k) {
+            const double *t = st.threshold + 6 * size_t(k);
+            tensile = !std::isfinite(R(t[2])) && !std::isfinite(R(t[4]));
+        }
+        tensile_only = tensile;
 
+        // Layout in the arena.
+        arena.reset();
+        const uint32_t np = edge_palette;
+        struct Off {
+            size_t x04, u4, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
+            size_t nbr_key, spal, sdelta, nbr_bond, epal, nodir, rec, alive, lambda, pext;
+            size_t bnd, alive_n, thr, thr2, pvd, mode, pstr, slots;
+            size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
+            size_t affected, affected_list, tile_plastic, load_part, fext, freq, cand_all, cand_tile, cand_lo, cand_cnt;
+            size_t nbb, first_fail;
+        } o{};
+        o.x04 = arena.reserve<R4<R>>(N);
+        o.u4 = arena.reserve<R4<R>>(N);
+        o.up4 = arena.reserve<R4<R>>(N);
+        o.v4 = arena.reserve<R4<R>>(N);
+        o.rinv = arena.reserve<R>(9 * size_t(N));
+        o.unm = arena.reserve<R>(3 * size_t(N));
+        o.sa = arena.reserve<R4<R>>(N);
+        o.sb = arena.reserve<R4<R>>(N);
+        o.valid = arena.reserve<uint8_t>(N);
+        o.approach = arena.reserve<R>(2 * size_t(N));
+        o.engaged = arena.reserve<uint8_t>(2 * size_t(N));
+        o.cand = arena.reserve<uint8_t>(N);
+        o.nbr_key = arena.reserve<uint32_t>(P);
+        o.spal = arena.reserve<R4<R>>(spal.size());
+        o.sdelta = arena.reserve<int32_t>(sdelta.size());
+        o.nbr_bond = arena.reserve<uint32_t>(P);
+        o.epal = arena.reserve<EdgePal<R>>(np);
+        o.nodir = arena.reserve<uint32_t>(np);
+        o.rec = arena.reserve<BondRec<R>>(B);
+        o.alive = arena.reserve<uint8_t>(B);
+        o.lambda = arena.reserve<R>(B);
+        o.pext = arena.reserve<R>(B);
+        o.bnd = arena.reserve<BondNode>(B);
+        o.alive_n = arena.reserve<uint8_t>(B);
+        o.thr = arena.reserve<Thresholds<R>>(tensile ? 0 : B);
+        o.thr2 = arena.reserve<Tensile<R>>(tensile ? B : 0);
+        o.pvd = arena.reserve<R4<R>>(B);
+        o.mode = arena.reserve<uint8_t>(B);
+        o.pstr = arena.reserve<R>(B);
+        o.slots = arena.reserve<uint2>(B);
+        o.ev_bits = arena.reserve<uint64_t>(event_words);
+        o.brk_bits = arena.reserve<uint64_t>(event_words);
+        o.grp_bits = arena.reserve<uint64_t>(group_words);
+        o.ev_d = arena.reserve<float>(B);
+        o.ev_e = arena.reserve<R>(B);
+        o.flag_list = arena.reserve<uint32_t>(event_words);
+        o.tile_cand = arena.reserve<uint32_t>(node_tiles);
+        o.dirty_list = arena.reserve<uint32_t>(N);
+        o.dirty_stamp = arena.reserve<uint32_t>(N);
+        o.rank_stamp = arena.reserve<uint32_t>(N);
+        o.affected = arena.reserve<uint32_t>(N);
+        o.affected_list = arena.reserve<uint32_t>(N);
+        o.tile_plastic = arena.reserve<double>(bond_tiles);
+        o.load_part = arena.reserve<double>(13 * size_t(node_tiles));
+        o.fext = arena.reserve<R4<R>>(N);
+        o.freq = arena.reserve<double>(3 * size_t(N));
+        o.cand_all = arena.reserve<uint32_t>(std::max<size_t>(N, size_t(sch.blocks) * kCap));
+        o.cand_tile = arena.reserve<uint32_t>(size_t(node_tiles) * kWG);
+        o.cand_lo = arena.reserve<uint32_t>(sch.blocks);
+        o.cand_cnt = arena.reserve<uint32_t>(sch.blocks);
+        o.nbb = arena.reserve<uint32_t>(sch.nbb.size());
+        o.first_fail = arena.reserve<uint32_t>(B);
+        arena.commit();
+
+        // Zero everything, then copy the data in.
+        HIP_OK(hipMemsetAsync(arena.at<char>(0), 0, arena.used(), stream));
+        auto put = [&](size_t off, const void *src, size_t bytes) {
+            if (bytes) HIP_OK(hipMemcpyAsync(arena.at<char>(off), src, bytes, hipMemcpyHostToDevice, stream));
+        };
+        auto putv = [&](size_t off, const auto &v) { put(off, v.data(), v.size() * sizeof(v[0])); };
+        std::vector<uint32_t> all(N);
+        for (uint32_t i = 0; i < N; ++i) all[i] = i;
+        putv(o.x04, x04);
+        putv(o.u4, u4);
+        putv(o.up4, up4);
+        putv(o.v4, v4);
+        putv(o.nbr_key, key);
+        putv(o.nbr_bond, nbond);
+        putv(o.spal, spal);
+        putv(o.sdelta, sdelta);
+        putv(o.epal, epal);
+        putv(o.nbb, sch.nbb);
+        putv(o.dirty_list, all);  // upload marks every node dirty
+        // Palette directions on the device, then the bond arrays that carry
+        // the no-direction flag.
+        hipLaunchKernelGGL((k_edge_palette<R>), dim3(std::max(1U, std::min(4096U, (np + kWG - 1) / kWG))), dim3(kWG), 0,
+                           stream, arena.at<EdgePal<R>>(o.epal), arena.at<uint32_t>(o.nodir), np);
+        HIP_OK(hipGetLastError());
+        std::vector<uint32_t> no_dir(np);
+        HIP_OK(hipMemcpyAsync(no_dir.data(), arena.at<uint32_t>(o.nodir), size_t(np) * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (uint32_t k = 0; k < B; ++k) pal_of[k] |= no_dir[pal_of[k]] ? kNoDir : 0U;
+        {
             std::vector<BondRec<R>> rec(B);
             std::vector<BondNode> bnd(B);
             std::vector<R4<R>> pvd(B);
             std::vector<R> pext(B), pstr(B);
             std::vector<uint2> slots(B);
             std::vector<uint8_t> alive_n(B), mode(B);
-            bool tensile = true;
-            for (uint32_t k = 0; k < B && tensile; ++k) {
-                const double *t = st.threshold + 6 * size_t(k);
-                tensile = !std::isfinite(R(t[2])) && !std::isfinite(R(t[4]));
-            }
-            tensile_only_ = tensile;
             std::vector<Thresholds<R>> thr(tensile ? 0 : B);
             std::vector<Tensile<R>> thr2(tensile ? B : 0);
             for (uint32_t k = 0; k < B; ++k) {
@@ -324,113 +521,255 @@ public:
                 rec[k] = r;
                 pext[k] = R(st.plastic_extension[k]);
             }
-            for (uint32_t o = 0; o < B; ++o) {
-                const uint32_t k = node_of[o];
-                bnd[o] = BondNode{st.bond_a[k], st.bond_b[k], k, pal_of[k]};
+            for (uint32_t q = 0; q < B; ++q) {
+                const uint32_t k = node_of[q];
+                bnd[q] = BondNode{st.bond_a[k], st.bond_b[k], k, pal_of[k]};
                 const double *t = st.threshold + 6 * size_t(k);
-                if (tensile) thr2[o] = {R(t[0]), R(t[1])};
-                else thr[o] = {R(t[0]), R(t[1]), R(t[2]), R(t[3]), R(t[4]), R(t[5])};
-                pvd[o] = {R(st.prev_tensile[k]), R(st.prev_compressive[k]), R(st.prev_shear[k]), R(st.damage[k])};
-                pstr[o] = R(st.plastic_strain[k]);
-                slots[o] = make_uint2(st.bond_slot_a[k], st.bond_slot_b[k]);
-                alive_n[o] = st.alive[k];
-                mode[o] = st.failure_mode[k];
+                if (tensile) thr2[q] = {R(t[0]), R(t[1])};
+                else thr[q] = {R(t[0]), R(t[1]), R(t[2]), R(t[3]), R(t[4]), R(t[5])};
+                pvd[q] = {R(st.prev_tensile[k]), R(st.prev_compressive[k]), R(st.prev_shear[k]), R(st.damage[k])};
+                pstr[q] = R(st.plastic_strain[k]);
+                slots[q] = make_uint2(st.bond_slot_a[k], st.bond_slot_b[k]);
+                alive_n[q] = st.alive[k];
+                mode[q] = st.failure_mode[k];
             }
-            toDevice(rec_, rec);
-            toDevice(bnd_, bnd);
-            toDevice(thr_, thr);
-            toDevice(thr2_, thr2);
-            toDevice(pvd_, pvd);
-            toDevice(pext_, pext);
-            toDevice(pstr_, pstr);
-            toDevice(slots_, slots);
-            toDevice(alive_, std::vector<uint8_t>(st.alive, st.alive + B));
-            toDevice(alive_n_, alive_n);
-            toDevice(mode_, mode);
+            putv(o.rec, rec);
+            putv(o.bnd, bnd);
+            putv(o.thr, thr);
+            putv(o.thr2, thr2);
+            putv(o.pvd, pvd);
+            putv(o.pext, pext);
+            putv(o.pstr, pstr);
+            putv(o.slots, slots);
+            put(o.alive, st.alive, B);
+            putv(o.alive_n, alive_n);
+            putv(o.mode, mode);
+            HIP_OK(hipStreamSynchronize(stream));
         }
-        zero<R>(lambda_, B);
-        zero<uint32_t>(first_fail_, B);
-        zero<R>(rinv_, 9 * size_t(N));
-        zero<R>(unm_, 3 * size_t(N));
-        zero<R4<R>>(sa_, N);
-        zero<R4<R>>(sb_, N);
-        zero<uint8_t>(valid_, N);
-        zero<R>(approach_, 2 * size_t(N));
-        zero<uint8_t>(engaged_, 2 * size_t(N));
-        zero<uint8_t>(cand_, N);
-        node_tiles_ = (N + kWG - 1) / kWG;
-        bond_tiles_ = (B + kWG - 1) / kWG;
-        const uint32_t event_words = (B + 63) / 64;
-        group_words_ = (event_words + 63) / 64;
-        zero<uint32_t>(tile_cand_, node_tiles_);
-        zero<uint64_t>(ev_bits_, event_words);
-        zero<uint64_t>(brk_bits_, event_words);
-        zero<uint64_t>(grp_bits_, group_words_);
-        zero<float>(ev_d_, B);
-        zero<R>(ev_e_, B);
-        zero<uint32_t>(flag_list_, event_words);
-        zero<double>(tile_plastic_, bond_tiles_);
-        zero<double>(load_part_, 13 * size_t(node_tiles_));
-        zero<R4<R>>(fext_, N);
-        zero<double>(freq_, 3 * size_t(N));
-        zero<uint32_t>(cand_all_, std::max<size_t>(N, size_t(sch_.blocks) * kCap));
-        zero<uint32_t>(rank_stamp_, N);
-        zero<uint32_t>(affected_, N);
-        zero<uint32_t>(affected_list_, N);
-        zero<uint32_t>(cand_tile_, size_t(node_tiles_) * kWG);
-        {
-            std::vector<uint32_t> all(N);
-            for (uint32_t i = 0; i < N; ++i) all[i] = i;
-            toDevice(dirty_list_, all);  // upload marks every node dirty
-        }
-        zero<uint32_t>(dirty_stamp_, N);
-        zero<uint32_t>(cand_lo_, sch_.blocks);
-        zero<uint32_t>(cand_cnt_, sch_.blocks);
-        toDevice(nbb_, sch_.nbb);
-        // Control block and sphere.
+
+        // The device view.
+        DevLattice<R> &l = L;
+        l = DevLattice<R>{};
+        l.N = N;
+        l.B = B;
+        l.D = D;
+        l.blocks = sch.blocks;
+        l.node_tiles = node_tiles;
+        l.bond_tiles = bond_tiles;
+        l.group_words = group_words;
+        l.tensile_only = tensile ? 1U : 0U;
+        l.u4 = arena.at<R4<R>>(o.u4);
+        l.up4 = arena.at<R4<R>>(o.up4);
+        l.v4 = arena.at<R4<R>>(o.v4);
+        l.x04 = arena.at<R4<R>>(o.x04);
+        l.rinv = arena.at<R>(o.rinv);
+        l.unm = arena.at<R>(o.unm);
+        l.sa = arena.at<R4<R>>(o.sa);
+        l.sb = arena.at<R4<R>>(o.sb);
+        l.valid = arena.at<uint8_t>(o.valid);
+        l.approach = arena.at<R>(o.approach);
+        l.engaged = arena.at<uint8_t>(o.engaged);
+        l.cand = arena.at<uint8_t>(o.cand);
+        l.nbr_key = arena.at<uint32_t>(o.nbr_key);
+        l.spal = arena.at<R4<R>>(o.spal);
+        l.sdelta = arena.at<int>(o.sdelta);
+        l.nbr_bond = arena.at<uint32_t>(o.nbr_bond);
+        l.epal = arena.at<EdgePal<R>>(o.epal);
+        l.rec = arena.at<BondRec<R>>(o.rec);
+        l.alive = arena.at<uint8_t>(o.alive);
+        l.lambda = arena.at<R>(o.lambda);
+        l.pext = arena.at<R>(o.pext);
+        l.bnd = arena.at<BondNode>(o.bnd);
+        l.alive_n = arena.at<uint8_t>(o.alive_n);
+        l.thr = arena.at<Thresholds<R>>(o.thr);
+        l.thr2 = arena.at<Tensile<R>>(o.thr2);
+        l.pvd = arena.at<R4<R>>(o.pvd);
+        l.mode = arena.at<uint8_t>(o.mode);
+        l.pstr = arena.at<R>(o.pstr);
+        l.slots = arena.at<uint2>(o.slots);
+        l.ev_bits = arena.at<uint64_t>(o.ev_bits);
+        l.brk_bits = arena.at<uint64_t>(o.brk_bits);
+        l.grp_bits = arena.at<uint64_t>(o.grp_bits);
+        l.ev_d = arena.at<float>(o.ev_d);
+        l.ev_e = arena.at<R>(o.ev_e);
+        l.flag_list = arena.at<uint32_t>(o.flag_list);
+        l.tile_cand = arena.at<uint32_t>(o.tile_cand);
+        l.dirty_list = arena.at<uint32_t>(o.dirty_list);
+        l.dirty_stamp = arena.at<uint32_t>(o.dirty_stamp);
+        l.rank_stamp = arena.at<uint32_t>(o.rank_stamp);
+        l.affected = arena.at<uint32_t>(o.affected);
+        l.affected_list = arena.at<uint32_t>(o.affected_list);
+        l.tile_plastic = arena.at<double>(o.tile_plastic);
+        l.load_part = arena.at<double>(o.load_part);
+        l.fext = arena.at<R4<R>>(o.fext);
+        l.freq = arena.at<double>(o.freq);
+        l.cand_all = arena.at<uint32_t>(o.cand_all);
+        l.cand_tile = arena.at<uint32_t>(o.cand_tile);
+        l.cand_lo = arena.at<uint32_t>(o.cand_lo);
+        l.cand_cnt = arena.at<uint32_t>(o.cand_cnt);
+        l.nbb = arena.at<uint32_t>(o.nbb);
+        l.first_fail = arena.at<uint32_t>(o.first_fail);
+        l.ctl = ctl_dev;
+        l.sphere = sphere_dev;
+        for (int k = 0; k < 3; ++k) l.origin[k] = st.origin[k];
+
+        // Control block and sphere. A slot stays stopped until it is armed.
         DevCtl c{};
         c.first_failure_step = c.last_failure_step = c.last_energy_gain_step = ~0ULL;
         c.dirty_start = 1;
         c.dirty_count = N;
-        c.rank_slot = 1;            // booked by the first substep's k_finish
+        c.rank_slot = 1;  // booked by the first substep's k_finish
         c.rinv_stamp = 0xffffffffU;
         c.flat_fraction = 1.0e-3;
         c.calm_margin = 0.5;
-        ctl_host_ = c;
-        toDevice(ctl_, std::vector<DevCtl>{c});
-        Sphere<R> sph{{R(sp.center[0]), R(sp.center[1]), R(sp.center[2])},
-                      {R(sp.velocity[0]), R(sp.velocity[1]), R(sp.velocity[2])},
-                      {R(sp.angular_velocity[0]), R(sp.angular_velocity[1]), R(sp.angular_velocity[2])},
-                      R(sp.radius), R(sp.mass), R(sp.inertia)};
-        toDevice(sphere_, std::vector<Sphere<R>>{sph});
-        for (int k = 0; k < 3; ++k) origin_[k] = st.origin[k];
-        loads_set_ = false;
-        frames_.clear();
-        run_status_ = blx_run_status{};
-        run_status_.first_failure_step = run_status_.last_failure_step = run_status_.last_energy_gain_step = ~0ULL;
-        bind();
-        // Every node starts dirty: its inverse rest covariance before the first substep.
-        hipLaunchKernelGGL((k_rinv<R>), dim3(gridFor(N)), dim3(kWG), 0, stream_, L_);
-        HIP_OK(hipGetLastError());
+        c.stop = 1;
+        ctl_host = c;
+        ctl0 = c;
+        sph0 = Sphere<R>{{R(sp.center[0]), R(sp.center[1]), R(sp.center[2])},
+                         {R(sp.velocity[0]), R(sp.velocity[1]), R(sp.velocity[2])},
+                         {R(sp.angular_velocity[0]), R(sp.angular_velocity[1]), R(sp.angular_velocity[2])},
+                         R(sp.radius), R(sp.mass), R(sp.inertia)};
+        loads_set = false;
+        frames.clear();
+        status = blx_run_status{};
+        status.first_failure_step = status.last_failure_step = status.last_energy_gain_step = ~0ULL;
+        uploaded = true;
+        armed = false;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// The engine: shared settings, streams, launch maps and the run loop.
+
+template <class R> class Engine {
+public:
+    Engine(int device, uint32_t capacity) : device_(device), slots_(capacity) {
+        if (capacity == 0) throw Invalid("a batch holds at least one lattice");
+        HIP_OK(hipSetDevice(device_));
+        HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&early_, hipStreamNonBlocking));
+        for (hipEvent_t *e : {&ev_bond_end_, &ev_rinv_, &ev_pass1_, &ev_velocity_, &ev_early_})
+            HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        HIP_OK(hipEventCreate(&ev0_));
+        HIP_OK(hipEventCreate(&ev1_));
+        hipDeviceProp_t prop{};
+        HIP_OK(hipGetDeviceProperties(&prop, device_));
+        cus_ = uint32_t(prop.multiProcessorCount);
+        for (auto &s : slots_) s = std::make_unique<Slot<R>>();
+        ls_.alloc(size_t(capacity) * sizeof(DevLattice<R>));
+        ctls_.alloc(size_t(capacity) * sizeof(DevCtl));
+        spheres_.alloc(size_t(capacity) * sizeof(Sphere<R>));
+        HIP_OK(hipMemset(ls_.p, 0, ls_.bytes));
+        HIP_OK(hipMemset(ctls_.p, 0, ctls_.bytes));
+        HIP_OK(hipMemset(spheres_.p, 0, spheres_.bytes));
+    }
+    ~Engine() {
+        (void)hipSetDevice(device_);
+        for (hipStream_t s : {stream_, side_, early_})
+            if (s) (void)hipStreamSynchronize(s);
+        for (hipEvent_t e : {ev_bond_end_, ev_rinv_, ev_pass1_, ev_velocity_, ev_early_, ev0_, ev1_})
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+        for (hipStream_t s : {early_, side_, stream_})
+            if (s) (void)hipStreamDestroy(s);
+    }
+
+    uint32_t capacity() const { return uint32_t(slots_.size()); }
+    std::string name() const { return std::string("hip-gfx950-") + (sizeof(R) == 4 ? "float" : "double"); }
+
+    Slot<R> &slot(uint32_t k) {
+        if (k >= slots_.size()) throw Invalid("slot index out of range");
+        return *slots_[k];
+    }
+    const Slot<R> &slot(uint32_t k) const {
+        if (k >= slots_.size()) throw Invalid("slot index out of range");
+        return *slots_[k];
+    }
+
+    void setSchedule(uint32_t k, Schedule s) {
+        slot(k).setSchedule(std::move(s));
+        maps_dirty_ = true;
+        updateShared();
+    }
+
+    // LatticeBackend::upload for one slot. The first upload into an empty
+    // batch sets the shared settings; every later one must match them.
+    void upload(uint32_t k, const blx_state_view &st, const blx_settings &s, const blx_sphere &sp) {
+        HIP_OK(hipSetDevice(device_));
+        Slot<R> &sl = slot(k);
+        if (!sl.scheduled) throw Logic("slot has no schedule");
+        validateSettings(s);
+        const Step<R> S = convertStep<R>(s);
+        std::vector<unsigned char> bytes(sizeof(Step<R>));
+        std::memcpy(bytes.data(), &S, sizeof(S));
+        bool others = false;
+        for (uint32_t q = 0; q < slots_.size(); ++q)
+            if (q != k && slots_[q]->uploaded) others = true;
+        if (others && shared_.set) {
+            if (bytes != shared_.step_bytes || (s.direct_arithmetic != 0) != shared_.direct ||
+                (s.plastic_yield_stretch > 0.0) != shared_.plastic || s.constraint_iterations != shared_.iterations)
+                throw Invalid("every lattice of a batch shares the step settings");
+        } else {
+            shared_.set = true;
+            shared_.step_bytes = bytes;
+            shared_.direct = s.direct_arithmetic != 0;
+            shared_.plastic = s.plastic_yield_stretch > 0.0;
+            shared_.iterations = s.constraint_iterations;
+            S_ = S;
+        }
         HIP_OK(hipStreamSynchronize(stream_));
-        uploaded_ = true;
-        HIP_OK(hipDeviceSynchronize());
+        sl.uploaded = false;
+        DevCtl c0{};
+        Sphere<R> s0{};
+        sl.upload(st, S_, sp, stream_, ctls_.as<DevCtl>() + k, spheres_.as<Sphere<R>>() + k, c0, s0);
+        HIP_OK(hipMemcpy(ctls_.as<DevCtl>() + k, &c0, sizeof(DevCtl), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(spheres_.as<Sphere<R>>() + k, &s0, sizeof(s0), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(ls_.as<DevLattice<R>>() + k, &sl.L, sizeof(DevLattice<R>), hipMemcpyHostToDevice));
+        // Every node starts dirty: its inverse rest covariance before the first substep.
+        {
+            std::vector<WgMap> m;
+            const uint32_t slices = std::max(1U, std::min(4096U, (sl.N + kWG - 1) / kWG));
+            for (uint32_t q = 0; q < slices; ++q) m.push_back(WgMap{k, q, slices, 0});
+            upload_map_.alloc(m.size() * sizeof(WgMap));
+            HIP_OK(hipMemcpy(upload_map_.p, m.data(), m.size() * sizeof(WgMap), hipMemcpyHostToDevice));
+            const Batch<R> b{ls_.as<DevLattice<R>>(), upload_map_.as<WgMap>(), uint32_t(m.size())};
+            hipLaunchKernelGGL((k_rinv<R>), dim3(uint32_t(m.size())), dim3(kWG), 0, stream_, b);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipStreamSynchronize(stream_));
+        }
+        maps_dirty_ = true;
+    }
+
+    // The slot leaves the batch and keeps its arena for the next upload.
+    void release(uint32_t k) {
+        Slot<R> &sl = slot(k);
+        HIP_OK(hipStreamSynchronize(stream_));
+        sl.uploaded = false;
+        sl.armed = false;
+        maps_dirty_ = true;
+        updateShared();
     }
 
     // ExternalLoads.hpp:28-50: validate the whole field, then replace the load.
-    void setForces(const double *f, uint64_t substeps) override {
-        if (!uploaded_) throw Logic("external forces need an uploaded lattice");
+    void setForces(uint32_t k, const double *f, uint64_t substeps) {
+        Slot<R> &sl = slot(k);
+        if (!sl.uploaded) throw Logic("external forces need an uploaded lattice");
         HIP_OK(hipSetDevice(device_));
+        HIP_OK(hipStreamSynchronize(stream_));
         if (f == nullptr) {
             if (substeps != 0) throw Invalid("empty external load needs zero substeps");
-            setRemaining(0);
+            setRemaining(k, 0);
+            sl.loads_set = false;
+            maps_dirty_ = true;
             return;
         }
         if (substeps == 0) throw Invalid("external load needs one force per node and a finite nonzero span");
-        std::vector<R4<R>> fe(N_);
-        std::vector<double> fr(3 * size_t(N_));
+        const uint32_t N = sl.N;
+        std::vector<R4<R>> fe(N);
+        std::vector<double> fr(3 * size_t(N));
         bool any = false;
-        for (uint32_t i = 0; i < N_; ++i) {
+        for (uint32_t i = 0; i < N; ++i) {
             const double x = f[3 * size_t(i)], y = f[3 * size_t(i) + 1], z = f[3 * size_t(i) + 2];
             const R rx = R(x), ry = R(y), rz = R(z);
             if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z) || !std::isfinite(std::hypot(x, y, z)) ||
@@ -440,8 +779,8 @@ public:
                 fe[i] = {R(0), R(0), R(0), R(0)};
                 continue;
             }
-            if (!(mass_host_[i] > 0) || !(inv_mass_host_[i] > 0) || !std::isfinite(mass_host_[i]) ||
-                !std::isfinite(inv_mass_host_[i]))
+            if (!(sl.mass_host[i] > 0) || !(sl.inv_mass_host[i] > 0) || !std::isfinite(sl.mass_host[i]) ||
+                !std::isfinite(sl.inv_mass_host[i]))
                 throw Invalid("external force needs a finite positive movable node mass");
             fe[i] = {rx, ry, rz, R(1)};
             fr[3 * size_t(i)] = x;
@@ -449,19 +788,24 @@ public:
             fr[3 * size_t(i) + 2] = z;
             any = true;
         }
-        HIP_OK(hipMemcpy(fext_.p, fe.data(), fe.size() * sizeof(R4<R>), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(freq_.p, fr.data(), fr.size() * sizeof(double), hipMemcpyHostToDevice));
-        loads_set_ = any;
-        setRemaining(any ? substeps : 0);
+        HIP_OK(hipMemcpy(sl.L.fext, fe.data(), fe.size() * sizeof(R4<R>), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(const_cast<double *>(sl.L.freq), fr.data(), fr.size() * sizeof(double), hipMemcpyHostToDevice));
+        sl.loads_set = any;
+        maps_dirty_ = true;
+        setRemaining(k, any ? substeps : 0);
     }
 
-    void run(const blx_run_control &c, blx_run_status &out) override {
-        if (!uploaded_) throw Logic("run before upload");
+    // The run control of a slot's next advance: its exit rules and its step
+    // budget (RunControl, FastLattice.hpp:79-128).
+    void arm(uint32_t k, const blx_run_control &c) {
+        Slot<R> &sl = slot(k);
+        if (!sl.uploaded) throw Logic("run before upload");
         HIP_OK(hipSetDevice(device_));
-        const auto t0 = std::chrono::steady_clock::now();
-        DevCtl ctl = readCtl();
-        ctl.stop = 0;
+        HIP_OK(hipStreamSynchronize(stream_));
+        DevCtl ctl = readCtl(k);
+        ctl.stop = c.max_steps == 0 ? 1U : 0U;
         ctl.exit_reason = 0;
+        ctl.budget = c.max_steps;
         ctl.quiet = c.quiet_steps;
         ctl.min_steps = c.min_steps;
         ctl.no_failure = c.no_failure_steps;
@@ -470,52 +814,101 @@ public:
         ctl.flat_fraction = c.energy_flat_fraction;
         ctl.calm_margin = c.calm_damage_margin;
         ctl.removable = c.removable_energy_j;
-        writeCtl(ctl);
-        const uint64_t batch = profilingBatch(c.steps_per_launch ? c.steps_per_launch : kDefaultBatch);
+        writeCtl(k, ctl);
+        sl.ctl_host = ctl;
+        sl.armed = c.max_steps > 0;
+        maps_dirty_ = true;
+    }
+
+    // Advances every armed slot until it stops (its exit rule or its budget)
+    // or `substeps` substeps have been enqueued; returns the number of slots
+    // still running. Abandon is polled between device batches.
+    uint32_t advance(uint64_t substeps, uint64_t batch_len, const volatile uint8_t *abandon, double *kernel) {
+        HIP_OK(hipSetDevice(device_));
         uint64_t done = 0;
-        unsigned reason = 0;
-        while (done < c.max_steps) {
-            const uint64_t step = ctl.step;
-            uint64_t n = std::min<uint64_t>(batch, c.max_steps - done);
-            if (c.capture_stride != 0) {
-                if (step % c.capture_stride == 0 && run_status_.frames_captured < c.max_frames) {
-                    captureFrame(step);
-                    ++run_status_.frames_captured;
-                }
-                const uint64_t to_next = c.capture_stride - step % c.capture_stride;
-                n = std::min(n, to_next);
-            }
+        uint32_t running = countRunning();
+        double ksec = 0.0;
+        while (running > 0 && done < substeps) {
+            if (maps_dirty_) buildMaps();
+            const uint64_t n = std::min<uint64_t>(profilingBatch(batch_len), substeps - done);
             HIP_OK(hipEventRecord(ev0_, stream_));
-            for (uint64_t k = 0; k < n; ++k) enqueueSubstep();
+            for (uint64_t q = 0; q < n; ++q) enqueueSubstep();
             HIP_OK(hipEventRecord(ev1_, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
             float ms = 0.f;
             HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
-            run_status_.kernel_seconds += double(ms) * 1e-3;
+            ksec += double(ms) * 1e-3;
             profCollect();
-            ++run_status_.launches;
-            ctl = readCtl();
-            done += ctl.step - step;
-            if (ctl.error) throw Overflow("external force kick exceeds backend precision");
-            if (ctl.stop) {
-                reason = ctl.exit_reason;
+            ++launches_;
+            done += n;
+            readAllCtls();
+            uint32_t now = 0;
+            for (auto &s : slots_) {
+                Slot<R> &sl = *s;
+                if (!sl.armed) continue;
+                if (sl.ctl_host.error) throw Overflow("external force kick exceeds backend precision");
+                sl.status.kernel_seconds += double(ms) * 1e-3;
+                ++sl.status.launches;
+                if (sl.ctl_host.stop) {
+                    sl.armed = false;
+                    maps_dirty_ = true;
+                } else {
+                    ++now;
+                }
+            }
+            running = now;
+            if (abandon && *abandon) break;
+        }
+        if (kernel) *kernel = ksec;
+        return running;
+    }
+
+    // LatticeBackend::run on slot 0 of a single-lattice engine, with frames.
+    void run(const blx_run_control &c, blx_run_status &out) {
+        Slot<R> &sl = slot(0);
+        if (!sl.uploaded) throw Logic("run before upload");
+        HIP_OK(hipSetDevice(device_));
+        const auto t0 = std::chrono::steady_clock::now();
+        arm(0, c);
+        const uint64_t batch = c.steps_per_launch ? c.steps_per_launch : kDefaultBatch;
+        uint64_t done = 0;
+        unsigned reason = 0;
+        while (done < c.max_steps) {
+            const uint64_t step = sl.ctl_host.step;
+            uint64_t n = c.max_steps - done;
+            if (c.capture_stride != 0) {
+                if (step % c.capture_stride == 0 && sl.status.frames_captured < c.max_frames) {
+                    captureFrame(0, step);
+                    ++sl.status.frames_captured;
+                }
+                n = std::min(n, c.capture_stride - step % c.capture_stride);
+            }
+            advance(n, batch, c.abandon, nullptr);
+            done += sl.ctl_host.step - step;
+            if (sl.ctl_host.stop && sl.ctl_host.exit_reason != kBudgetSpent) {
+                reason = sl.ctl_host.exit_reason;
                 break;
             }
-            if (c.abandon && *c.abandon) {
+            if (c.abandon && *c.abandon && done < c.max_steps) {
                 reason = 7;
                 break;
             }
         }
         if (reason == 0 && done >= c.max_steps) reason = 3;
-        run_status_.exit_reason = reason;
-        run_status_.wall_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        ctl_host_ = ctl;
-        status(out);
+        if (sl.armed) {
+            sl.armed = false;
+            maps_dirty_ = true;
+        }
+        sl.status.exit_reason = reason;
+        sl.status.wall_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        status(0, out);
+        out.exit_reason = reason;
     }
 
-    void status(blx_run_status &o) const override {
-        const DevCtl &c = ctl_host_;
-        o = run_status_;
+    void status(uint32_t k, blx_run_status &o) const {
+        const Slot<R> &sl = slot(k);
+        const DevCtl &c = sl.ctl_host;
+        o = sl.status;
         o.total_steps = c.step;
         o.first_failure_step = c.first_failure_step;
         o.last_failure_step = c.last_failure_step;
@@ -526,7 +919,7 @@ public:
         o.max_damage = c.max_damage;
         o.last_damage_gain_step = c.last_damage_gain_step;
         o.contact.impulse_contacts = c.impulse_contacts;
-        for (int k = 0; k < 3; ++k) o.contact.impulse_to_material[k] = c.impulse[k];
+        for (int q = 0; q < 3; ++q) o.contact.impulse_to_material[q] = c.impulse[q];
         o.contact.dissipated_kinetic_energy_j = c.dissipated;
         o.contact.maximum_penetration_m = c.max_penetration;
         o.contact.maximum_position_correction_m = c.max_correction;
@@ -535,11 +928,11 @@ public:
         o.contact.ball_support_events = c.ball_support_events;
         o.external_load.steps = c.ledger_steps;
         o.external_load.elapsed_s = c.ledger_elapsed;
-        for (int k = 0; k < 3; ++k) {
-            o.external_load.requested_impulse_n_s[k] = c.ledger_req[k];
-            o.external_load.impulse_n_s[k] = c.ledger_imp[k];
-            o.external_load.requested_angular_impulse_kg_m2_s[k] = c.ledger_areq[k];
-            o.external_load.angular_impulse_kg_m2_s[k] = c.ledger_aimp[k];
+        for (int q = 0; q < 3; ++q) {
+            o.external_load.requested_impulse_n_s[q] = c.ledger_req[q];
+            o.external_load.impulse_n_s[q] = c.ledger_imp[q];
+            o.external_load.requested_angular_impulse_kg_m2_s[q] = c.ledger_areq[q];
+            o.external_load.angular_impulse_kg_m2_s[q] = c.ledger_aimp[q];
         }
         o.external_load.work_j = c.ledger_work;
         std::memcpy(&o.max_tensile_stretch, &c.peak_bits[0], 4);
@@ -548,122 +941,142 @@ public:
         std::memcpy(&o.max_plastic_stretch, &c.pstretch_bits, 4);
         o.rank_deficient_nodes = c.rank_deficient;
         o.plastic_work_j = c.plastic_work;
+        // A batch slot reports how its last advance ended; a spent budget is
+        // the reference's max-steps reason.
+        if (c.stop && c.exit_reason) o.exit_reason = c.exit_reason == kBudgetSpent ? 3U : c.exit_reason;
     }
 
-    void download(blx_state_view &st, blx_sphere &sp) override {
-        if (!uploaded_) throw Logic("download before upload");
+    void download(uint32_t k, blx_state_view &st, blx_sphere &sp) {
+        Slot<R> &sl = slot(k);
+        if (!sl.uploaded) throw Logic("download before upload");
         HIP_OK(hipSetDevice(device_));
         HIP_OK(hipStreamSynchronize(stream_));
-        std::vector<R4<R>> u4(N_), up4(N_), v4(N_), pvd(B_);
-        std::vector<R> pext(B_), pstr(B_);
-        std::vector<uint8_t> mode(B_);
-        copyDown(u4, u4_);
-        copyDown(up4, up4_);
-        copyDown(v4, v4_);
-        copyDown(pvd, pvd_);
-        copyDown(pext, pext_);
-        copyDown(pstr, pstr_);
-        copyDown(mode, mode_);
-        HIP_OK(hipMemcpy(st.alive, alive_.p, B_, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < N_; ++i) {
-            st.u[3 * size_t(i)] = double(u4[i].x);
-            st.u[3 * size_t(i) + 1] = double(u4[i].y);
-            st.u[3 * size_t(i) + 2] = double(u4[i].z);
-            st.u_prev[3 * size_t(i)] = double(up4[i].x);
-            st.u_prev[3 * size_t(i) + 1] = double(up4[i].y);
-            st.u_prev[3 * size_t(i) + 2] = double(up4[i].z);
-            st.v[3 * size_t(i)] = double(v4[i].x);
-            st.v[3 * size_t(i) + 1] = double(v4[i].y);
-            st.v[3 * size_t(i) + 2] = double(v4[i].z);
+        const uint32_t N = sl.N, B = sl.B;
+        std::vector<R4<R>> u4(N), up4(N), v4(N), pvd(B);
+        std::vector<R> pext(B), pstr(B);
+        std::vector<uint8_t> mode(B);
+        copyDown(u4, sl.L.u4);
+        copyDown(up4, sl.L.up4);
+        copyDown(v4, sl.L.v4);
+        copyDown(pvd, sl.L.pvd);
+        copyDown(pext, sl.L.pext);
+        copyDown(pstr, sl.L.pstr);
+        copyDown(mode, sl.L.mode);
+        HIP_OK(hipMemcpy(st.alive, sl.L.alive, B, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < N; ++i) {
+            const size_t q = 3 * size_t(i);
+            st.u[q] = double(u4[i].x);
+            st.u[q + 1] = double(u4[i].y);
+            st.u[q + 2] = double(u4[i].z);
+            st.u_prev[q] = double(up4[i].x);
+            st.u_prev[q + 1] = double(up4[i].y);
+            st.u_prev[q + 2] = double(up4[i].z);
+            st.v[q] = double(v4[i].x);
+            st.v[q + 1] = double(v4[i].y);
+            st.v[q + 2] = double(v4[i].z);
         }
-        for (uint32_t k = 0; k < B_; ++k) st.plastic_extension[k] = double(pext[k]);
+        for (uint32_t q = 0; q < B; ++q) st.plastic_extension[q] = double(pext[q]);
         // The end-sample arrays are node-major on the device.
-        for (uint32_t o = 0; o < B_; ++o) {
-            const uint32_t k = node_j_host_[o];
-            st.prev_tensile[k] = double(pvd[o].x);
-            st.prev_compressive[k] = double(pvd[o].y);
-            st.prev_shear[k] = double(pvd[o].z);
-            st.damage[k] = double(pvd[o].w);
-            st.plastic_strain[k] = double(pstr[o]);
-            st.failure_mode[k] = mode[o];
+        for (uint32_t o = 0; o < B; ++o) {
+            const uint32_t q = sl.node_j_host[o];
+            st.prev_tensile[q] = double(pvd[o].x);
+            st.prev_compressive[q] = double(pvd[o].y);
+            st.prev_shear[q] = double(pvd[o].z);
+            st.damage[q] = double(pvd[o].w);
+            st.plastic_strain[q] = double(pstr[o]);
+            st.failure_mode[q] = mode[o];
         }
-        sphereOut(readSphere(), sp);
+        sphereOut(readSphere(k), sp);
     }
 
-    uint64_t firstFailure(uint32_t *out, uint64_t cap) const override {
-        const uint64_t n = ctl_host_.first_failure_count;
-        if (out && cap) HIP_OK(hipMemcpy(out, first_fail_.p, std::min(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint64_t firstFailure(uint32_t k, uint32_t *out, uint64_t cap) const {
+        const Slot<R> &sl = slot(k);
+        const uint64_t n = sl.ctl_host.first_failure_count;
+        if (out && cap)
+            HIP_OK(hipMemcpy(out, sl.L.first_fail, std::min(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost));
         return n;
     }
 
-    uint32_t frameCount() const override { return uint32_t(frames_.size()); }
-    void takeFrame(uint32_t k, uint64_t *step, float *u, uint8_t *alive, float *damage, blx_sphere *sp) override {
-        if (k >= frames_.size()) throw Invalid("frame index out of range");
-        const HostFrame<R> &f = frames_[k];
-        *step = f.step;
-        std::copy(f.u.begin(), f.u.end(), u);
-        std::copy(f.alive.begin(), f.alive.end(), alive);
-        std::copy(f.damage.begin(), f.damage.end(), damage);
-        if (sp) sphereOut(f.sphere, *sp);
+    uint32_t frameCount(uint32_t k) const { return uint32_t(slot(k).frames.size()); }
+    void takeFrame(uint32_t k, uint32_t f, uint64_t *step, float *u, uint8_t *alive, float *damage, blx_sphere *sp) {
+        const Slot<R> &sl = slot(k);
+        if (f >= sl.frames.size()) throw Invalid("frame index out of range");
+        const HostFrame<R> &fr = sl.frames[f];
+        *step = fr.step;
+        std::copy(fr.u.begin(), fr.u.end(), u);
+        std::copy(fr.alive.begin(), fr.alive.end(), alive);
+        std::copy(fr.damage.begin(), fr.damage.end(), damage);
+        if (sp) sphereOut(fr.sphere, *sp);
     }
-    void clearFrames() override {
-        frames_.clear();
-        run_status_.frames_captured = 0;
+    void clearFrames(uint32_t k) {
+        slot(k).frames.clear();
+        slot(k).status.frames_captured = 0;
     }
 
-    uint32_t labels(uint32_t *label) override {
-        if (!uploaded_) throw Logic("fragments before upload");
+    // Union-find, roots, canonical rank and labels of one slot, on the device.
+    uint32_t labels(uint32_t k, uint32_t *label) {
+        Slot<R> &sl = slot(k);
+        if (!sl.uploaded) throw Logic("fragments before upload");
         HIP_OK(hipSetDevice(device_));
-        const uint32_t c = labelsOnDevice();
-        HIP_OK(hipMemcpyAsync(label, cc_label_.p, size_t(N_) * 4, hipMemcpyDeviceToHost, stream_));
         HIP_OK(hipStreamSynchronize(stream_));
-        return c;
-    }
-
-    // Union-find, roots, canonical rank and labels, all on the device; the
-    // labels are left in cc_label_. Returns the component count.
-    uint32_t labelsOnDevice() {
-        const size_t n4 = size_t(N_) * 4;
-        cc_parent_.alloc(n4);
-        cc_root_.alloc(n4);
-        cc_size_.alloc(n4);
-        cc_rank_.alloc(n4);
-        cc_label_.alloc(n4);
-        cc_keys_.alloc(size_t(N_) * 8);
-        cc_sorted_.alloc(size_t(N_) * 8);
-        cc_count_.alloc(4);
+        const uint32_t N = sl.N;
+        const size_t n4 = size_t(N) * 4;
+        sl.cc_parent.alloc(n4);
+        sl.cc_root.alloc(n4);
+        sl.cc_size.alloc(n4);
+        sl.cc_rank.alloc(n4);
+        sl.cc_label.alloc(n4);
+        sl.cc_keys.alloc(size_t(N) * 8);
+        sl.cc_sorted.alloc(size_t(N) * 8);
+        sl.cc_count.alloc(4);
         size_t temp = 0;
-        HIP_OK(rocprim::radix_sort_keys(nullptr, temp, cc_keys_.as<uint64_t>(), cc_sorted_.as<uint64_t>(), N_, 0, 64,
-                                        stream_));
-        cc_temp_.alloc(temp);
-        const uint32_t g = gridFor(N_);
-        auto *parent = cc_parent_.as<uint32_t>(), *root = cc_root_.as<uint32_t>(), *size = cc_size_.as<uint32_t>();
-        HIP_OK(hipMemsetAsync(cc_count_.p, 0, 4, stream_));
-        hipLaunchKernelGGL(k_cc_init, dim3(g), dim3(kWG), 0, stream_, parent, size, N_);
-        hipLaunchKernelGGL(k_cc_hook<R>, dim3(gridFor(B_)), dim3(kWG), 0, stream_, L_, parent);
-        hipLaunchKernelGGL(k_cc_roots, dim3(g), dim3(kWG), 0, stream_, parent, root, size, N_);
-        hipLaunchKernelGGL(k_cc_keys, dim3(g), dim3(kWG), 0, stream_, root, size, cc_keys_.as<uint64_t>(),
-                           cc_count_.as<uint32_t>(), N_);
+        uint64_t *keys = sl.cc_keys.template as<uint64_t>(), *sorted = sl.cc_sorted.template as<uint64_t>();
+        HIP_OK(rocprim::radix_sort_keys(nullptr, temp, keys, sorted, N, 0, 64, stream_));
+        sl.cc_temp.alloc(temp);
+        const uint32_t g = gridFor(N);
+        uint32_t *parent = sl.cc_parent.template as<uint32_t>(), *root = sl.cc_root.template as<uint32_t>(),
+                 *size = sl.cc_size.template as<uint32_t>(), *count = sl.cc_count.template as<uint32_t>();
+        HIP_OK(hipMemsetAsync(count, 0, 4, stream_));
+        hipLaunchKernelGGL(k_cc_init, dim3(g), dim3(kWG), 0, stream_, parent, size, N);
+        hipLaunchKernelGGL(k_cc_hook<R>, dim3(gridFor(sl.B)), dim3(kWG), 0, stream_, sl.L, parent);
+        hipLaunchKernelGGL(k_cc_roots, dim3(g), dim3(kWG), 0, stream_, parent, root, size, N);
+        hipLaunchKernelGGL(k_cc_keys, dim3(g), dim3(kWG), 0, stream_, root, size, keys, count, N);
         HIP_OK(hipGetLastError());
         uint32_t c = 0;
-        HIP_OK(hipMemcpyAsync(&c, cc_count_.p, 4, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipMemcpyAsync(&c, count, 4, hipMemcpyDeviceToHost, stream_));
         HIP_OK(hipStreamSynchronize(stream_));
-        size_t bytes = cc_temp_.bytes;
-        HIP_OK(rocprim::radix_sort_keys(cc_temp_.p, bytes, cc_keys_.as<uint64_t>(), cc_sorted_.as<uint64_t>(), c, 0, 64,
-                                        stream_));
-        hipLaunchKernelGGL(k_cc_rank, dim3(gridFor(c)), dim3(kWG), 0, stream_, cc_sorted_.as<uint64_t>(),
-                           cc_rank_.as<uint32_t>(), c);
-        hipLaunchKernelGGL(k_cc_label, dim3(g), dim3(kWG), 0, stream_, root, cc_rank_.as<uint32_t>(),
-                           cc_label_.as<uint32_t>(), N_);
+        size_t bytes = sl.cc_temp.bytes;
+        HIP_OK(rocprim::radix_sort_keys(sl.cc_temp.p, bytes, keys, sorted, c, 0, 64, stream_));
+        hipLaunchKernelGGL(k_cc_rank, dim3(gridFor(c)), dim3(kWG), 0, stream_, sorted, sl.cc_rank.template as<uint32_t>(), c);
+        hipLaunchKernelGGL(k_cc_label, dim3(g), dim3(kWG), 0, stream_, root, sl.cc_rank.template as<uint32_t>(),
+                           sl.cc_label.template as<uint32_t>(), N);
         HIP_OK(hipGetLastError());
+        if (label) HIP_OK(hipMemcpyAsync(label, sl.cc_label.p, n4, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
         return c;
+    }
+
+    // Device bytes one slot holds (its arena).
+    size_t slotBytes(uint32_t k) const { return slot(k).arena.bytes(); }
+
+    // Per-launch HIP-event timing of every kernel kind (off by default).
+    void setProfiling(bool on) { prof_ = on; }
+    uint32_t kernelStats(uint32_t k, std::string &name, double &seconds, uint64_t &launches) const {
+        if (k < KCount) {
+            name = kernelName(k);
+            seconds = kernel_seconds_[k];
+            launches = kernel_launches_[k];
+        }
+        return KCount;
     }
 
 private:
     static constexpr uint64_t kDefaultBatch = 240;
 
     int device_;
-    Schedule sch_;
+    std::vector<std::unique_ptr<Slot<R>>> slots_;
+    Shared shared_;
+    Step<R> S_{};
     // stream_ carries the substep; side_ the inverse rest covariances that
     // overlap the folds; early_ the end sample of the nodes and bonds striker
     // pass 2 cannot reach, which overlaps pass 2.
@@ -672,235 +1085,164 @@ private:
                ev_early_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t cus_ = 256;
-    uint32_t N_ = 0, B_ = 0, D_ = 0, iterations_ = 1;
-    uint32_t node_tiles_ = 0, bond_tiles_ = 0, group_words_ = 0;
-    uint32_t slot_palette_ = 0, edge_palette_ = 0;
-    bool direct_ = false, plastic_ = false, uploaded_ = false, loads_set_ = false, tensile_only_ = false;
-    std::vector<uint32_t> node_j_host_;  // node-major bond position -> schedule index
-    Step<R> S_{};
-    double origin_[3]{};
-    std::vector<R> mass_host_, inv_mass_host_;
-    DevLattice<R> L_{};
-    DevCtl ctl_host_{};
-    blx_run_status run_status_{};
-    std::vector<HostFrame<R>> frames_;
-    // Stage tables: per colour stage, a run of workgroups over [begin, end) bond ranges.
-    std::vector<uint2> wg_host_;
-    std::vector<std::pair<uint32_t, uint32_t>> stages_; // (first workgroup, workgroup count)
-    size_t interior_stages_ = 0;
-    DevBuf wg_;
-    DevBuf x04_, u4_, up4_, v4_, rinv_, unm_, sa_, sb_, valid_, approach_, engaged_, cand_;
-    DevBuf affected_, affected_list_;
-    DevBuf rank_stamp_, dirty_list_, dirty_stamp_, cand_tile_, nbr_bond_, ev_bits_, brk_bits_, grp_bits_, ev_d_, ev_e_;
-    DevBuf flag_list_, nbr_key_, spal_, sdelta_, epal_;
-    DevBuf rec_, alive_, lambda_, pext_, bnd_, alive_n_, pstr_, thr_, thr2_, pvd_, mode_, slots_;
-    DevBuf tile_cand_, tile_plastic_, load_part_, fext_, freq_, cand_all_, cand_lo_, cand_cnt_, nbb_;
-    DevBuf first_fail_, ctl_, sphere_;
-    DevBuf cc_parent_, cc_root_, cc_size_, cc_rank_, cc_label_, cc_keys_, cc_sorted_, cc_count_, cc_temp_;
+    uint64_t launches_ = 0;
+    DevBuf ls_, ctls_, spheres_, upload_map_;
+    std::vector<DevCtl> ctl_all_;
 
-    template <class T> void copyDown(std::vector<T> &dst, const DevBuf &src) {
-        if (!dst.empty()) HIP_OK(hipMemcpy(dst.data(), src.p, dst.size() * sizeof(T), hipMemcpyDeviceToHost));
-    }
+    // Launch maps of the running slots, all in one device buffer.
+    struct MapRef {
+        uint32_t offset = 0, count = 0;
+    };
+    bool maps_dirty_ = true;
+    DevBuf maps_;
+    MapRef node_tiles_map_, bond_tiles_map_, list_map_, small_map_, mark_map_, lattice_map_;
+    std::vector<MapRef> stage_maps_;
+    size_t interior_stages_ = 0;
+    bool any_loads_ = false;
 
     uint32_t gridFor(uint32_t n) const {
         const uint32_t need = (n + kWG - 1) / kWG;
         return std::max(1U, std::min(need, cus_ * 16U));
     }
 
-    void validate(const blx_state_view &st, const blx_settings &s) const {
-        if (s.bond_integrator == BLX_BOND_VELOCITY_VERLET)
-            throw Invalid("the Verlet bond integrator is the serial double CPU reference only");
-        if (s.bond_integrator != BLX_BOND_XPBD) throw Invalid("unknown bond integrator");
-        if (s.node_contact.mode != BLX_NODE_CONTACT_OFF) throw Invalid("node-node contact is not available on the HIP backend");
-        if (s.plate_half_thickness != 0.0) throw Invalid("plate bending is not available on the HIP backend");
-        if (s.audit_energy) throw Invalid("the kinetic energy audit is not available on the HIP backend");
-        if (s.plane_count > BLX_MAX_SUPPORT_PLANES) throw Invalid("at most two support planes");
-        for (uint32_t p = 0; p < s.plane_count; ++p)
-            if (s.planes[p].footprint_count > BLX_MAX_FOOTPRINTS) throw Invalid("at most four footprints per plane");
-        if (s.constraint_iterations == 0) throw Invalid("at least one constraint iteration");
-        if (st.node_count != sch_.N || st.bond_count != sch_.B) throw Invalid("state does not match the schedule");
-        if (size_t(st.max_degree) * st.node_count > 0xffffffffULL) throw Invalid("neighbour slots exceed 32-bit indices");
-        if (st.bond_count >= 0x80000000U) throw Invalid("at most 2^31 - 1 bonds");
-        if (!(s.dt > 0.0) || !std::isfinite(s.dt)) throw Invalid("the substep must be finite and positive");
-        if (s.sphere_enabled && (st.node_count + kWG - 1) / kWG > kMaxNodeTiles)
-            throw Invalid("the striker pass holds at most 4,194,304 nodes");
+    uint32_t countRunning() const {
+        uint32_t n = 0;
+        for (auto &s : slots_) n += s->armed ? 1U : 0U;
+        return n;
     }
 
-    static Step<R> convertStep(const blx_settings &s) {
-        Step<R> o{};
-        o.dt = R(s.dt);
-        o.gravity = {R(s.gravity[0]), R(s.gravity[1]), R(s.gravity[2])};
-        o.damping = R(s.damping_fraction);
-        o.yield = R(s.plastic_yield_stretch);
-        o.hardening = R(s.plastic_hardening);
-        o.c_sf = R(s.contact.static_friction);
-        o.c_df = R(s.contact.dynamic_friction);
-        o.c_rest = R(s.contact.restitution);
-        o.c_rest_thr = R(s.contact.restitution_speed_threshold);
-        o.c_node_radius = R(s.contact.node_contact_radius);
-        o.c_margin = R(s.contact.contact_margin);
-        o.c_slack = R(s.contact.prefilter_slack);
-        o.plane_count = s.plane_count;
-        o.sphere_enabled = s.sphere_enabled;
-        for (int p = 0; p < 2; ++p) {
-            const blx_support_plane &in = s.planes[p];
-            Plane<R> &pl = o.planes[p];
-            pl.point = {R(in.point[0]), R(in.point[1]), R(in.point[2])};
-            pl.normal = {R(in.normal[0]), R(in.normal[1]), R(in.normal[2])};
-            pl.tangent = {R(in.tangent[0]), R(in.tangent[1]), R(in.tangent[2])};
-            pl.bitangent = {R(in.bitangent[0]), R(in.bitangent[1]), R(in.bitangent[2])};
-            pl.restitution = R(in.restitution);
-            pl.static_friction = R(in.static_friction);
-            pl.dynamic_friction = R(in.dynamic_friction);
-            pl.node_radius = R(in.node_radius);
-            pl.reach_capped = in.reach_capped;
-            pl.footprint_count = in.footprint_count;
-            for (int f = 0; f < 4; ++f) {
-                pl.fp[f][0] = R(in.footprints[f].center_t);
-                pl.fp[f][1] = R(in.footprints[f].center_b);
-                pl.fp[f][2] = R(in.footprints[f].half_t);
-                pl.fp[f][3] = R(in.footprints[f].half_b);
-            }
-        }
-        return o;
+    void updateShared() {
+        bool any = false;
+        for (auto &s : slots_) any |= s->uploaded;
+        if (!any) shared_.set = false;
     }
 
-    // Colour stages in sweep order (CpuLatticeBackend.cpp:395-404): interior
-    // bonds of every block colour by colour, then the boundary bonds of even
-    // blocks, then of odd blocks. Interior bonds of different blocks share no
-    // node, nor do boundary bonds of blocks of one parity, so the ranges of
-    // one colour across those blocks form one node-disjoint stage.
-    void buildStages() {
-        const uint32_t cc = std::max(sch_.colors, 1U);
-        auto add = [&](uint32_t boundary, uint32_t parity_first, uint32_t step) {
-            for (uint32_t c = 0; c < sch_.colors; ++c) {
-                const uint32_t first = uint32_t(wg_host_.size());
-                for (uint32_t b = parity_first; b < sch_.blocks; b += step) {
-                    const size_t key = (size_t(b) * 2 + boundary) * cc + c;
-                    const uint32_t lo = sch_.range_begin[key], hi = sch_.range_end[key];
-                    for (uint32_t j = lo; j < hi; j += kWG) wg_host_.push_back(make_uint2(j, std::min(hi, j + kWG)));
-                }
-                const uint32_t count = uint32_t(wg_host_.size()) - first;
-                if (count) stages_.push_back({first, count});
+    void buildMaps() {
+        std::vector<WgMap> all;
+        auto begin = [&](MapRef &r) { r.offset = uint32_t(all.size()); };
+        auto end = [&](MapRef &r) { r.count = uint32_t(all.size()) - r.offset; };
+        std::vector<uint32_t> run;
+        uint64_t total_nodes = 0;
+        any_loads_ = false;
+        for (uint32_t k = 0; k < slots_.size(); ++k)
+            if (slots_[k]->armed) {
+                run.push_back(k);
+                total_nodes += slots_[k]->N;
+                any_loads_ |= slots_[k]->loads_set;
             }
+        // List launches share a workgroup budget among the slots by size.
+        auto slices = [&](uint32_t n, uint32_t cap) {
+            const uint64_t need = std::max(1U, (n + kWG - 1) / kWG);
+            const uint64_t fair = total_nodes ? (uint64_t(cap) * n + total_nodes - 1) / total_nodes : cap;
+            return uint32_t(std::max<uint64_t>(1, std::min(need, fair)));
         };
-        add(0, 0, 1);
-        interior_stages_ = stages_.size();
-        add(1, 0, 2);
-        add(1, 1, 2);
-        toDevice(wg_, wg_host_);
+        begin(node_tiles_map_);
+        for (uint32_t k : run)
+            for (uint32_t t = 0; t < slots_[k]->node_tiles; ++t) all.push_back(WgMap{k, t, 0, 0});
+        end(node_tiles_map_);
+        begin(bond_tiles_map_);
+        for (uint32_t k : run)
+            for (uint32_t t = 0; t < slots_[k]->bond_tiles; ++t) all.push_back(WgMap{k, t, 0, 0});
+        end(bond_tiles_map_);
+        begin(list_map_);
+        for (uint32_t k : run) {
+            const uint32_t s = slices(slots_[k]->N, cus_ * 16U);
+            for (uint32_t q = 0; q < s; ++q) all.push_back(WgMap{k, q, s, 0});
+        }
+        end(list_map_);
+        begin(small_map_);
+        for (uint32_t k : run) {
+            const uint32_t s = slices(slots_[k]->N, cus_ * 4U);
+            for (uint32_t q = 0; q < s; ++q) all.push_back(WgMap{k, q, s, 0});
+        }
+        end(small_map_);
+        begin(mark_map_);
+        for (uint32_t k : run) {
+            const Slot<R> &sl = *slots_[k];
+            const uint64_t work = uint64_t(sl.sch.blocks) * kCap * (sl.D + 1);
+            const uint64_t cap = std::max<uint64_t>(1, cus_ * 4U / std::max<size_t>(1, run.size()));
+            const uint32_t s = uint32_t(std::max<uint64_t>(1, std::min((work + kWG - 1) / kWG, cap)));
+            for (uint32_t q = 0; q < s; ++q) all.push_back(WgMap{k, q, s, 0});
+        }
+        end(mark_map_);
+        begin(lattice_map_);
+        for (uint32_t k : run) all.push_back(WgMap{k, 0, 0, 0});
+        end(lattice_map_);
+        // Stage s of the batch is stage s of every running slot: lattices are
+        // independent, so only each slot's own stage order matters.
+        size_t stages = 0;
+        interior_stages_ = 0;
+        for (uint32_t k : run) {
+            stages = std::max(stages, slots_[k]->stages.size());
+            interior_stages_ = std::max(interior_stages_, slots_[k]->interior_stages);
+        }
+        stage_maps_.assign(stages, MapRef{});
+        for (size_t s = 0; s < stages; ++s) {
+            begin(stage_maps_[s]);
+            for (uint32_t k : run) {
+                const Slot<R> &sl = *slots_[k];
+                if (s < sl.stages.size())
+                    for (const uint2 &r : sl.stages[s]) all.push_back(WgMap{k, r.x, r.y, 0});
+            }
+            end(stage_maps_[s]);
+        }
+        maps_.alloc(std::max<size_t>(1, all.size()) * sizeof(WgMap));
+        if (!all.empty()) HIP_OK(hipMemcpy(maps_.p, all.data(), all.size() * sizeof(WgMap), hipMemcpyHostToDevice));
+        maps_dirty_ = false;
     }
 
-    void bind() {
-        DevLattice<R> &L = L_;
-        L.N = N_;
-        L.B = B_;
-        L.D = D_;
-        L.blocks = sch_.blocks;
-        L.node_tiles = node_tiles_;
-        L.bond_tiles = bond_tiles_;
-        L.group_words = group_words_;
-        L.tensile_only = tensile_only_ ? 1U : 0U;
-        L.u4 = u4_.as<R4<R>>();
-        L.up4 = up4_.as<R4<R>>();
-        L.v4 = v4_.as<R4<R>>();
-        L.x04 = x04_.as<R4<R>>();
-        L.rinv = rinv_.as<R>();
-        L.unm = unm_.as<R>();
-        L.sa = sa_.as<R4<R>>();
-        L.sb = sb_.as<R4<R>>();
-        L.valid = valid_.as<uint8_t>();
-        L.approach = approach_.as<R>();
-        L.engaged = engaged_.as<uint8_t>();
-        L.cand = cand_.as<uint8_t>();
-        L.nbr_key = nbr_key_.as<uint32_t>();
-        L.spal = spal_.as<R4<R>>();
-        L.sdelta = sdelta_.as<int>();
-        L.nbr_bond = nbr_bond_.as<uint32_t>();
-        L.epal = epal_.as<EdgePal<R>>();
-        L.rec = rec_.as<BondRec<R>>();
-        L.alive = alive_.as<uint8_t>();
-        L.lambda = lambda_.as<R>();
-        L.pext = pext_.as<R>();
-        L.bnd = bnd_.as<BondNode>();
-        L.alive_n = alive_n_.as<uint8_t>();
-        L.thr = thr_.as<Thresholds<R>>();
-        L.thr2 = thr2_.as<Tensile<R>>();
-        L.pvd = pvd_.as<R4<R>>();
-        L.mode = mode_.as<uint8_t>();
-        L.pstr = pstr_.as<R>();
-        L.slots = slots_.as<uint2>();
-        L.ev_bits = ev_bits_.as<uint64_t>();
-        L.brk_bits = brk_bits_.as<uint64_t>();
-        L.grp_bits = grp_bits_.as<uint64_t>();
-        L.ev_d = ev_d_.as<float>();
-        L.ev_e = ev_e_.as<R>();
-        L.flag_list = flag_list_.as<uint32_t>();
-        L.tile_cand = tile_cand_.as<uint32_t>();
-        L.tile_plastic = tile_plastic_.as<double>();
-        L.load_part = load_part_.as<double>();
-        L.fext = fext_.as<R4<R>>();
-        L.freq = freq_.as<double>();
-        L.cand_all = cand_all_.as<uint32_t>();
-        L.cand_tile = cand_tile_.as<uint32_t>();
-        L.dirty_list = dirty_list_.as<uint32_t>();
-        L.dirty_stamp = dirty_stamp_.as<uint32_t>();
-        L.rank_stamp = rank_stamp_.as<uint32_t>();
-        L.affected = affected_.as<uint32_t>();
-        L.affected_list = affected_list_.as<uint32_t>();
-        L.cand_lo = cand_lo_.as<uint32_t>();
-        L.cand_cnt = cand_cnt_.as<uint32_t>();
-        L.nbb = nbb_.as<uint32_t>();
-        L.first_fail = first_fail_.as<uint32_t>();
-        L.ctl = ctl_.as<DevCtl>();
-        L.sphere = sphere_.as<Sphere<R>>();
-        for (int k = 0; k < 3; ++k) L.origin[k] = origin_[k];
+    Batch<R> batch(const MapRef &r) const {
+        return Batch<R>{ls_.as<DevLattice<R>>(), maps_.as<WgMap>() + r.offset, r.count};
     }
 
-    DevCtl readCtl() {
+    DevCtl readCtl(uint32_t k) {
         DevCtl c;
-        HIP_OK(hipMemcpy(&c, ctl_.p, sizeof(DevCtl), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(&c, ctls_.as<DevCtl>() + k, sizeof(DevCtl), hipMemcpyDeviceToHost));
         return c;
     }
-    void writeCtl(const DevCtl &c) { HIP_OK(hipMemcpy(ctl_.p, &c, sizeof(DevCtl), hipMemcpyHostToDevice)); }
-    void setRemaining(uint64_t n) {
-        DevCtl c = readCtl();
-        c.load_remaining = n;
-        writeCtl(c);
-        ctl_host_.load_remaining = n;
+    void writeCtl(uint32_t k, const DevCtl &c) {
+        HIP_OK(hipMemcpy(ctls_.as<DevCtl>() + k, &c, sizeof(DevCtl), hipMemcpyHostToDevice));
     }
-    Sphere<R> readSphere() {
+    void readAllCtls() {
+        ctl_all_.resize(slots_.size());
+        HIP_OK(hipMemcpy(ctl_all_.data(), ctls_.p, slots_.size() * sizeof(DevCtl), hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < slots_.size(); ++k)
+            if (slots_[k]->uploaded) slots_[k]->ctl_host = ctl_all_[k];
+    }
+    void setRemaining(uint32_t k, uint64_t n) {
+        DevCtl c = readCtl(k);
+        c.load_remaining = n;
+        writeCtl(k, c);
+        slot(k).ctl_host.load_remaining = n;
+    }
+    Sphere<R> readSphere(uint32_t k) {
         Sphere<R> s;
-        HIP_OK(hipMemcpy(&s, sphere_.p, sizeof(s), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(&s, spheres_.as<Sphere<R>>() + k, sizeof(s), hipMemcpyDeviceToHost));
         return s;
     }
-    static void sphereOut(const Sphere<R> &s, blx_sphere &o) {
-        o.center[0] = double(s.center.x); o.center[1] = double(s.center.y); o.center[2] = double(s.center.z);
-        o.velocity[0] = double(s.velocity.x); o.velocity[1] = double(s.velocity.y); o.velocity[2] = double(s.velocity.z);
-        o.angular_velocity[0] = double(s.omega.x); o.angular_velocity[1] = double(s.omega.y); o.angular_velocity[2] = double(s.omega.z);
-        o.radius = double(s.radius);
-        o.mass = double(s.mass);
-        o.inertia = double(s.inertia);
+    template <class T> void copyDown(std::vector<T> &dst, const T *src) {
+        if (!dst.empty()) HIP_OK(hipMemcpy(dst.data(), src, dst.size() * sizeof(T), hipMemcpyDeviceToHost));
     }
 
-    void captureFrame(uint64_t step) {
+    void captureFrame(uint32_t k, uint64_t step) {
+        Slot<R> &sl = slot(k);
         HostFrame<R> f;
         f.step = step;
-        std::vector<R4<R>> u4(N_), pvd(B_);
-        copyDown(u4, u4_);
-        copyDown(pvd, pvd_);
-        f.u.resize(3 * size_t(N_));
-        for (uint32_t i = 0; i < N_; ++i) {
+        std::vector<R4<R>> u4(sl.N), pvd(sl.B);
+        copyDown(u4, sl.L.u4);
+        copyDown(pvd, sl.L.pvd);
+        f.u.resize(3 * size_t(sl.N));
+        for (uint32_t i = 0; i < sl.N; ++i) {
             f.u[3 * size_t(i)] = float(u4[i].x);
             f.u[3 * size_t(i) + 1] = float(u4[i].y);
             f.u[3 * size_t(i) + 2] = float(u4[i].z);
         }
-        f.alive.resize(B_);
-        HIP_OK(hipMemcpy(f.alive.data(), alive_.p, B_, hipMemcpyDeviceToHost));
-        f.damage.resize(B_);
-        for (uint32_t o = 0; o < B_; ++o) f.damage[node_j_host_[o]] = float(pvd[o].w);
-        f.sphere = readSphere();
-        frames_.push_back(std::move(f));
+        f.alive.resize(sl.B);
+        HIP_OK(hipMemcpy(f.alive.data(), sl.L.alive, sl.B, hipMemcpyDeviceToHost));
+        f.damage.resize(sl.B);
+        for (uint32_t o = 0; o < sl.B; ++o) f.damage[sl.node_j_host[o]] = float(pvd[o].w);
+        f.sphere = readSphere(k);
+        sl.frames.push_back(std::move(f));
     }
 
     // Kernel kinds for the optional per-launch event timing, each with the
@@ -948,24 +1290,13 @@ private:
             HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
             kernel_seconds_[r.kernel] += double(ms) * 1e-3;
             kernel_launches_[r.kernel] += 1;
-            run_status_.phase_seconds[r.phase] += double(ms) * 1e-3;
+            for (auto &s : slots_)
+                if (s->armed) s->status.phase_seconds[r.phase] += double(ms) * 1e-3;
         }
         records_.clear();
     }
-
-public:
-    void setProfiling(bool on) override { prof_ = on; }
-    uint32_t kernelStats(uint32_t k, std::string &name, double &seconds, uint64_t &launches) const override {
-        if (k < KCount) {
-            name = kernelName(k);
-            seconds = kernel_seconds_[k];
-            launches = kernel_launches_[k];
-        }
-        return KCount;
-    }
     uint64_t profilingBatch(uint64_t batch) const { return prof_ ? std::min<uint64_t>(batch, 4) : batch; }
 
-private:
 #define BLX_LAUNCH(kernel, phase, ...) \
     do {                                \
         profBegin(kernel, phase);       \
@@ -973,119 +1304,158 @@ private:
         profEnd();                      \
     } while (0)
 
+    static dim3 tiles(const MapRef &r) { return dim3(std::max(1U, xcdGrid(r.count))); }
+    static dim3 entries(const MapRef &r) { return dim3(std::max(1U, r.count)); }
+
     template <bool DIRECT> void enqueueT() {
-        const DevLattice<R> L = L_;
         const Step<R> S = S_;
-        const uint32_t gn = gridFor(N_);
         const bool sphere = S.sphere_enabled != 0;
+        const bool plastic = shared_.plastic;
         // With a striker, the end sample of what pass 2 cannot reach overlaps pass 2.
-        const bool split = sphere && !plastic_;
-        const uint32_t gnt = xcdGrid(node_tiles_), gbt = xcdGrid(bond_tiles_);
-        BLX_LAUNCH(KStrainStart, 1, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainDirty>), dim3(gn), dim3(kWG), 0, stream_, L));
-        BLX_LAUNCH(KBondStart, 2, hipLaunchKernelGGL((k_bond_start<R, DIRECT>), dim3(gn), dim3(kWG), 0, stream_, L));
-        const dim3 pg(gnt);
+        const bool split = sphere && !plastic;
+        const Batch<R> nodes = batch(node_tiles_map_), bonds = batch(bond_tiles_map_), list = batch(list_map_),
+                       small = batch(small_map_), lats = batch(lattice_map_);
+        const dim3 ng = tiles(node_tiles_map_), bg = tiles(bond_tiles_map_), lg = entries(list_map_),
+                   sg = entries(small_map_), fg = entries(lattice_map_);
+        BLX_LAUNCH(KStrainStart, 1, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainDirty>), lg, dim3(kWG), 0, stream_, list));
+        BLX_LAUNCH(KBondStart, 2, hipLaunchKernelGGL((k_bond_start<R, DIRECT>), lg, dim3(kWG), 0, stream_, list));
         profBegin(KPredict, 3);
-        if (sphere && loads_set_) hipLaunchKernelGGL((k_predict<R, true, true, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
-        else if (sphere) hipLaunchKernelGGL((k_predict<R, true, false, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
-        else if (loads_set_) hipLaunchKernelGGL((k_predict<R, false, true, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
-        else hipLaunchKernelGGL((k_predict<R, false, false, DIRECT>), pg, dim3(kWG), 0, stream_, L, S);
+        if (sphere && any_loads_) hipLaunchKernelGGL((k_predict<R, true, true, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
+        else if (sphere) hipLaunchKernelGGL((k_predict<R, true, false, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
+        else if (any_loads_) hipLaunchKernelGGL((k_predict<R, false, true, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
+        else hipLaunchKernelGGL((k_predict<R, false, false, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
         profEnd();
-        if (sphere) BLX_LAUNCH(KPass1, 4, hipLaunchKernelGGL((k_pass<R, 1>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
+        if (sphere) BLX_LAUNCH(KPass1, 4, hipLaunchKernelGGL((k_pass<R, 1>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
         if (split) {
             HIP_OK(hipEventRecord(ev_pass1_, stream_));
             HIP_OK(hipStreamWaitEvent(early_, ev_pass1_, 0));
-            const uint32_t marks = std::min<uint32_t>(cus_ * 4U, (sch_.blocks * kCap * (D_ + 1) + kWG - 1) / kWG);
             profBegin(KMark, 4, early_);
-            hipLaunchKernelGGL((k_mark_affected<R>), dim3(std::max(1U, marks)), dim3(kWG), 0, early_, L);
+            hipLaunchKernelGGL((k_mark_affected<R>), entries(mark_map_), dim3(kWG), 0, early_, batch(mark_map_));
             profEnd(early_);
         }
-        const uint2 *wg = wg_.as<uint2>();
-        for (uint32_t it = 0; it < iterations_; ++it) {
-            const uint32_t flags = (it == 0 ? kSolveFirst : 0U) | (it + 1 < iterations_ ? kSolveKeepLambda : 0U);
+        const uint32_t iterations = shared_.iterations;
+        for (uint32_t it = 0; it < iterations; ++it) {
+            const uint32_t flags = (it == 0 ? kSolveFirst : 0U) | (it + 1 < iterations ? kSolveKeepLambda : 0U);
             // One timing bracket over the back-to-back stage launches of the
             // interior phase and one over the boundary phases.
             for (int phase = 0; phase < 2; ++phase) {
-                const size_t lo = phase == 0 ? 0 : interior_stages_, hi = phase == 0 ? interior_stages_ : stages_.size();
-                if (lo == hi) continue;
+                const size_t lo = phase == 0 ? 0 : interior_stages_;
+                const size_t hi = phase == 0 ? std::min(interior_stages_, stage_maps_.size()) : stage_maps_.size();
+                if (lo >= hi) continue;
                 profBegin(KSolve, phase == 0 ? 5 : 6);
                 for (size_t k = lo; k < hi; ++k) {
-                    const auto &st = stages_[k];
-                    if (plastic_)
-                        hipLaunchKernelGGL((k_solve<R, DIRECT, true>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L,
-                                           S, wg, st.first, st.second, flags);
+                    const MapRef &m = stage_maps_[k];
+                    if (plastic)
+                        hipLaunchKernelGGL((k_solve<R, DIRECT, true>), tiles(m), dim3(kWG), 0, stream_, batch(m), S, flags);
                     else
-                        hipLaunchKernelGGL((k_solve<R, DIRECT, false>), dim3(xcdGrid(st.second)), dim3(kWG), 0, stream_, L,
-                                           S, wg, st.first, st.second, flags);
+                        hipLaunchKernelGGL((k_solve<R, DIRECT, false>), tiles(m), dim3(kWG), 0, stream_, batch(m), S, flags);
                 }
                 profEnd();
                 profAddLaunches(KSolve, uint32_t(hi - lo) - 1U);
             }
-            if (S.plane_count > 0 && it + 1 < iterations_)
-                BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), dim3(gn), dim3(kWG), 0, stream_, L, S));
+            if (S.plane_count > 0 && it + 1 < iterations)
+                BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
         profBegin(KVelocity, 8);
-        if (S.plane_count > 0) hipLaunchKernelGGL((k_velocity<R, true>), dim3(gnt), dim3(kWG), 0, stream_, L, S);
-        else hipLaunchKernelGGL((k_velocity<R, false>), dim3(gnt), dim3(kWG), 0, stream_, L, S);
+        if (S.plane_count > 0) hipLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nodes, S);
+        else hipLaunchKernelGGL((k_velocity<R, false>), ng, dim3(kWG), 0, stream_, nodes, S);
         profEnd();
         if (S.damping > R(0)) {
-            for (const auto &st : stages_)
-                BLX_LAUNCH(KDamp, 9, hipLaunchKernelGGL((k_damp<R, DIRECT>), dim3(st.second), dim3(kWG), 0, stream_, L, S, wg, st.first));
-            BLX_LAUNCH(KSupportVelocity, 9, hipLaunchKernelGGL((k_support_velocity<R>), dim3(gn), dim3(kWG), 0, stream_, L, S));
+            for (const MapRef &m : stage_maps_)
+                BLX_LAUNCH(KDamp, 9, hipLaunchKernelGGL((k_damp<R, DIRECT>), tiles(m), dim3(kWG), 0, stream_, batch(m), S));
+            BLX_LAUNCH(KSupportVelocity, 9, hipLaunchKernelGGL((k_support_velocity<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
         if (split) {
             HIP_OK(hipEventRecord(ev_velocity_, stream_));
             HIP_OK(hipStreamWaitEvent(early_, ev_velocity_, 0));
             profBegin(KStrain, 11, early_);
-            hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), dim3(gnt), dim3(kWG), 0, early_, L);
+            hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), ng, dim3(kWG), 0, early_, nodes);
             profEnd(early_);
             profBegin(KBondEnd, 12, early_);
-            hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), dim3(gbt), dim3(kWG), 0, early_, L, S);
+            hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), 0, early_, bonds, S);
             profEnd(early_);
             HIP_OK(hipEventRecord(ev_early_, early_));
-            BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
-            const uint32_t late = std::min<uint32_t>(cus_ * 4U, gridFor(N_));
-            BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), dim3(late), dim3(kWG), 0, stream_, L));
+            BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
+            BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), sg, dim3(kWG), 0, stream_, small));
             HIP_OK(hipStreamWaitEvent(stream_, ev_early_, 0));
-            BLX_LAUNCH(KBondEndLate, 12, hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), dim3(late), dim3(kWG), 0, stream_, L, S));
+            BLX_LAUNCH(KBondEndLate, 12,
+                       hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), 0, stream_, small, S));
         } else {
-            if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S));
-            BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAll>), dim3(gnt), dim3(kWG), 0, stream_, L));
+            if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
+            BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAll>), ng, dim3(kWG), 0, stream_, nodes));
             profBegin(KBondEnd, 12);
-            if (plastic_) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true, kBondAll>), dim3(gbt), dim3(kWG), 0, stream_, L, S);
-            else hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAll>), dim3(gbt), dim3(kWG), 0, stream_, L, S);
+            if (plastic) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true, kBondAll>), bg, dim3(kWG), 0, stream_, bonds, S);
+            else hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAll>), bg, dim3(kWG), 0, stream_, bonds, S);
             profEnd();
         }
         // The dirty nodes' inverse rest covariances overlap the ordered folds.
         HIP_OK(hipEventRecord(ev_bond_end_, stream_));
         HIP_OK(hipStreamWaitEvent(side_, ev_bond_end_, 0));
         profBegin(KRinv, 1, side_);
-        hipLaunchKernelGGL((k_rinv<R>), dim3(gridFor(N_)), dim3(kWG), 0, side_, L);
+        hipLaunchKernelGGL((k_rinv<R>), lg, dim3(kWG), 0, side_, list);
         profEnd(side_);
         HIP_OK(hipEventRecord(ev_rinv_, side_));
         profBegin(KFinish, 13);
-        if (loads_set_ && plastic_) hipLaunchKernelGGL((k_finish<R, true, true>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
-        else if (loads_set_) hipLaunchKernelGGL((k_finish<R, true, false>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
-        else if (plastic_) hipLaunchKernelGGL((k_finish<R, false, true>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
-        else hipLaunchKernelGGL((k_finish<R, false, false>), dim3(1), dim3(kFinishThreads), 0, stream_, L, S);
+        if (any_loads_ && plastic) hipLaunchKernelGGL((k_finish<R, true, true>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
+        else if (any_loads_) hipLaunchKernelGGL((k_finish<R, true, false>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
+        else if (plastic) hipLaunchKernelGGL((k_finish<R, false, true>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
+        else hipLaunchKernelGGL((k_finish<R, false, false>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
         profEnd();
         HIP_OK(hipStreamWaitEvent(stream_, ev_rinv_, 0));
         HIP_OK(hipGetLastError());
     }
 
     void enqueueSubstep() {
-        if (direct_) enqueueT<true>();
+        if (shared_.direct) enqueueT<true>();
         else enqueueT<false>();
     }
 };
 
-struct Handle {
-    std::unique_ptr<BackendBase> impl;
+// Opaque handles of the C ABI.
+struct EngineBase {
+    virtual ~EngineBase() = default;
 };
+template <class R> struct EngineHolder final : EngineBase {
+    Engine<R> e;
+    EngineHolder(int device, uint32_t capacity) : e(device, capacity) {}
+};
+struct Handle {
+    int precision = BLX_PRECISION_FLOAT;
+    std::unique_ptr<EngineBase> impl;
+};
+
+template <class F> auto dispatch(Handle *h, F &&f) {
+    if (!h) throw Invalid("null handle");
+    if (h->precision == BLX_PRECISION_FLOAT) return f(static_cast<EngineHolder<float> *>(h->impl.get())->e);
+    return f(static_cast<EngineHolder<double> *>(h->impl.get())->e);
+}
+template <class F> auto dispatch(const Handle *h, F &&f) { return dispatch(const_cast<Handle *>(h), std::forward<F>(f)); }
+
+Handle *makeHandle(int device, int precision, uint32_t capacity) {
+    if (deviceCount() <= device || device < 0)
+        throw Runtime("no HIP device " + std::to_string(device) + " for the lattice backend");
+    auto h = std::make_unique<Handle>();
+    h->precision = precision;
+    if (precision == BLX_PRECISION_FLOAT) h->impl = std::make_unique<EngineHolder<float>>(device, capacity);
+    else if (precision == BLX_PRECISION_DOUBLE) h->impl = std::make_unique<EngineHolder<double>>(device, capacity);
+    else throw Invalid("unknown precision");
+    return h.release();
+}
+
+void copyName(const std::string &s, char *out, size_t cap) {
+    if (out && cap) {
+        std::strncpy(out, s.c_str(), cap - 1);
+        out[cap - 1] = 0;
+    }
+}
 
 } // namespace
 } // namespace blx
 
 using namespace blx;
+
+#define H(b) reinterpret_cast<blx::Handle *>(b)
+#define CH(b) reinterpret_cast<const blx::Handle *>(b)
 
 extern "C" {
 
@@ -1113,10 +1483,7 @@ int blx_description(char *out, size_t cap) {
                           p.gcnArchName, p.multiProcessorCount, double(p.totalGlobalMem) / (1024.0 * 1024.0 * 1024.0));
             s += line;
         }
-        if (out && cap) {
-            std::strncpy(out, s.c_str(), cap - 1);
-            out[cap - 1] = 0;
-        }
+        copyName(s, out, cap);
     });
 }
 
@@ -1127,75 +1494,133 @@ uint32_t blx_multiprocessor_count(void) {
     return uint32_t(p.multiProcessorCount);
 }
 
+// ---- single-lattice backend: an engine with one slot ----------------------
+
 int blx_backend_create(int device, int precision, const blx_schedule_view *sv, blx_backend **out) {
     return guard([&] {
         if (!sv || !out) throw Invalid("null argument");
-        if (deviceCount() <= device || device < 0)
-            throw Runtime("no HIP device " + std::to_string(device) + " for the lattice backend");
-        blx::Schedule s;
-        s.N = sv->node_count;
-        s.B = sv->bond_count;
-        s.blocks = sv->block_count;
-        s.colors = sv->color_count;
-        s.nbb.assign(sv->node_block_begin, sv->node_block_begin + sv->block_count + 1);
-        const size_t ranges = size_t(sv->block_count) * 2 * std::max(sv->color_count, 1U);
-        s.range_begin.assign(sv->range_begin, sv->range_begin + ranges);
-        s.range_end.assign(sv->range_end, sv->range_end + ranges);
-        auto h = std::make_unique<Handle>();
-        if (precision == BLX_PRECISION_FLOAT) h->impl = std::make_unique<Backend<float>>(device, std::move(s));
-        else if (precision == BLX_PRECISION_DOUBLE) h->impl = std::make_unique<Backend<double>>(device, std::move(s));
-        else throw Invalid("unknown precision");
+        std::unique_ptr<Handle> h(makeHandle(device, precision, 1));
+        dispatch(h.get(), [&](auto &e) { e.setSchedule(0, scheduleOf(*sv)); });
         *out = reinterpret_cast<blx_backend *>(h.release());
     });
 }
-
-#define IMPL(b) (reinterpret_cast<Handle *>(b)->impl)
-
-int blx_backend_destroy(blx_backend *b) { return guard([&] { delete reinterpret_cast<Handle *>(b); }); }
+int blx_backend_destroy(blx_backend *b) { return guard([&] { delete H(b); }); }
 int blx_backend_name(const blx_backend *b, char *out, size_t cap) {
-    return guard([&] {
-        const std::string s = reinterpret_cast<const Handle *>(b)->impl->name();
-        if (out && cap) {
-            std::strncpy(out, s.c_str(), cap - 1);
-            out[cap - 1] = 0;
-        }
-    });
+    return guard([&] { copyName(dispatch(CH(b), [&](auto &e) { return e.name(); }), out, cap); });
 }
 int blx_backend_upload(blx_backend *b, const blx_state_view *st, const blx_settings *s, const blx_sphere *sp) {
     return guard([&] {
         if (!b || !st || !s || !sp) throw Invalid("null argument");
-        IMPL(b)->upload(*st, *s, *sp);
+        dispatch(H(b), [&](auto &e) { e.upload(0, *st, *s, *sp); });
     });
 }
 int blx_backend_set_external_forces(blx_backend *b, const double *f, uint64_t n) {
-    return guard([&] { IMPL(b)->setForces(f, n); });
+    return guard([&] { dispatch(H(b), [&](auto &e) { e.setForces(0, f, n); }); });
 }
 int blx_backend_run(blx_backend *b, const blx_run_control *c, blx_run_status *st) {
     return guard([&] {
         if (!b || !c || !st) throw Invalid("null argument");
-        IMPL(b)->run(*c, *st);
+        dispatch(H(b), [&](auto &e) { e.run(*c, *st); });
     });
 }
 int blx_backend_status(const blx_backend *b, blx_run_status *st) {
-    return guard([&] { reinterpret_cast<const Handle *>(b)->impl->status(*st); });
+    return guard([&] { dispatch(CH(b), [&](auto &e) { e.status(0, *st); }); });
 }
 int blx_backend_download(blx_backend *b, blx_state_view *st, blx_sphere *sp) {
-    return guard([&] { IMPL(b)->download(*st, *sp); });
+    return guard([&] { dispatch(H(b), [&](auto &e) { e.download(0, *st, *sp); }); });
 }
 int blx_backend_first_failure_bonds(const blx_backend *b, uint32_t *out, uint64_t cap, uint64_t *count) {
-    return guard([&] { *count = reinterpret_cast<const Handle *>(b)->impl->firstFailure(out, cap); });
+    return guard([&] { *count = dispatch(CH(b), [&](auto &e) { return e.firstFailure(0, out, cap); }); });
 }
 int blx_backend_frame_count(const blx_backend *b, uint32_t *n) {
-    return guard([&] { *n = reinterpret_cast<const Handle *>(b)->impl->frameCount(); });
+    return guard([&] { *n = dispatch(CH(b), [&](auto &e) { return e.frameCount(0); }); });
 }
 int blx_backend_take_frame(blx_backend *b, uint32_t k, uint64_t *step, float *u, uint8_t *alive, float *damage,
                            blx_sphere *sp) {
-    return guard([&] { IMPL(b)->takeFrame(k, step, u, alive, damage, sp); });
+    return guard([&] { dispatch(H(b), [&](auto &e) { e.takeFrame(0, k, step, u, alive, damage, sp); }); });
 }
-int blx_backend_clear_frames(blx_backend *b) { return guard([&] { IMPL(b)->clearFrames(); }); }
+int blx_backend_clear_frames(blx_backend *b) {
+    return guard([&] { dispatch(H(b), [&](auto &e) { e.clearFrames(0); }); });
+}
 int blx_backend_fragment_labels(blx_backend *b, uint32_t *label, uint32_t *count) {
-    return guard([&] { *count = IMPL(b)->labels(label); });
+    return guard([&] { *count = dispatch(H(b), [&](auto &e) { return e.labels(0, label); }); });
 }
+int blx_backend_set_profiling(blx_backend *b, int on) {
+    return guard([&] { dispatch(H(b), [&](auto &e) { e.setProfiling(on != 0); }); });
+}
+int blx_backend_kernel_stats(const blx_backend *b, uint32_t index, char *name, size_t cap, double *seconds,
+                             uint64_t *launches, uint32_t *count) {
+    return guard([&] {
+        std::string n;
+        double s = 0.0;
+        uint64_t l = 0;
+        *count = dispatch(CH(b), [&](auto &e) { return e.kernelStats(index, n, s, l); });
+        if (index < *count) {
+            copyName(n, name, cap);
+            if (seconds) *seconds = s;
+            if (launches) *launches = l;
+        }
+    });
+}
+
+// ---- batch of lattices ------------------------------------------------------
+
+int blx_batch_create(int device, int precision, uint32_t capacity, blx_batch **out) {
+    return guard([&] {
+        if (!out) throw Invalid("null argument");
+        *out = reinterpret_cast<blx_batch *>(makeHandle(device, precision, capacity));
+    });
+}
+int blx_batch_destroy(blx_batch *b) { return guard([&] { delete H(b); }); }
+int blx_batch_capacity(const blx_batch *b, uint32_t *capacity) {
+    return guard([&] { *capacity = dispatch(CH(b), [&](auto &e) { return e.capacity(); }); });
+}
+int blx_batch_upload(blx_batch *b, uint32_t slot, const blx_schedule_view *sv, const blx_state_view *st,
+                     const blx_settings *s, const blx_sphere *sp) {
+    return guard([&] {
+        if (!b || !sv || !st || !s || !sp) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) {
+            e.setSchedule(slot, scheduleOf(*sv));
+            e.upload(slot, *st, *s, *sp);
+        });
+    });
+}
+int blx_batch_set_external_forces(blx_batch *b, uint32_t slot, const double *f, uint64_t n) {
+    return guard([&] { dispatch(H(b), [&](auto &e) { e.setForces(slot, f, n); }); });
+}
+int blx_batch_arm(blx_batch *b, uint32_t slot, const blx_run_control *c) {
+    return guard([&] {
+        if (!c) throw Invalid("null argument");
+        if (c->capture_stride != 0) throw Invalid("frames are captured by the single-lattice backend only");
+        dispatch(H(b), [&](auto &e) { e.arm(slot, *c); });
+    });
+}
+int blx_batch_advance(blx_batch *b, uint64_t substeps, uint64_t steps_per_launch, const volatile uint8_t *abandon,
+                      uint32_t *running) {
+    return guard([&] {
+        const uint32_t r = dispatch(H(b), [&](auto &e) {
+            return e.advance(substeps, steps_per_launch ? steps_per_launch : 240, abandon, nullptr);
+        });
+        if (running) *running = r;
+    });
+}
+int blx_batch_status(const blx_batch *b, uint32_t slot, blx_run_status *st) {
+    return guard([&] { dispatch(CH(b), [&](auto &e) { e.status(slot, *st); }); });
+}
+int blx_batch_download(blx_batch *b, uint32_t slot, blx_state_view *st, blx_sphere *sp) {
+    return guard([&] { dispatch(H(b), [&](auto &e) { e.download(slot, *st, *sp); }); });
+}
+int blx_batch_fragment_labels(blx_batch *b, uint32_t slot, uint32_t *label, uint32_t *count) {
+    return guard([&] { *count = dispatch(H(b), [&](auto &e) { return e.labels(slot, label); }); });
+}
+int blx_batch_release(blx_batch *b, uint32_t slot) {
+    return guard([&] { dispatch(H(b), [&](auto &e) { e.release(slot); }); });
+}
+int blx_batch_slot_bytes(const blx_batch *b, uint32_t slot, uint64_t *bytes) {
+    return guard([&] { *bytes = dispatch(CH(b), [&](auto &e) { return uint64_t(e.slotBytes(slot)); }); });
+}
+
+// ---- measurement ------------------------------------------------------------
 
 int blx_hbm_copy_probe(int device, uint64_t bytes, uint32_t repeats, double *seconds_per_copy) {
     return guard([&] {
@@ -1223,25 +1648,6 @@ int blx_hbm_copy_probe(int device, uint64_t bytes, uint32_t repeats, double *sec
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         *seconds_per_copy = double(ms) * 1e-3 / repeats;
-    });
-}
-
-int blx_backend_set_profiling(blx_backend *b, int on) { return guard([&] { IMPL(b)->setProfiling(on != 0); }); }
-int blx_backend_kernel_stats(const blx_backend *b, uint32_t index, char *name, size_t cap, double *seconds,
-                             uint64_t *launches, uint32_t *count) {
-    return guard([&] {
-        std::string n;
-        double s = 0.0;
-        uint64_t l = 0;
-        *count = reinterpret_cast<const Handle *>(b)->impl->kernelStats(index, n, s, l);
-        if (index < *count) {
-            if (name && cap) {
-                std::strncpy(name, n.c_str(), cap - 1);
-                name[cap - 1] = 0;
-            }
-            if (seconds) *seconds = s;
-            if (launches) *launches = l;
-        }
     });
 }
 

@@ -70,6 +70,7 @@ struct DevCtl {
     u64 first_failure_count;
     u32 affected_count;  // entries in affected_list
     u32 pad0;
+    u64 budget;          // substeps left in the current run; the run stops at 0
     // RunControl of the current run() call.
     u64 quiet, min_steps, no_failure, energy_flat, calm;
     double flat_fraction, calm_margin, removable;
@@ -147,6 +148,25 @@ template <class R> struct DevLattice {
     Sphere<R> *sphere;
     double origin[3];
 };
+
+// A launch over a batch of lattices. Every workgroup finds its lattice and
+// its share of that lattice's work in the launch's map, so lattices of any
+// sizes advance together in one launch per phase; a single lattice is a batch
+// of one. The meaning of a, b depends on the kernel: a tile index, a slice of
+// a list (slice, slice count), or a bond range (begin, end).
+struct WgMap {
+    u32 lat, a, b, pad;
+};
+template <class R> struct Batch {
+    const DevLattice<R> *L;
+    const WgMap *map;
+    u32 count;  // map entries; a launch has xcdGrid(count) workgroups (tile maps) or count
+};
+// The map entry of this workgroup for tile launches (XCD-aware), or nullptr.
+template <class R> __device__ __forceinline__ const WgMap *tileEntry(const Batch<R> &B) {
+    const u32 w = xcdTile(blockIdx.x, B.count);
+    return w < B.count ? &B.map[w] : nullptr;
+}
 
 // ---------------------------------------------------------------------------
 // Nodes.
@@ -254,19 +274,23 @@ enum StrainCover : int {
 // only its candidates, so the strain of a node with no candidate among itself
 // and its neighbours can be taken while pass 2 runs.
 template <class R, bool DIRECT, int COVER>
-__global__ __launch_bounds__(kWG) void k_strain(DevLattice<R> L) {
-    if (L.ctl->stop) return;
+__global__ __launch_bounds__(kWG) void k_strain(Batch<R> B) {
     if (COVER == kStrainDirty || COVER == kStrainAffected) {
+        const WgMap m = B.map[blockIdx.x];
+        const DevLattice<R> L = B.L[m.lat];
+        if (L.ctl->stop) return;
         if (COVER == kStrainDirty && !L.ctl->dirty_start) return;
         const u32 count = COVER == kStrainDirty ? L.ctl->dirty_count : L.ctl->affected_count;
         const u32 *list = COVER == kStrainDirty ? L.dirty_list : L.affected_list;
-        for (u32 k = blockIdx.x * kWG + threadIdx.x; k < count; k += gridDim.x * kWG)
-            strainNode<R, DIRECT>(L, list[k]);
+        for (u32 k = m.a * kWG + threadIdx.x; k < count; k += m.b * kWG) strainNode<R, DIRECT>(L, list[k]);
         return;
     }
-    const u32 tile = xcdTile(blockIdx.x, L.node_tiles);
-    const u32 i = tile * kWG + threadIdx.x;
-    if (tile >= L.node_tiles || i >= L.N) return;
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
+    if (L.ctl->stop) return;
+    const u32 i = m->a * kWG + threadIdx.x;
+    if (i >= L.N) return;
     if (COVER == kStrainUnaffected && L.affected[i] == u32(L.ctl->step) + 1U) return;
     strainNode<R, DIRECT>(L, i);
 }
@@ -274,13 +298,15 @@ __global__ __launch_bounds__(kWG) void k_strain(DevLattice<R> L) {
 // The nodes striker pass 2 can reach this substep: its candidates (the
 // staged lists of pass 1) and their neighbours, each listed once. Runs on a
 // side stream while the colour sweep runs.
-template <class R> __global__ __launch_bounds__(kWG) void k_mark_affected(DevLattice<R> L) {
+template <class R> __global__ __launch_bounds__(kWG) void k_mark_affected(Batch<R> B) {
+    const WgMap m = B.map[blockIdx.x];
+    const DevLattice<R> L = B.L[m.lat];
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
     const u32 stamp = u32(ctl->step) + 1U;
     const u32 per = L.D + 1;
     const u64 total = u64(L.blocks) * kCap * per;
-    for (u64 q = u64(blockIdx.x) * kWG + threadIdx.x; q < total; q += u64(gridDim.x) * kWG) {
+    for (u64 q = u64(m.a) * kWG + threadIdx.x; q < total; q += u64(m.b) * kWG) {
         const u32 b = u32(q / (u64(kCap) * per));
         const u32 rem = u32(q % (u64(kCap) * per));
         const u32 e = rem / per, k = rem % per;
@@ -303,12 +329,13 @@ template <class R> __global__ __launch_bounds__(kWG) void k_mark_affected(DevLat
 // to a per-parity slot that the next substep's k_finish books.
 // It ignores the stop flag, which k_finish may raise while it runs: the
 // recomputation is idempotent and the count is stamped per node.
-template <class R> __global__ __launch_bounds__(kWG) void k_rinv(DevLattice<R> L) {
+template <class R> __global__ __launch_bounds__(kWG) void k_rinv(Batch<R> B) {
+    const WgMap m = B.map[blockIdx.x];
+    const DevLattice<R> L = B.L[m.lat];
     const u32 count = L.ctl->dirty_count;
     u32 *counter = &L.ctl->rank_pending[L.ctl->rank_slot];
     const u32 stamp = L.ctl->rinv_stamp;
-    for (u32 k = blockIdx.x * kWG + threadIdx.x; k < count; k += gridDim.x * kWG)
-        rinvNode<R>(L, L.dirty_list[k], counter, stamp);
+    for (u32 k = m.a * kWG + threadIdx.x; k < count; k += m.b * kWG) rinvNode<R>(L, L.dirty_list[k], counter, stamp);
 }
 
 // ---------------------------------------------------------------------------
@@ -376,10 +403,12 @@ template <class R> __global__ void k_edge_palette(EdgePal<R> *pal, u32 *no_dir, 
 // A bond with no dirty end samples exactly what its end sample stored, so
 // only the bonds in dirty nodes' slots are resampled; a bond with both ends
 // dirty is written twice with the same bits.
-template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_start(DevLattice<R> L) {
+template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_start(Batch<R> B) {
+    const WgMap m = B.map[blockIdx.x];
+    const DevLattice<R> L = B.L[m.lat];
     if (L.ctl->stop || !L.ctl->dirty_start) return;
     const u64 pairs = u64(L.ctl->dirty_count) * L.D;
-    for (u64 q = u64(blockIdx.x) * kWG + threadIdx.x; q < pairs; q += u64(gridDim.x) * kWG) {
+    for (u64 q = u64(m.a) * kWG + threadIdx.x; q < pairs; q += u64(m.b) * kWG) {
         const u32 i = L.dirty_list[q / L.D], k = u32(q % L.D);
         const u32 o = L.nbr_bond[size_t(k) * L.N + i];
         if (o == kNone || !L.alive_n[o]) continue;
@@ -410,17 +439,19 @@ template <class R> __device__ void blockSum13(double (*sh)[kWG], double v[13]) {
 }
 
 template <class R, bool SPHERE, bool LOADS, bool DIRECT>
-__global__ __launch_bounds__(kWG) void k_predict(DevLattice<R> L, Step<R> S) {
+__global__ __launch_bounds__(kWG) void k_predict(Batch<R> B, Step<R> S) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const u32 tile = m->a;
+    if (tile == 0 && threadIdx.x == 0) {
         ctl->dirty_start = 0;
         ctl->dirty_count = 0;
         ctl->affected_count = 0;
     }
     const bool loads = LOADS && ctl->load_remaining > 0;
-    const u32 tile = xcdTile(blockIdx.x, L.node_tiles);
-    if (tile >= L.node_tiles) return;
     const u32 i = tile * kWG + threadIdx.x;
     bool cand = false;
     double led[13];
@@ -650,7 +681,8 @@ constexpr u32 kMaxNodeTiles = 16384;  // candidate tile prefix held in LDS: up t
 // over the window. A centre move (pass 2) re-speculates the window from the
 // next candidate.
 template <class R, int PASS>
-__global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R> S) {
+__global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) {
+    const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
     // Pass 1's tile prefix is dead once the candidate lists are staged, so the
@@ -844,14 +876,13 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(DevLattice<R> L, Step<R
 // stage's bonds share no node, so every thread owns its two ends.
 
 template <class R, bool DIRECT, bool PLASTIC>
-__global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const uint2 *wg_range, u32 wg_base,
-                                                u32 wg_count, u32 flags) {
+__global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
     if (L.ctl->stop) return;
-    const u32 w = xcdTile(blockIdx.x, wg_count);
-    if (w >= wg_count) return;
-    const uint2 rg = wg_range[wg_base + w];
-    const u32 j = rg.x + threadIdx.x;
-    if (j >= rg.y) return;
+    const u32 j = m->a + threadIdx.x;
+    if (j >= m->b) return;
     const bool first = flags & kSolveFirst;
     // The record is loaded with the alive flag rather than after it, so the
     // node gathers wait on one memory round trip, not two.
@@ -889,11 +920,13 @@ __global__ __launch_bounds__(kWG) void k_solve(DevLattice<R> L, Step<R> S, const
 
 // Radial pair damping, one colour stage (LatticePhysics.hpp:1171-1184).
 template <class R, bool DIRECT>
-__global__ __launch_bounds__(kWG) void k_damp(DevLattice<R> L, Step<R> S, const uint2 *wg_range, u32 wg_base) {
+__global__ __launch_bounds__(kWG) void k_damp(Batch<R> B, Step<R> S) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
     if (L.ctl->stop) return;
-    const uint2 rg = wg_range[wg_base + blockIdx.x];
-    const u32 j = rg.x + threadIdx.x;
-    if (j >= rg.y) return;
+    const u32 j = m->a + threadIdx.x;
+    if (j >= m->b) return;
     if (!L.alive[j]) return;
     const BondRec<R> r = L.rec[j];
     const R ma = L.x04[r.a].w, mb = L.x04[r.b].w;
@@ -927,25 +960,30 @@ template <class R> __device__ __forceinline__ void projectNode(const DevLattice<
     }
 }
 
-template <class R> __global__ __launch_bounds__(kWG) void k_project(DevLattice<R> L, Step<R> S) {
+template <class R> __global__ __launch_bounds__(kWG) void k_project(Batch<R> B, Step<R> S) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
     if (L.ctl->stop) return;
-    for (u32 i = blockIdx.x * kWG + threadIdx.x; i < L.N; i += gridDim.x * kWG) {
-        R4<R> u = L.u4[i];
-        bool moved = false;
-        projectNode(L, S, i, u, moved);
-        if (moved) L.u4[i] = u;
-    }
+    const u32 i = m->a * kWG + threadIdx.x;
+    if (i >= L.N) return;
+    R4<R> u = L.u4[i];
+    bool moved = false;
+    projectNode(L, S, i, u, moved);
+    if (moved) L.u4[i] = u;
 }
 
 // Last iteration's support projection fused with nodeVelocityUpdate
 // (LatticePhysics.hpp:1726-1730): both only touch node i.
 template <class R, bool PROJECT>
-__global__ __launch_bounds__(kWG) void k_velocity(DevLattice<R> L, Step<R> S) {
+__global__ __launch_bounds__(kWG) void k_velocity(Batch<R> B, Step<R> S) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
     if (L.ctl->stop) return;
-    const u32 tile = xcdTile(blockIdx.x, L.node_tiles);
     {
-        const u32 i = tile * kWG + threadIdx.x;
-        if (tile >= L.node_tiles || i >= L.N) return;
+        const u32 i = m->a * kWG + threadIdx.x;
+        if (i >= L.N) return;
         R4<R> u = L.u4[i];
         if (PROJECT) {
             bool moved = false;
@@ -961,10 +999,14 @@ __global__ __launch_bounds__(kWG) void k_velocity(DevLattice<R> L, Step<R> S) {
 }
 
 // Support velocity of non-candidates after the damping sweep (CpuLatticeBackend.cpp:414-415).
-template <class R> __global__ __launch_bounds__(kWG) void k_support_velocity(DevLattice<R> L, Step<R> S) {
+template <class R> __global__ __launch_bounds__(kWG) void k_support_velocity(Batch<R> B, Step<R> S) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
     if (L.ctl->stop) return;
-    for (u32 i = blockIdx.x * kWG + threadIdx.x; i < L.N; i += gridDim.x * kWG) {
-        if (L.cand[i]) continue;
+    const u32 i = m->a * kWG + threadIdx.x;
+    if (i >= L.N || L.cand[i]) return;
+    {
         V3<R> vel = xyz(L.v4[i]);
         bool any = false;
         for (u32 p = 0; p < S.plane_count; ++p)
@@ -1135,15 +1177,18 @@ __device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, f
 }
 
 template <class R, bool DIRECT, bool PLASTIC, int COVER>
-__global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
+__global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
     static_assert(!PLASTIC || COVER == kBondAll, "plastic work is summed per end-sample tile in a fixed order");
+    const WgMap *m = COVER == kBondAffected ? &B.map[blockIdx.x] : tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
     __shared__ float s_max[4][kWG / 64];
     __shared__ double s_pw[kWG];
     const double m0 = ctl->max_damage;
     const u32 stamp = u32(ctl->step) + 1U;
-    if (COVER != kBondAffected && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (COVER != kBondAffected && m->a == 0 && threadIdx.x == 0) {
         ctl->rank_slot = u32(ctl->step & 1U);
         ctl->rinv_stamp = stamp;
     }
@@ -1152,7 +1197,7 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
         // Bonds with an affected end, each taken once: from its first end when
         // that end is affected, else from its second.
         const u64 pairs = u64(ctl->affected_count) * L.D;
-        for (u64 base = u64(blockIdx.x) * kWG; base < pairs; base += u64(gridDim.x) * kWG) {
+        for (u64 base = u64(m->a) * kWG; base < pairs; base += u64(m->b) * kWG) {
             const u64 q = base + threadIdx.x;
             u32 claim_a = kNone, claim_b = kNone;
             double pw = 0.0;
@@ -1170,8 +1215,7 @@ __global__ __launch_bounds__(kWG) void k_bond_end(DevLattice<R> L, Step<R> S) {
         publishMaxima(ctl, mt, mc, ms, mp, s_max);
         return;
     }
-    const u32 tile = xcdTile(blockIdx.x, L.bond_tiles);
-    if (tile >= L.bond_tiles) return;
+    const u32 tile = m->a;
     {
         const u32 o = tile * kWG + threadIdx.x;
         u32 claim_a = kNone, claim_b = kNone;
@@ -1257,6 +1301,7 @@ __device__ double fixedOrderSum(const double *part, u32 count, u32 stride, u32 o
 }
 
 constexpr u32 kEventBatch = 4096;
+constexpr unsigned kBudgetSpent = 3U;  // the reference's max-steps exit reason
 
 // Position of the k-th (from 0) set bit of m; k < popcount(m).
 __device__ __forceinline__ u32 selectBit(u64 m, u32 k) {
@@ -1273,7 +1318,8 @@ __device__ __forceinline__ u32 selectBit(u64 m, u32 k) {
 }
 
 template <class R, bool LOADS, bool PLASTIC>
-__global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step<R> S) {
+__global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S) {
+    const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
     __shared__ double s_red[kFinishThreads];
@@ -1446,6 +1492,13 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(DevLattice<R> L, Step
                      ctl->calm_margin, ctl->removed_energy_j, ctl->removable > 0 ? available : 0.0);
     ctl->exit_reason = reason;
     if (reason != 0 || ctl->error) ctl->stop = 1;
+    // The run's step budget (RunControl::max_steps, reason 3).
+    const u64 budget = ctl->budget > 0 ? ctl->budget - 1 : 0;
+    ctl->budget = budget;
+    if (reason == 0 && budget == 0) {
+        ctl->exit_reason = kBudgetSpent;
+        ctl->stop = 1;
+    }
 }
 
 // ---------------------------------------------------------------------------

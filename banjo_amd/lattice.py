@@ -352,6 +352,82 @@ class Backend:
         return label, count.value
 
 
+class Batch:
+    """A batch of independent lattices advanced together (blx_batch_* of
+    include/banjo_lattice.h). Slots share the step settings; each slot has
+    its own state, sphere, loads, run control and status."""
+
+    def __init__(self, library: LatticeLibrary, precision, capacity, device=0):
+        self.L = library
+        self.capacity = int(capacity)
+        self.precision = precision
+        h = C.c_void_p()
+        self.L.check(self.L.fn("batch_create")(C.c_int(device), C.c_int(precision), C.c_uint32(self.capacity),
+                                               C.byref(h)), "batch_create")
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            self.L.fn("batch_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, slot, schedule: Schedule, state: State, settings: abi.Settings, sphere: abi.Sphere):
+        sv, st = schedule.view(), state.view()
+        self.L.check(self.L.fn("batch_upload")(self.h, C.c_uint32(slot), C.byref(sv), C.byref(st), C.byref(settings),
+                                               C.byref(sphere)), "batch_upload")
+        self.sizes[slot] = (schedule.node_count, schedule.bond_count)
+
+    def set_external_forces(self, slot, forces, substeps):
+        f = None if forces is None else np.ascontiguousarray(forces, np.float64).reshape(-1)
+        self.L.check(self.L.fn("batch_set_external_forces")(self.h, C.c_uint32(slot),
+                                                            None if f is None else _ptr(f, C.c_double),
+                                                            C.c_uint64(substeps)), "batch_set_external_forces")
+
+    def arm(self, slot, control: abi.RunControl):
+        self.L.check(self.L.fn("batch_arm")(self.h, C.c_uint32(slot), C.byref(control)), "batch_arm")
+
+    def advance(self, substeps, steps_per_launch=0):
+        running = C.c_uint32()
+        self.L.check(self.L.fn("batch_advance")(self.h, C.c_uint64(substeps), C.c_uint64(steps_per_launch), None,
+                                                C.byref(running)), "batch_advance")
+        return running.value
+
+    def status(self, slot) -> abi.RunStatus:
+        st = abi.RunStatus()
+        self.L.check(self.L.fn("batch_status")(self.h, C.c_uint32(slot), C.byref(st)), "batch_status")
+        return st
+
+    def download(self, slot, state: State):
+        sphere = abi.Sphere()
+        sv = state.view()
+        self.L.check(self.L.fn("batch_download")(self.h, C.c_uint32(slot), C.byref(sv), C.byref(sphere)),
+                     "batch_download")
+        return sphere
+
+    def fragment_labels(self, slot):
+        label = np.zeros(self.sizes[slot][0], np.uint32)
+        count = C.c_uint32()
+        self.L.check(self.L.fn("batch_fragment_labels")(self.h, C.c_uint32(slot), _ptr(label, C.c_uint32),
+                                                        C.byref(count)), "batch_fragment_labels")
+        return label, count.value
+
+    def release(self, slot):
+        self.L.check(self.L.fn("batch_release")(self.h, C.c_uint32(slot)), "batch_release")
+        self.sizes.pop(slot, None)
+
+    def slot_bytes(self, slot):
+        b = C.c_uint64()
+        self.L.check(self.L.fn("batch_slot_bytes")(self.h, C.c_uint32(slot), C.byref(b)), "batch_slot_bytes")
+        return b.value
+
+
 def control(max_steps, **kw) -> abi.RunControl:
     """RunControl with the reference's defaults (src/fastlattice/FastLattice.hpp:79-128)."""
     c = abi.RunControl()

@@ -70,3 +70,58 @@ def run_scene(lib: LatticeLibrary, scene, precision=abi.PRECISION_FLOAT, substep
     else:
         be.close()
     return res
+
+
+def collect(lib: LatticeLibrary, scene, sched, state_out, sphere, status, first_failure_sched, labels):
+    """Results of a finished run in asset order (the same fields as run_scene)."""
+    order = sched.bond_order
+    B = scene.bond_count
+    alive = np.empty(B, np.uint8)
+    alive[order] = state_out.alive
+    damage = np.empty(B)
+    damage[order] = state_out.damage
+    mode = np.empty(B, np.uint8)
+    mode[order] = state_out.failure_mode
+    res = dict(u=state_out.u.reshape(-1, 3), v=state_out.v.reshape(-1, 3), u_prev=state_out.u_prev.reshape(-1, 3),
+               alive=alive, damage=damage, mode=mode, status=status, sphere=sphere, schedule=sched, state=state_out)
+    if first_failure_sched is not None:
+        res["first_failure"] = np.sort(order[first_failure_sched])
+    if labels is not None:
+        res["labels"], res["components"] = labels
+    return res
+
+
+def run_batch(lib: LatticeLibrary, scenes_, precision=abi.PRECISION_FLOAT, substeps=None, control_kw=None,
+              steps_per_launch=0, capacity=None, labels=True):
+    """Advance several scenes together in one batch (they must share step
+    settings) and return each one's results as run_scene would."""
+    from .lattice import Batch
+    cap = capacity or len(scenes_)
+    bt = Batch(lib, precision, cap)
+    prepared = []
+    try:
+        for k, sc in enumerate(scenes_):
+            sched, state = prepare(lib, sc)
+            bt.upload(k, sched, state, sc.settings, sc.sphere)
+            if sc.forces is not None:
+                bt.set_external_forces(k, sc.forces, sc.force_substeps)
+            n = sc.substeps if substeps is None else (substeps[k] if hasattr(substeps, "__len__") else substeps)
+            kw = control_kw[k] if isinstance(control_kw, (list, tuple)) else (control_kw or {})
+            bt.arm(k, make_control(n, **kw))
+            prepared.append((sched, state))
+        t0 = time.perf_counter()
+        while bt.advance(1 << 62, steps_per_launch):
+            pass
+        wall = time.perf_counter() - t0
+        out = []
+        for k, sc in enumerate(scenes_):
+            sched, state = prepared[k]
+            st_out = state.copy()
+            sphere = bt.download(k, st_out)
+            res = collect(lib, sc, sched, st_out, sphere, bt.status(k), None,
+                          bt.fragment_labels(k) if labels else None)
+            res["wall"] = wall
+            out.append(res)
+        return out
+    finally:
+        bt.close()

@@ -297,3 +297,26 @@ def plastic_impact(nx=16, ny=8, nz=16, substeps=120, blocks=1, seed=975):
     sc.settings.plastic_yield_stretch = 0.004
     sc.settings.plastic_hardening = 0.1
     return sc
+
+
+def batch_edges(seed, count, lo=8, hi=48):
+    """Edge lengths round(exp(U(ln lo, ln hi))) per axis for `count` lattices."""
+    out = []
+    for t in range(count):
+        u = [(splitmix64((seed * 1000003 + t * 3 + ax) & 0xFFFFFFFFFFFFFFFF) >> 11) * 2.0 ** -53 for ax in range(3)]
+        out.append(tuple(int(round(math.exp(math.log(lo) + x * (math.log(hi) - math.log(lo))))) for x in u))
+    return out
+
+
+def config3_trial(t, seed=971, edges=None, substeps=8 * 240):
+    """One lattice of the batched form: a 26-neighbour cantilever with edges
+    log-uniform in [8, 48] per axis, per-bond compliance log-uniform in
+    [1e-7, 1e-5] and Weibull(k=6, 1.5%) break strain (as in the impact
+    workload), clamped on x=0, gravity as loads and a tip load uniform in
+    [1, 50] N; dt = 1/1920 s (8 substeps of a 1/240 s step)."""
+    nx, ny, nz = edges if edges is not None else batch_edges(seed, t + 1)[t]
+    u = (splitmix64((seed * 7919 + t) & 0xFFFFFFFFFFFFFFFF) >> 11) * 2.0 ** -53
+    tip = 1.0 + 49.0 * u
+    s = seed + 17 * t
+    return cantilever(nx, ny, nz, compliance=log_uniform_compliance(s), strain=weibull_strain(s + 1, 6.0, 0.015),
+                      tip_load=tip, dt=1.0 / 1920.0, substeps=substeps, name=f"c3_{t}_{nx}x{ny}x{nz}")

@@ -320,6 +320,47 @@ int blx_backend_clear_frames(blx_backend *backend);
 int blx_backend_fragment_labels(blx_backend *backend, uint32_t *label,
                                 uint32_t *component_count);
 
+/* ---- batches of lattices (no counterpart in the reference's API) ---------
+ *
+ * A batch advances many independent lattices of any sizes together: each
+ * phase of the substep is one launch over every running lattice ("slot").
+ * It is for callers that run many trials at once -- the reference runs one
+ * LatticeBackend per island or trial (TileImpactScene.cpp:345-351) -- and each
+ * slot's results are those of the same lattice on its own backend, bit for
+ * bit. The slots share the step settings (blx_settings: substep, gravity,
+ * supports, contact, iterations); the state, the sphere, external loads, the
+ * run control and the status are per slot. A slot keeps its device memory
+ * after blx_batch_release and reuses it for the next upload that fits.
+ * Frames are not captured in batches (capture_stride must be 0). */
+typedef struct blx_batch blx_batch;
+
+int blx_batch_create(int device, int precision, uint32_t capacity, blx_batch **out);
+int blx_batch_destroy(blx_batch *batch);
+int blx_batch_capacity(const blx_batch *batch, uint32_t *capacity);
+/* makeCudaLatticeBackend + LatticeBackend::upload for one slot. The first
+ * upload into an empty batch fixes the shared settings; a later upload with
+ * different settings fails with BLX_ERR_INVALID. */
+int blx_batch_upload(blx_batch *batch, uint32_t slot, const blx_schedule_view *schedule,
+                     const blx_state_view *state, const blx_settings *settings, const blx_sphere *sphere);
+int blx_batch_set_external_forces(blx_batch *batch, uint32_t slot, const double *forces_3n,
+                                  uint64_t substeps);
+/* The slot's run control for the following advances: its exit rules and a
+ * budget of max_steps substeps, after which it stops with exit reason 3. */
+int blx_batch_arm(blx_batch *batch, uint32_t slot, const blx_run_control *control);
+/* Advances every armed slot until it stops or `substeps` substeps have run,
+ * in device batches of steps_per_launch (0: 240); abandon is polled between
+ * device batches. *running is the number of slots still running. */
+int blx_batch_advance(blx_batch *batch, uint64_t substeps, uint64_t steps_per_launch,
+                      const volatile uint8_t *abandon, uint32_t *running);
+int blx_batch_status(const blx_batch *batch, uint32_t slot, blx_run_status *status);
+int blx_batch_download(blx_batch *batch, uint32_t slot, blx_state_view *state, blx_sphere *sphere);
+int blx_batch_fragment_labels(blx_batch *batch, uint32_t slot, uint32_t *label,
+                              uint32_t *component_count);
+/* The slot leaves the batch; its device memory is kept for reuse. */
+int blx_batch_release(blx_batch *batch, uint32_t slot);
+/* Device bytes the slot holds. */
+int blx_batch_slot_bytes(const blx_batch *batch, uint32_t slot, uint64_t *bytes);
+
 /* ---- measurement (no counterpart in the reference's API) ---------------- */
 
 /* Per-launch HIP-event timing of every kernel (off by default: it shortens
