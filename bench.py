@@ -141,6 +141,63 @@ def gpu_leg(args, sc):
     return result
 
 
+def batched_leg(args):
+    """The batched form: many cantilevers of uneven sizes advanced together in
+    one engine, against the plain per-trial backend on a random sample of the
+    same trials (whose results must be identical, bit for bit)."""
+    import numpy as np
+    from banjo_amd import abi, scenes
+    from banjo_amd.lattice import Batch, control, product
+    from banjo_amd.runner import prepare
+
+    hip = product()
+    T, sub = args.batch_trials, args.batch_substeps
+    edges = scenes.batch_edges(971, T)
+    t0 = time.perf_counter()
+    scs = [scenes.config3_trial(t, edges=edges[t], substeps=sub) for t in range(T)]
+    prepared = [prepare(hip, sc) for sc in scs]
+    t_build = time.perf_counter() - t0
+    bt = Batch(hip, abi.PRECISION_FLOAT, T)
+    t0 = time.perf_counter()
+    for k, (sc, (sched, state)) in enumerate(zip(scs, prepared)):
+        bt.upload(k, sched, state, sc.settings, sc.sphere)
+        bt.set_external_forces(k, sc.forces, sc.force_substeps)
+        bt.arm(k, control(sub))
+    t_upload = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    while bt.advance(1 << 62, args.batch_steps_per_launch):
+        pass
+    t_run = time.perf_counter() - t0
+    bonds = sum(sc.bond_count for sc in scs)
+    rng = np.random.default_rng(971)
+    sample = sorted(rng.choice(T, size=min(args.plain_sample, T), replace=False).tolist())
+    t_plain = 0.0
+    identical = True
+    for k in sample:
+        sc, (sched, state) = scs[k], prepared[k]
+        be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+        be.upload(state, sc.settings, sc.sphere)
+        be.set_external_forces(sc.forces, sc.force_substeps)
+        t0 = time.perf_counter()
+        be.run(control(sub))
+        t_plain += time.perf_counter() - t0
+        a, b = state.copy(), state.copy()
+        be.download(a)
+        bt.download(k, b)
+        identical &= bool(np.array_equal(a.u, b.u) and np.array_equal(a.v, b.v) and np.array_equal(a.alive, b.alive))
+        be.close()
+    broken = sum(bt.status(k).broken_bonds for k in range(T))
+    bt.close()
+    plain_tps = len(sample) / t_plain
+    return {"trials": T, "substeps_per_trial": sub, "nodes": sum(sc.node_count for sc in scs), "bonds": bonds,
+            "broken_bonds": broken, "seconds": t_run, "trials_per_s": T / t_run,
+            "bond_solves_per_s": bonds * sub / t_run, "plain_trials_per_s": plain_tps,
+            "speedup_vs_plain": (T / t_run) / plain_tps, "plain_sample": len(sample),
+            "identical_to_plain": identical, "host_build_s": t_build, "upload_s": t_upload,
+            "workload": "cantilevers, edges log-uniform in [8, 48] per axis, 26-neighbour, Weibull(6, 1.5%) "
+                        "break strain, log-uniform compliance, tip load U(1, 50) N, dt 1/1920 s"}
+
+
 def cpu_leg(args, sc):
     """The reference's threaded CPU lane over a bounded window of the same workload."""
     from banjo_amd import abi
@@ -197,6 +254,12 @@ def main():
     ap.add_argument("--profile-substeps", type=int, default=1200)
     ap.add_argument("--cpu-substeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain,batched",
+                    help="comma list of plain (the headline impact), batched (many uneven lattices)")
+    ap.add_argument("--batch-trials", type=int, default=512)
+    ap.add_argument("--batch-substeps", type=int, default=8 * 240)
+    ap.add_argument("--batch-steps-per-launch", type=int, default=0)
+    ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the HBM copy probe of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -208,6 +271,13 @@ def main():
               f"(the lattice step would shard by z-slabs across GPUs, which is designed but not built)",
               file=sys.stderr)
         raise SystemExit(2)
+    forms = set(args.forms.split(","))
+    if "plain" not in forms:
+        out = {"metric": METRIC, "n_gpus": 1}
+        if "batched" in forms:
+            out["batched"] = batched_leg(args)
+        print(json.dumps(out))
+        return
     t_build = time.perf_counter()
     sc = make_workload(args)
     N, B = sc.node_count, sc.bond_count
@@ -252,6 +322,8 @@ def main():
         "copy_probe_gbps": probe["gbps_read_plus_write"],
         "cpu_baseline": cpu,
     }
+    if "batched" in forms:
+        out["batched"] = batched_leg(args)
     print(json.dumps(out))
 
 
