@@ -247,6 +247,7 @@ public:
     bool scheduled = false;
     bool uploaded = false;
     bool armed = false;  // takes part in the next advance
+    bool pristine = false;  // uploaded and never run: a template for clones
     bool loads_set = false;
     uint32_t N = 0, B = 0, D = 0, node_tiles = 0, bond_tiles = 0, group_words = 0;
     uint32_t slot_palette = 0, edge_palette = 0;
@@ -265,6 +266,16 @@ public:
     // is a list of bond ranges of at most one workgroup each.
     std::vector<std::vector<uint2>> stages;
     size_t interior_stages = 0;
+    // Offsets of the device arrays in the arena.
+    struct Off {
+        size_t x04, u4, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
+        size_t nbr_key, spal, sdelta, nbr_bond, epal, nodir, rec, alive, lambda, pext;
+        size_t bnd, alive_n, thr, thr2, pvd, mode, pstr, slots;
+        size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
+        size_t affected, affected_list, tile_plastic, load_part, fext, freq, cand_all, cand_tile, cand_lo, cand_cnt;
+        size_t nbb, first_fail;
+    } off{};
+    size_t used = 0;  // arena bytes the arrays occupy
     Arena arena;
     DevBuf cc_parent, cc_root, cc_size, cc_rank, cc_label, cc_keys, cc_sorted, cc_count, cc_temp;
 
@@ -289,6 +300,113 @@ public:
         add(1, 1, 2);
         scheduled = true;
         uploaded = false;
+        armed = false;
+    }
+
+    // The device view of the arrays in this slot's arena.
+    void bind(DevCtl *ctl_dev, Sphere<R> *sphere_dev, const double *origin) {
+        const Off &o = off;
+        DevLattice<R> &l = L;
+        l = DevLattice<R>{};
+        l.N = N;
+        l.B = B;
+        l.D = D;
+        l.blocks = sch.blocks;
+        l.node_tiles = node_tiles;
+        l.bond_tiles = bond_tiles;
+        l.group_words = group_words;
+        l.tensile_only = tensile_only ? 1U : 0U;
+        l.u4 = arena.at<R4<R>>(o.u4);
+        l.up4 = arena.at<R4<R>>(o.up4);
+        l.v4 = arena.at<R4<R>>(o.v4);
+        l.x04 = arena.at<R4<R>>(o.x04);
+        l.rinv = arena.at<R>(o.rinv);
+        l.unm = arena.at<R>(o.unm);
+        l.sa = arena.at<R4<R>>(o.sa);
+        l.sb = arena.at<R4<R>>(o.sb);
+        l.valid = arena.at<uint8_t>(o.valid);
+        l.approach = arena.at<R>(o.approach);
+        l.engaged = arena.at<uint8_t>(o.engaged);
+        l.cand = arena.at<uint8_t>(o.cand);
+        l.nbr_key = arena.at<uint32_t>(o.nbr_key);
+        l.spal = arena.at<R4<R>>(o.spal);
+        l.sdelta = arena.at<int>(o.sdelta);
+        l.nbr_bond = arena.at<uint32_t>(o.nbr_bond);
+        l.epal = arena.at<EdgePal<R>>(o.epal);
+        l.rec = arena.at<BondRec<R>>(o.rec);
+        l.alive = arena.at<uint8_t>(o.alive);
+        l.lambda = arena.at<R>(o.lambda);
+        l.pext = arena.at<R>(o.pext);
+        l.bnd = arena.at<BondNode>(o.bnd);
+        l.alive_n = arena.at<uint8_t>(o.alive_n);
+        l.thr = arena.at<Thresholds<R>>(o.thr);
+        l.thr2 = arena.at<Tensile<R>>(o.thr2);
+        l.pvd = arena.at<R4<R>>(o.pvd);
+        l.mode = arena.at<uint8_t>(o.mode);
+        l.pstr = arena.at<R>(o.pstr);
+        l.slots = arena.at<uint2>(o.slots);
+        l.ev_bits = arena.at<uint64_t>(o.ev_bits);
+        l.brk_bits = arena.at<uint64_t>(o.brk_bits);
+        l.grp_bits = arena.at<uint64_t>(o.grp_bits);
+        l.ev_d = arena.at<float>(o.ev_d);
+        l.ev_e = arena.at<R>(o.ev_e);
+        l.flag_list = arena.at<uint32_t>(o.flag_list);
+        l.tile_cand = arena.at<uint32_t>(o.tile_cand);
+        l.dirty_list = arena.at<uint32_t>(o.dirty_list);
+        l.dirty_stamp = arena.at<uint32_t>(o.dirty_stamp);
+        l.rank_stamp = arena.at<uint32_t>(o.rank_stamp);
+        l.affected = arena.at<uint32_t>(o.affected);
+        l.affected_list = arena.at<uint32_t>(o.affected_list);
+        l.tile_plastic = arena.at<double>(o.tile_plastic);
+        l.load_part = arena.at<double>(o.load_part);
+        l.fext = arena.at<R4<R>>(o.fext);
+        l.freq = arena.at<double>(o.freq);
+        l.cand_all = arena.at<uint32_t>(o.cand_all);
+        l.cand_tile = arena.at<uint32_t>(o.cand_tile);
+        l.cand_lo = arena.at<uint32_t>(o.cand_lo);
+        l.cand_cnt = arena.at<uint32_t>(o.cand_cnt);
+        l.nbb = arena.at<uint32_t>(o.nbb);
+        l.first_fail = arena.at<uint32_t>(o.first_fail);
+        l.ctl = ctl_dev;
+        l.sphere = sphere_dev;
+        for (int k = 0; k < 3; ++k) l.origin[k] = origin[k];
+    }
+
+    // Takes the lattice of `src` as uploaded (its arrays copied on the
+    // device, its host-side tables copied); the caller places the control
+    // block and the sphere.
+    void cloneFrom(const Slot &src, hipStream_t stream, DevCtl *ctl_dev, Sphere<R> *sphere_dev) {
+        if (!src.uploaded) throw Logic("clone of a slot with no lattice");
+        sch = src.sch;
+        stages = src.stages;
+        interior_stages = src.interior_stages;
+        scheduled = true;
+        N = src.N;
+        B = src.B;
+        D = src.D;
+        node_tiles = src.node_tiles;
+        bond_tiles = src.bond_tiles;
+        group_words = src.group_words;
+        slot_palette = src.slot_palette;
+        edge_palette = src.edge_palette;
+        tensile_only = src.tensile_only;
+        mass_host = src.mass_host;
+        inv_mass_host = src.inv_mass_host;
+        node_j_host = src.node_j_host;
+        off = src.off;
+        used = src.used;
+        arena.reset();
+        arena.template reserve<char>(used);
+        arena.commit();
+        HIP_OK(hipMemcpyAsync(arena.template at<char>(0), src.arena.template at<char>(0), used, hipMemcpyDeviceToDevice,
+                              stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        bind(ctl_dev, sphere_dev, src.L.origin);
+        loads_set = false;
+        frames.clear();
+        status = blx_run_status{};
+        status.first_failure_step = status.last_failure_step = status.last_energy_gain_step = ~0ULL;
+        uploaded = true;
         armed = false;
     }
 
@@ -413,14 +531,8 @@ public:
         // Layout in the arena.
         arena.reset();
         const uint32_t np = edge_palette;
-        struct Off {
-            size_t x04, u4, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
-            size_t nbr_key, spal, sdelta, nbr_bond, epal, nodir, rec, alive, lambda, pext;
-            size_t bnd, alive_n, thr, thr2, pvd, mode, pstr, slots;
-            size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
-            size_t affected, affected_list, tile_plastic, load_part, fext, freq, cand_all, cand_tile, cand_lo, cand_cnt;
-            size_t nbb, first_fail;
-        } o{};
+        Off &o = off;
+        o = Off{};
         o.x04 = arena.reserve<R4<R>>(N);
         o.u4 = arena.reserve<R4<R>>(N);
         o.up4 = arena.reserve<R4<R>>(N);
@@ -547,71 +659,8 @@ public:
             HIP_OK(hipStreamSynchronize(stream));
         }
 
-        // The device view.
-        DevLattice<R> &l = L;
-        l = DevLattice<R>{};
-        l.N = N;
-        l.B = B;
-        l.D = D;
-        l.blocks = sch.blocks;
-        l.node_tiles = node_tiles;
-        l.bond_tiles = bond_tiles;
-        l.group_words = group_words;
-        l.tensile_only = tensile ? 1U : 0U;
-        l.u4 = arena.at<R4<R>>(o.u4);
-        l.up4 = arena.at<R4<R>>(o.up4);
-        l.v4 = arena.at<R4<R>>(o.v4);
-        l.x04 = arena.at<R4<R>>(o.x04);
-        l.rinv = arena.at<R>(o.rinv);
-        l.unm = arena.at<R>(o.unm);
-        l.sa = arena.at<R4<R>>(o.sa);
-        l.sb = arena.at<R4<R>>(o.sb);
-        l.valid = arena.at<uint8_t>(o.valid);
-        l.approach = arena.at<R>(o.approach);
-        l.engaged = arena.at<uint8_t>(o.engaged);
-        l.cand = arena.at<uint8_t>(o.cand);
-        l.nbr_key = arena.at<uint32_t>(o.nbr_key);
-        l.spal = arena.at<R4<R>>(o.spal);
-        l.sdelta = arena.at<int>(o.sdelta);
-        l.nbr_bond = arena.at<uint32_t>(o.nbr_bond);
-        l.epal = arena.at<EdgePal<R>>(o.epal);
-        l.rec = arena.at<BondRec<R>>(o.rec);
-        l.alive = arena.at<uint8_t>(o.alive);
-        l.lambda = arena.at<R>(o.lambda);
-        l.pext = arena.at<R>(o.pext);
-        l.bnd = arena.at<BondNode>(o.bnd);
-        l.alive_n = arena.at<uint8_t>(o.alive_n);
-        l.thr = arena.at<Thresholds<R>>(o.thr);
-        l.thr2 = arena.at<Tensile<R>>(o.thr2);
-        l.pvd = arena.at<R4<R>>(o.pvd);
-        l.mode = arena.at<uint8_t>(o.mode);
-        l.pstr = arena.at<R>(o.pstr);
-        l.slots = arena.at<uint2>(o.slots);
-        l.ev_bits = arena.at<uint64_t>(o.ev_bits);
-        l.brk_bits = arena.at<uint64_t>(o.brk_bits);
-        l.grp_bits = arena.at<uint64_t>(o.grp_bits);
-        l.ev_d = arena.at<float>(o.ev_d);
-        l.ev_e = arena.at<R>(o.ev_e);
-        l.flag_list = arena.at<uint32_t>(o.flag_list);
-        l.tile_cand = arena.at<uint32_t>(o.tile_cand);
-        l.dirty_list = arena.at<uint32_t>(o.dirty_list);
-        l.dirty_stamp = arena.at<uint32_t>(o.dirty_stamp);
-        l.rank_stamp = arena.at<uint32_t>(o.rank_stamp);
-        l.affected = arena.at<uint32_t>(o.affected);
-        l.affected_list = arena.at<uint32_t>(o.affected_list);
-        l.tile_plastic = arena.at<double>(o.tile_plastic);
-        l.load_part = arena.at<double>(o.load_part);
-        l.fext = arena.at<R4<R>>(o.fext);
-        l.freq = arena.at<double>(o.freq);
-        l.cand_all = arena.at<uint32_t>(o.cand_all);
-        l.cand_tile = arena.at<uint32_t>(o.cand_tile);
-        l.cand_lo = arena.at<uint32_t>(o.cand_lo);
-        l.cand_cnt = arena.at<uint32_t>(o.cand_cnt);
-        l.nbb = arena.at<uint32_t>(o.nbb);
-        l.first_fail = arena.at<uint32_t>(o.first_fail);
-        l.ctl = ctl_dev;
-        l.sphere = sphere_dev;
-        for (int k = 0; k < 3; ++k) l.origin[k] = st.origin[k];
+        used = arena.used();
+        bind(ctl_dev, sphere_dev, st.origin);
 
         // Control block and sphere. A slot stays stopped until it is armed.
         DevCtl c{};
@@ -635,6 +684,7 @@ public:
         status.first_failure_step = status.last_failure_step = status.last_energy_gain_step = ~0ULL;
         uploaded = true;
         armed = false;
+        pristine = true;
     }
 };
 
@@ -741,6 +791,30 @@ public:
         maps_dirty_ = true;
     }
 
+    // Slot dst takes a copy of the lattice of slot src as it was uploaded
+    // (src must not have run) with its own sphere: admitting a trial of a
+    // lattice already on the device is a device-to-device copy.
+    void clone(uint32_t dst, uint32_t src, const blx_sphere &sp) {
+        HIP_OK(hipSetDevice(device_));
+        if (dst == src) throw Invalid("a slot cannot be cloned into itself");
+        Slot<R> &d = slot(dst);
+        const Slot<R> &s = slot(src);
+        if (!s.uploaded || !s.pristine) throw Logic("clone source must be an uploaded lattice that has not run");
+        HIP_OK(hipStreamSynchronize(stream_));
+        d.uploaded = false;
+        d.cloneFrom(s, stream_, ctls_.as<DevCtl>() + dst, spheres_.as<Sphere<R>>() + dst);
+        DevCtl c = readCtl(src);
+        writeCtl(dst, c);
+        d.ctl_host = c;
+        const Sphere<R> s0{{R(sp.center[0]), R(sp.center[1]), R(sp.center[2])},
+                           {R(sp.velocity[0]), R(sp.velocity[1]), R(sp.velocity[2])},
+                           {R(sp.angular_velocity[0]), R(sp.angular_velocity[1]), R(sp.angular_velocity[2])},
+                           R(sp.radius), R(sp.mass), R(sp.inertia)};
+        HIP_OK(hipMemcpy(spheres_.as<Sphere<R>>() + dst, &s0, sizeof(s0), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(ls_.as<DevLattice<R>>() + dst, &d.L, sizeof(DevLattice<R>), hipMemcpyHostToDevice));
+        maps_dirty_ = true;
+    }
+
     // The slot leaves the batch and keeps its arena for the next upload.
     void release(uint32_t k) {
         Slot<R> &sl = slot(k);
@@ -817,6 +891,7 @@ public:
         writeCtl(k, ctl);
         sl.ctl_host = ctl;
         sl.armed = c.max_steps > 0;
+        sl.pristine = false;
         maps_dirty_ = true;
     }
 
@@ -986,6 +1061,13 @@ public:
             st.plastic_strain[q] = double(pstr[o]);
             st.failure_mode[q] = mode[o];
         }
+        sphereOut(readSphere(k), sp);
+    }
+
+    void sphereOf(uint32_t k, blx_sphere &sp) {
+        if (!slot(k).uploaded) throw Logic("sphere of an empty slot");
+        HIP_OK(hipSetDevice(device_));
+        HIP_OK(hipStreamSynchronize(stream_));
         sphereOut(readSphere(k), sp);
     }
 
@@ -1585,6 +1667,12 @@ int blx_batch_upload(blx_batch *b, uint32_t slot, const blx_schedule_view *sv, c
         });
     });
 }
+int blx_batch_clone(blx_batch *b, uint32_t dst, uint32_t src, const blx_sphere *sp) {
+    return guard([&] {
+        if (!sp) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) { e.clone(dst, src, *sp); });
+    });
+}
 int blx_batch_set_external_forces(blx_batch *b, uint32_t slot, const double *f, uint64_t n) {
     return guard([&] { dispatch(H(b), [&](auto &e) { e.setForces(slot, f, n); }); });
 }
@@ -1609,6 +1697,15 @@ int blx_batch_status(const blx_batch *b, uint32_t slot, blx_run_status *st) {
 }
 int blx_batch_download(blx_batch *b, uint32_t slot, blx_state_view *st, blx_sphere *sp) {
     return guard([&] { dispatch(H(b), [&](auto &e) { e.download(slot, *st, *sp); }); });
+}
+int blx_batch_sphere(blx_batch *b, uint32_t slot, blx_sphere *sp) {
+    return guard([&] {
+        if (!sp) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) { e.sphereOf(slot, *sp); });
+    });
+}
+int blx_batch_first_failure_bonds(const blx_batch *b, uint32_t slot, uint32_t *out, uint64_t cap, uint64_t *count) {
+    return guard([&] { *count = dispatch(CH(b), [&](auto &e) { return e.firstFailure(slot, out, cap); }); });
 }
 int blx_batch_fragment_labels(blx_batch *b, uint32_t slot, uint32_t *label, uint32_t *count) {
     return guard([&] { *count = dispatch(H(b), [&](auto &e) { return e.labels(slot, label); }); });

@@ -384,6 +384,11 @@ class Batch:
                                                C.byref(sphere)), "batch_upload")
         self.sizes[slot] = (schedule.node_count, schedule.bond_count)
 
+    def clone(self, dst, src, sphere: abi.Sphere):
+        self.L.check(self.L.fn("batch_clone")(self.h, C.c_uint32(dst), C.c_uint32(src), C.byref(sphere)),
+                     "batch_clone")
+        self.sizes[dst] = self.sizes[src]
+
     def set_external_forces(self, slot, forces, substeps):
         f = None if forces is None else np.ascontiguousarray(forces, np.float64).reshape(-1)
         self.L.check(self.L.fn("batch_set_external_forces")(self.h, C.c_uint32(slot),
@@ -410,6 +415,20 @@ class Batch:
         self.L.check(self.L.fn("batch_download")(self.h, C.c_uint32(slot), C.byref(sv), C.byref(sphere)),
                      "batch_download")
         return sphere
+
+    def sphere(self, slot) -> abi.Sphere:
+        sp = abi.Sphere()
+        self.L.check(self.L.fn("batch_sphere")(self.h, C.c_uint32(slot), C.byref(sp)), "batch_sphere")
+        return sp
+
+    def first_failure_bonds(self, slot):
+        count = C.c_uint64()
+        f = self.L.fn("batch_first_failure_bonds")
+        self.L.check(f(self.h, C.c_uint32(slot), None, C.c_uint64(0), C.byref(count)), "batch_first_failure_bonds")
+        out = np.zeros(count.value, np.uint32)
+        self.L.check(f(self.h, C.c_uint32(slot), _ptr(out, C.c_uint32), C.c_uint64(count.value), C.byref(count)),
+                     "batch_first_failure_bonds")
+        return out
 
     def fragment_labels(self, slot):
         label = np.zeros(self.sizes[slot][0], np.uint32)

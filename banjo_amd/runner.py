@@ -118,7 +118,7 @@ def run_batch(lib: LatticeLibrary, scenes_, precision=abi.PRECISION_FLOAT, subst
             sched, state = prepared[k]
             st_out = state.copy()
             sphere = bt.download(k, st_out)
-            res = collect(lib, sc, sched, st_out, sphere, bt.status(k), None,
+            res = collect(lib, sc, sched, st_out, sphere, bt.status(k), bt.first_failure_bonds(k),
                           bt.fragment_labels(k) if labels else None)
             res["wall"] = wall
             out.append(res)

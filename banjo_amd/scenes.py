@@ -320,3 +320,52 @@ def config3_trial(t, seed=971, edges=None, substeps=8 * 240):
     s = seed + 17 * t
     return cantilever(nx, ny, nz, compliance=log_uniform_compliance(s), strain=weibull_strain(s + 1, 6.0, 0.015),
                       tip_load=tip, dt=1.0 / 1920.0, substeps=substeps, name=f"c3_{t}_{nx}x{ny}x{nz}")
+
+
+SERVE_DT = 1.0 / 1920.0          # 8 substeps of a 1/240 s step
+SERVE_SUBSTEPS = 8 * 720          # 3 s of simulated time
+MARBLE_RADIUS, MARBLE_MASS = 0.05, 0.05
+
+
+def served_lattice(edges, seed=971, cell=0.01, node_mass=1.0e-3, name=None):
+    """A lattice of the served trial stream: the batched form's material on a
+    ground plane at y = 0 (its bottom layer resting on it), so every trial
+    shares the step settings whatever its size. The sphere is set per trial
+    (served_marble)."""
+    nx, ny, nz = edges
+    pos, a, b, L = box26(nx, ny, nz, cell)
+    pos = pos.copy()
+    pos[:, 1] += 0.5 * (ny - 1) * cell + 0.5 * cell
+    N = pos.shape[0]
+    c = log_uniform_compliance(seed)(a, b)
+    e = weibull_strain(seed + 1, 6.0, 0.015)(a, b)
+    asset = Asset(reference_position=pos, mass=np.full(N, node_mass), bond_a=a, bond_b=b, rest_length=L,
+                  compliance=c, threshold=_thresholds(e, e))
+    st = settings(SERVE_DT, gravity=(0.0, -9.81, 0.0), planes=[ground_plane(0.0, 0.5 * cell)], sphere=True,
+                  cell=cell, ball_radius=MARBLE_RADIUS)
+    return Scene(name=name or f"served_{nx}x{ny}x{nz}", dims=(nx, ny, nz), cell=cell, asset=asset,
+                 slabs=box_slabs(nx, ny, nz, 1, 1), settings=st, sphere=served_marble(edges, 0.0, 0.0, 0.0),
+                 substeps=SERVE_SUBSTEPS)
+
+
+def served_marble(edges, speed, tilt, azimuth, cell=0.01, gap=0.002):
+    """The trial's marble: 5 cm, 50 g, just above the top-face centre, moving
+    at `speed` within `tilt` radians of straight down."""
+    ny = edges[1]
+    top = ny * cell
+    v = (speed * math.sin(tilt) * math.cos(azimuth), -speed * math.cos(tilt), speed * math.sin(tilt) * math.sin(azimuth))
+    return sphere_state(center=(0.0, top + MARBLE_RADIUS + gap, 0.0), velocity=v, radius=MARBLE_RADIUS,
+                        mass=MARBLE_MASS)
+
+
+def served_stream(seed, rate, seconds, templates):
+    """Poisson arrivals at `rate` per second for `seconds`: (arrival time,
+    template index, speed U(0.5, 3) m/s, tilt up to 0.3 rad, azimuth)."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            return out
+        out.append((t, int(rng.integers(templates)), float(rng.uniform(0.5, 3.0)), float(rng.uniform(0.0, 0.3)),
+                    float(rng.uniform(0.0, 2.0 * math.pi))))

@@ -198,6 +198,46 @@ def batched_leg(args):
                         "break strain, log-uniform compliance, tip load U(1, 50) N, dt 1/1920 s"}
 
 
+def served_leg(args):
+    """The served form: Poisson arrivals of marble trials over lattices of the
+    batched form's size distribution, admitted into a continuously running
+    batch as slots and the HBM budget allow; p50/p99 latency from arrival to
+    answer."""
+    from banjo_amd import scenes
+    from banjo_amd.lattice import product
+    from banjo_amd.serve import Trial, TrialServer, latency_summary
+
+    hip = product()
+    pool = args.serve_templates
+    edges = scenes.batch_edges(4242, pool)
+    t0 = time.perf_counter()
+    lats = [scenes.served_lattice(e, seed=4242 + k) for k, e in enumerate(edges)]
+    server = TrialServer(hip, lats, slots=args.serve_slots, hbm_budget_bytes=args.serve_hbm_gib << 30,
+                         chunk=args.serve_chunk)
+    t_setup = time.perf_counter() - t0
+    stream = scenes.served_stream(971, args.serve_rate, args.serve_seconds, pool)
+    trials = [Trial(id=i, arrival=a, template=k, sphere=scenes.served_marble(edges[k], v, tilt, az),
+                    max_steps=scenes.SERVE_SUBSTEPS)
+              for i, (a, k, v, tilt, az) in enumerate(stream)]
+    try:
+        answers = server.serve(trials)
+    finally:
+        server.close()
+    s = latency_summary(answers)
+    bond_solves = sum(lats[t.template].bond_count * a.substeps for a, t in
+                      zip(sorted(answers, key=lambda a: a.id), trials))
+    span = max(a.finished for a in answers) - min(a.arrival for a in answers)
+    s.update({"offered_rate_per_s": args.serve_rate, "stream_seconds": args.serve_seconds,
+              "bond_solves_per_s": bond_solves / span, "peak_concurrent_trials": server.peak_active,
+              "peak_admitted_bytes": server.peak_bytes, "hbm_budget_bytes": args.serve_hbm_gib << 30,
+              "slots": args.serve_slots, "chunk_substeps": args.serve_chunk, "lattice_pool": pool,
+              "setup_s": t_setup, "host_seconds": dict(server.seconds),
+              "workload": "marble 5 cm, 50 g, 0.5-3 m/s within 0.3 rad of vertical, onto lattices with edges "
+                          "log-uniform in [8, 48] on a ground plane; up to 3 s (5760 substeps of 1/1920 s), "
+                          "ending early by the calm/quiet rules (480 substeps)"})
+    return s
+
+
 def cpu_leg(args, sc):
     """The reference's threaded CPU lane over a bounded window of the same workload."""
     from banjo_amd import abi
@@ -254,12 +294,19 @@ def main():
     ap.add_argument("--profile-substeps", type=int, default=1200)
     ap.add_argument("--cpu-substeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain,batched",
-                    help="comma list of plain (the headline impact), batched (many uneven lattices)")
+    ap.add_argument("--forms", default="plain,batched,served",
+                    help="comma list of plain (the headline impact), batched (many uneven lattices), "
+                         "served (a stream of trials)")
     ap.add_argument("--batch-trials", type=int, default=512)
     ap.add_argument("--batch-substeps", type=int, default=8 * 240)
     ap.add_argument("--batch-steps-per-launch", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-templates", type=int, default=64)
+    ap.add_argument("--serve-slots", type=int, default=768)
+    ap.add_argument("--serve-hbm-gib", type=int, default=200)
+    ap.add_argument("--serve-chunk", type=int, default=48)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the HBM copy probe of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -276,6 +323,8 @@ def main():
         out = {"metric": METRIC, "n_gpus": 1}
         if "batched" in forms:
             out["batched"] = batched_leg(args)
+        if "served" in forms:
+            out["served"] = served_leg(args)
         print(json.dumps(out))
         return
     t_build = time.perf_counter()
@@ -324,6 +373,8 @@ def main():
     }
     if "batched" in forms:
         out["batched"] = batched_leg(args)
+    if "served" in forms:
+        out["served"] = served_leg(args)
     print(json.dumps(out))
 
 

@@ -342,6 +342,10 @@ int blx_batch_capacity(const blx_batch *batch, uint32_t *capacity);
  * different settings fails with BLX_ERR_INVALID. */
 int blx_batch_upload(blx_batch *batch, uint32_t slot, const blx_schedule_view *schedule,
                      const blx_state_view *state, const blx_settings *settings, const blx_sphere *sphere);
+/* Slot dst takes the lattice of slot src as it was uploaded (src must not
+ * have run yet), with its own sphere: a device-to-device copy, for trials of
+ * lattices kept resident as templates. */
+int blx_batch_clone(blx_batch *batch, uint32_t dst, uint32_t src, const blx_sphere *sphere);
 int blx_batch_set_external_forces(blx_batch *batch, uint32_t slot, const double *forces_3n,
                                   uint64_t substeps);
 /* The slot's run control for the following advances: its exit rules and a
@@ -356,6 +360,11 @@ int blx_batch_status(const blx_batch *batch, uint32_t slot, blx_run_status *stat
 int blx_batch_download(blx_batch *batch, uint32_t slot, blx_state_view *state, blx_sphere *sphere);
 int blx_batch_fragment_labels(blx_batch *batch, uint32_t slot, uint32_t *label,
                               uint32_t *component_count);
+/* The slot's sphere without a state download. */
+int blx_batch_sphere(blx_batch *batch, uint32_t slot, blx_sphere *sphere);
+/* LatticeBackend::firstFailureBonds for one slot (see blx_backend_first_failure_bonds). */
+int blx_batch_first_failure_bonds(const blx_batch *batch, uint32_t slot, uint32_t *out, uint64_t capacity,
+                                  uint64_t *count);
 /* The slot leaves the batch; its device memory is kept for reuse. */
 int blx_batch_release(blx_batch *batch, uint32_t slot);
 /* Device bytes the slot holds. */
