@@ -68,6 +68,12 @@ class Sphere(C.Structure):
                 ("radius", f64), ("mass", f64), ("inertia", f64)]
 
 
+class Blade(C.Structure):
+    """blx_blade: the kinematic blade extension (no reference counterpart)."""
+    _fields_ = [("enabled", u8), ("edge_point", f64 * 3), ("flat_normal", f64 * 3), ("direction", f64 * 3),
+                ("speed", f64), ("half_width", f64), ("node_radius", f64)]
+
+
 class RunControl(C.Structure):
     """RunControl, src/fastlattice/FastLattice.hpp:79-128."""
     _fields_ = [("max_steps", u64), ("quiet_steps", u64), ("min_steps", u64),

@@ -766,7 +766,9 @@ public:
             shared_.direct = s.direct_arithmetic != 0;
             shared_.plastic = s.plastic_yield_stretch > 0.0;
             shared_.iterations = s.constraint_iterations;
+            const Blade<R> blade = S_.blade;
             S_ = S;
+            S_.blade = blade;
         }
         HIP_OK(hipStreamSynchronize(stream_));
         sl.uploaded = false;
@@ -813,6 +815,26 @@ public:
         HIP_OK(hipMemcpy(spheres_.as<Sphere<R>>() + dst, &s0, sizeof(s0), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(ls_.as<DevLattice<R>>() + dst, &d.L, sizeof(DevLattice<R>), hipMemcpyHostToDevice));
         maps_dirty_ = true;
+    }
+
+    // The kinematic blade of every slot (an extension of the support planes).
+    void setBlade(const blx_blade &b) {
+        HIP_OK(hipSetDevice(device_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        Blade<R> o{};
+        o.enabled = b.enabled ? 1U : 0U;
+        if (o.enabled) {
+            if (!(b.speed >= 0.0) || !std::isfinite(b.speed) || !(b.half_width >= 0.0) || !(b.node_radius >= 0.0))
+                throw Invalid("blade speed, half width and node radius must be finite and non-negative");
+            auto v = [](const double *x) { return V3<R>{R(x[0]), R(x[1]), R(x[2])}; };
+            o.c0 = v(b.edge_point);
+            o.f = v(b.flat_normal);
+            o.m = v(b.direction);
+            o.speed = R(b.speed);
+            o.hw = R(b.half_width);
+            o.r = R(b.node_radius);
+        }
+        S_.blade = o;
     }
 
     // The slot leaves the batch and keeps its arena for the next upload.
@@ -1100,8 +1122,8 @@ public:
         Slot<R> &sl = slot(k);
         if (!sl.uploaded) throw Logic("fragments before upload");
         HIP_OK(hipSetDevice(device_));
-        HIP_OK(hipStreamSynchronize(stream_));
         const uint32_t N = sl.N;
+        if (N == 0) return 0;
         const size_t n4 = size_t(N) * 4;
         sl.cc_parent.alloc(n4);
         sl.cc_root.alloc(n4);
@@ -1119,9 +1141,10 @@ public:
         uint32_t *parent = sl.cc_parent.template as<uint32_t>(), *root = sl.cc_root.template as<uint32_t>(),
                  *size = sl.cc_size.template as<uint32_t>(), *count = sl.cc_count.template as<uint32_t>();
         HIP_OK(hipMemsetAsync(count, 0, 4, stream_));
-        hipLaunchKernelGGL(k_cc_init, dim3(g), dim3(kWG), 0, stream_, parent, size, N);
-        hipLaunchKernelGGL(k_cc_hook<R>, dim3(gridFor(sl.B)), dim3(kWG), 0, stream_, sl.L, parent);
-        hipLaunchKernelGGL(k_cc_roots, dim3(g), dim3(kWG), 0, stream_, parent, root, size, N);
+        const uint32_t ranges = std::max(1U, (N + kCcRange - 1) / kCcRange);
+        hipLaunchKernelGGL(k_cc_local<R>, dim3(ranges), dim3(kCcRange), 0, stream_, sl.L, parent, size);
+        hipLaunchKernelGGL(k_cc_cross<R>, dim3(ranges), dim3(kCcRange), 0, stream_, sl.L, parent);
+        hipLaunchKernelGGL(k_cc_flatten<true>, dim3(gridFew(N)), dim3(kWG), 0, stream_, parent, root, size, N);
         hipLaunchKernelGGL(k_cc_keys, dim3(g), dim3(kWG), 0, stream_, root, size, keys, count, N);
         HIP_OK(hipGetLastError());
         uint32_t c = 0;
@@ -1185,6 +1208,11 @@ private:
     uint32_t gridFor(uint32_t n) const {
         const uint32_t need = (n + kWG - 1) / kWG;
         return std::max(1U, std::min(need, cus_ * 16U));
+    }
+    // Two workgroups per CU: for launches that end in one atomic per workgroup.
+    uint32_t gridFew(uint32_t n) const {
+        const uint32_t need = (n + kWG - 1) / kWG;
+        return std::max(1U, std::min(need, cus_ * 2U));
     }
 
     uint32_t countRunning() const {
@@ -1435,11 +1463,12 @@ private:
                 profEnd();
                 profAddLaunches(KSolve, uint32_t(hi - lo) - 1U);
             }
-            if (S.plane_count > 0 && it + 1 < iterations)
+            if ((S.plane_count > 0 || S.blade.enabled) && it + 1 < iterations)
                 BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
         profBegin(KVelocity, 8);
-        if (S.plane_count > 0) hipLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nodes, S);
+        if (S.plane_count > 0 || S.blade.enabled)
+            hipLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nodes, S);
         else hipLaunchKernelGGL((k_velocity<R, false>), ng, dim3(kWG), 0, stream_, nodes, S);
         profEnd();
         if (S.damping > R(0)) {
@@ -1596,6 +1625,12 @@ int blx_backend_upload(blx_backend *b, const blx_state_view *st, const blx_setti
         dispatch(H(b), [&](auto &e) { e.upload(0, *st, *s, *sp); });
     });
 }
+int blx_backend_set_blade(blx_backend *b, const blx_blade *blade) {
+    return guard([&] {
+        if (!blade) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) { e.setBlade(*blade); });
+    });
+}
 int blx_backend_set_external_forces(blx_backend *b, const double *f, uint64_t n) {
     return guard([&] { dispatch(H(b), [&](auto &e) { e.setForces(0, f, n); }); });
 }
@@ -1709,6 +1744,12 @@ int blx_batch_first_failure_bonds(const blx_batch *b, uint32_t slot, uint32_t *o
 }
 int blx_batch_fragment_labels(blx_batch *b, uint32_t slot, uint32_t *label, uint32_t *count) {
     return guard([&] { *count = dispatch(H(b), [&](auto &e) { return e.labels(slot, label); }); });
+}
+int blx_batch_set_blade(blx_batch *b, const blx_blade *blade) {
+    return guard([&] {
+        if (!blade) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) { e.setBlade(*blade); });
+    });
 }
 int blx_batch_release(blx_batch *b, uint32_t slot) {
     return guard([&] { dispatch(H(b), [&](auto &e) { e.release(slot); }); });

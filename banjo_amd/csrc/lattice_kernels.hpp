@@ -958,6 +958,12 @@ template <class R> __device__ __forceinline__ void projectNode(const DevLattice<
         moved = true;
         if (-dist > R(0)) L.engaged[size_t(p) * L.N + i] = 1;
     }
+    // The kinematic blade after the planes (an extension; see bladePush).
+    V3<R> d;
+    if (S.blade.enabled && bladePush(S.blade, S.dt, L.ctl->step, x0 + xyz(u), d)) {
+        u = with(xyz(u) + d, u.w);
+        moved = true;
+    }
 }
 
 template <class R> __global__ __launch_bounds__(kWG) void k_project(Batch<R> B, Step<R> S) {
@@ -1506,16 +1512,22 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
 // relabel of findConnectedComponents (ConnectedComponents.cpp:77-85). Roots are
 // linked larger index under smaller with a compare-and-swap, so every root
 // ends as its component's smallest node: the tie-break of the canonical order.
-// Parent reads are agent-scope atomic loads, never a stale per-CU cache line.
+//
+// Finds read parents through the caches. A parent only ever moves to a smaller
+// ancestor and trees only merge, so a stale parent is still an ancestor and a
+// find over cached values ends at an ancestor of its node, never past the
+// root. Only the link needs the current value of a root's entry, and the
+// compare-and-swap returns it: a failed link climbs from that value, so the
+// larger end strictly decreases and every link attempt ends.
 
-__device__ __forceinline__ u32 loadParent(u32 *parent, u32 x) {
-    return __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ u32 cachedParent(u32 *parent, u32 x) {
+    return __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
 __device__ __forceinline__ u32 findRoot(u32 *parent, u32 x) {
-    u32 p = loadParent(parent, x);
+    u32 p = cachedParent(parent, x);
     while (p != x) {
-        const u32 g = loadParent(parent, p);
+        const u32 g = cachedParent(parent, p);
         // Path halving: g is an ancestor of x, so the store only shortens the path.
         if (g != p) __hip_atomic_store(&parent[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         x = p;
@@ -1524,38 +1536,200 @@ __device__ __forceinline__ u32 findRoot(u32 *parent, u32 x) {
     return x;
 }
 
-__global__ void k_cc_init(u32 *parent, u32 *size, u32 n) {
-    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        parent[i] = i;
+// Joins the trees of a and b. A root is linked under any node of a smaller
+// index in the other tree, not only under its root: the merged tree's root is
+// then still its smallest node, and no cycle can form, since a root is the
+// smallest node of its own tree. Linking a root under the other end itself
+// saves the find of that end whenever it has the smaller index.
+__device__ __forceinline__ void unite(u32 *parent, u32 a, u32 b) {
+    a = findRoot(parent, a);
+    for (;;) {
+        if (a == b) return;
+        if (b < a) {
+            const u32 old = atomicCAS(&parent[a], a, b);
+            if (old == a) return;
+            a = findRoot(parent, old);
+            continue;
+        }
+        const u32 rb = findRoot(parent, b);
+        if (rb == a) return;
+        if (rb < a) {
+            b = rb;
+            continue;
+        }
+        const u32 old = atomicCAS(&parent[rb], rb, a);
+        if (old == rb) return;
+        b = old;
+    }
+}
+
+// Linking is two-level. A workgroup owns a range of kCcRange consecutive
+// nodes and joins the live bonds inside it with a union-find in LDS; every
+// node's parent becomes its range-local root. A second pass reads the bonds
+// that leave the range (from their smaller end), reduces them to the distinct
+// pairs of local roots in an LDS hash set, and links only those pairs in
+// global memory. Trees keep the smallest node as root at both levels.
+constexpr u32 kCcRange = 1024;  // nodes per range: one per thread of a 1024-lane workgroup
+constexpr u32 kCcPairs = 1024;
+constexpr u32 kCcBatch = 8;     // slot keys loaded together before they are used
+constexpr u64 kCcEmpty = ~0ULL;
+
+__device__ __forceinline__ u32 ldsParent(u32 *lp, u32 x) {
+    return __hip_atomic_load(&lp[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ u32 ldsFind(u32 *lp, u32 x) {
+    u32 p = ldsParent(lp, x);
+    while (p != x) {
+        const u32 g = ldsParent(lp, p);
+        if (g != p) __hip_atomic_store(&lp[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        x = p;
+        p = g;
+    }
+    return x;
+}
+// The same linking rule as unite(), on range-local indices in LDS.
+__device__ __forceinline__ void ldsUnite(u32 *lp, u32 a, u32 b) {
+    a = ldsFind(lp, a);
+    for (;;) {
+        if (a == b) return;
+        if (b < a) {
+            u32 expect = a;
+            if (__hip_atomic_compare_exchange_strong(&lp[a], &expect, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
+                return;
+            a = ldsFind(lp, expect);
+            continue;
+        }
+        const u32 rb = ldsFind(lp, b);
+        if (rb == a) return;
+        if (rb < a) {
+            b = rb;
+            continue;
+        }
+        u32 expect = rb;
+        if (__hip_atomic_compare_exchange_strong(&lp[rb], &expect, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
+        b = expect;
+    }
+}
+
+// Range-local union-find: parent[i] = the smallest node of i's component
+// within its range; sizes are cleared for the final count.
+template <class R> __global__ __launch_bounds__(kCcRange) void k_cc_local(DevLattice<R> L, u32 *parent, u32 *size) {
+    __shared__ u32 lp[kCcRange];
+    const u32 base = blockIdx.x * kCcRange, k = threadIdx.x;
+    const u32 n = min(kCcRange, L.N - base), N = L.N, i = base + k;
+    const bool in = k < n;
+    lp[k] = k;
+    __syncthreads();
+    for (u32 q0 = 0; q0 < L.D; q0 += kCcBatch) {
+        u32 key[kCcBatch];
+#pragma unroll
+        for (u32 q = 0; q < kCcBatch; ++q) key[q] = in && q0 + q < L.D ? L.nbr_key[size_t(q0 + q) * N + i] : kNone;
+#pragma unroll
+        for (u32 q = 0; q < kCcBatch; ++q) {
+            if (key[q] & kDeadSlot) continue;  // dead, absent, or out of range
+            const u32 j = u32(int(i) + L.sdelta[key[q]]);
+            if (j > i && j < base + n) ldsUnite(lp, k, j - base);
+        }
+    }
+    __syncthreads();
+    if (in) {
+        parent[i] = base + ldsFind(lp, k);
         size[i] = 0;
     }
 }
 
-template <class R> __global__ void k_cc_hook(DevLattice<R> L, u32 *parent) {
-    for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < L.B; j += gridDim.x * blockDim.x) {
-        if (!L.alive[j]) continue;
-        const BondRec<R> r = L.rec[j];
-        u32 a = r.a, b = r.b;
-        while (true) {
-            a = findRoot(parent, a);
-            b = findRoot(parent, b);
-            if (a == b) break;
-            const u32 hi = a > b ? a : b, lo = a > b ? b : a;
-            if (atomicCAS(&parent[hi], hi, lo) == hi) break;
-            a = hi;
-            b = lo;
+// Bonds that leave a range, from their smaller end: the distinct pairs of
+// local roots are gathered in LDS, then each is linked once. A pair that finds
+// the hash set full is linked at once instead.
+template <class R> __global__ __launch_bounds__(kCcRange) void k_cc_cross(DevLattice<R> L, u32 *parent) {
+    __shared__ u64 pairs[kCcPairs];
+    const u32 base = blockIdx.x * kCcRange, k = threadIdx.x;
+    const u32 n = min(kCcRange, L.N - base), N = L.N, i = base + k;
+    const bool in = k < n;
+    for (u32 e = k; e < kCcPairs; e += kCcRange) pairs[e] = kCcEmpty;
+    __syncthreads();
+    const u32 ri = in ? cachedParent(parent, i) : 0U;
+    u64 last = kCcEmpty;
+    for (u32 q0 = 0; q0 < L.D; q0 += kCcBatch) {
+        u32 other[kCcBatch];
+#pragma unroll
+        for (u32 q = 0; q < kCcBatch; ++q) {
+            const u32 key = in && q0 + q < L.D ? L.nbr_key[size_t(q0 + q) * N + i] : kNone;
+            // Inside the range, or the bond's smaller end is the other node: not this pass's bond.
+            const bool live = !(key & kDeadSlot);
+            const u32 j = live ? u32(int(i) + L.sdelta[key]) : 0U;
+            other[q] = live && j >= base + n ? j : kNone;
         }
+#pragma unroll
+        for (u32 q = 0; q < kCcBatch; ++q) other[q] = other[q] != kNone ? cachedParent(parent, other[q]) : kNone;
+#pragma unroll
+        for (u32 q = 0; q < kCcBatch; ++q) {
+            if (other[q] == kNone) continue;
+            const u64 pr = (u64(ri) << 32) | u64(other[q]);
+            if (pr == last) continue;
+            last = pr;
+            u32 h = (ri * 0x9E3779B1U ^ other[q] * 0x85EBCA77U) & (kCcPairs - 1);
+            bool placed = false;
+            for (u32 probe = 0; probe < 32 && !placed; ++probe, h = (h + 1) & (kCcPairs - 1)) {
+                const u64 seen = atomicCAS(reinterpret_cast<unsigned long long *>(&pairs[h]), kCcEmpty, pr);
+                placed = seen == kCcEmpty || seen == pr;
+            }
+            if (!placed) unite(parent, ri, other[q]);
+        }
+    }
+    __syncthreads();
+    for (u32 e = k; e < kCcPairs; e += kCcRange) {
+        const u64 pr = pairs[e];
+        if (pr != kCcEmpty) unite(parent, u32(pr >> 32), u32(pr));
     }
 }
 
-// Roots go to their own array: a root stored into parent[] could be overwritten
-// by another thread's path halving of the same node.
-__global__ void k_cc_roots(u32 *parent, u32 *root, u32 *size, u32 n) {
-    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const u32 r = findRoot(parent, i);
-        root[i] = r;
-        atomicAdd(&size[r], 1U);
+// Flattens every tree to depth one (into parent[], or into root[] for the
+// final labels) and, with COUNT, counts the component sizes. No link runs in
+// this launch, so the finds end at the true roots. A wave counts its lanes
+// once per distinct root, and the counts of the root of the workgroup's first
+// node gather in LDS and land with one atomic per workgroup: the largest
+// component is the root of nearly every node, and per-wave atomics on its one
+// counter would serialize.
+template <bool COUNT>
+__global__ __launch_bounds__(kWG) void k_cc_flatten(u32 *parent, u32 *out, u32 *size, u32 n) {
+    __shared__ u32 s_common, s_sum;
+    const u32 stride = gridDim.x * kWG;
+    u32 common = kNone, mine = 0;
+    for (u32 base = blockIdx.x * kWG; base < n; base += stride) {
+        const u32 i = base + threadIdx.x;
+        u32 r = kNone;
+        if (i < n) {
+            r = findRoot(parent, i);
+            __hip_atomic_store(&out[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (!COUNT) continue;
+        if (common == kNone) {
+            if (threadIdx.x == 0) {
+                s_common = r;
+                s_sum = 0;
+            }
+            __syncthreads();
+            common = s_common;
+        }
+        mine += r == common;
+        u64 pending = __ballot(r != common && r != kNone);
+        while (pending) {
+            const u32 leader = u32(__ffsll((long long)pending) - 1);
+            const u32 lr = u32(__builtin_amdgcn_readlane(int(r), int(leader)));
+            const u64 same = __ballot(r == lr) & pending;
+            if ((threadIdx.x & 63U) == leader) atomicAdd(&size[lr], u32(__popcll(same)));
+            pending &= ~same;
+        }
     }
+    if (!COUNT || common == kNone) return;
+    for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o);
+    if ((threadIdx.x & 63U) == 0) atomicAdd(&s_sum, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(&size[common], s_sum);
 }
 
 // One sort key per component: (n - size) << 32 | root. Ascending keys are the

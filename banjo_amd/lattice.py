@@ -293,6 +293,9 @@ class Backend:
         sv = state.view()
         self.L.check(self.L.fn("backend_upload")(self.h, C.byref(sv), C.byref(settings), C.byref(sphere)), "upload")
 
+    def set_blade(self, blade):
+        self.L.check(self.L.fn("backend_set_blade")(self.h, C.byref(blade)), "set_blade")
+
     def set_external_forces(self, forces, substeps):
         if forces is None:
             self.L.check(self.L.fn("backend_set_external_forces")(self.h, None, C.c_uint64(substeps)),
@@ -394,6 +397,9 @@ class Batch:
         self.L.check(self.L.fn("batch_set_external_forces")(self.h, C.c_uint32(slot),
                                                             None if f is None else _ptr(f, C.c_double),
                                                             C.c_uint64(substeps)), "batch_set_external_forces")
+
+    def set_blade(self, blade):
+        self.L.check(self.L.fn("batch_set_blade")(self.h, C.byref(blade)), "batch_set_blade")
 
     def arm(self, slot, control: abi.RunControl):
         self.L.check(self.L.fn("batch_arm")(self.h, C.c_uint32(slot), C.byref(control)), "batch_arm")

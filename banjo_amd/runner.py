@@ -38,6 +38,8 @@ def run_scene(lib: LatticeLibrary, scene, precision=abi.PRECISION_FLOAT, substep
     sched, state = prepared if prepared is not None else prepare(lib, scene)
     be = lib.backend(precision, sched, *backend_args)
     be.upload(state, scene.settings, scene.sphere)
+    if scene.info.get("blade") is not None:
+        be.set_blade(scene.info["blade"])
     if scene.forces is not None:
         be.set_external_forces(scene.forces, scene.force_substeps)
     total = scene.substeps if substeps is None else substeps
@@ -103,6 +105,8 @@ def run_batch(lib: LatticeLibrary, scenes_, precision=abi.PRECISION_FLOAT, subst
         for k, sc in enumerate(scenes_):
             sched, state = prepare(lib, sc)
             bt.upload(k, sched, state, sc.settings, sc.sphere)
+            if sc.info.get("blade") is not None:
+                bt.set_blade(sc.info["blade"])
             if sc.forces is not None:
                 bt.set_external_forces(k, sc.forces, sc.force_substeps)
             n = sc.substeps if substeps is None else (substeps[k] if hasattr(substeps, "__len__") else substeps)

@@ -369,3 +369,49 @@ def served_stream(seed, rate, seconds, templates):
             return out
         out.append((t, int(rng.integers(templates)), float(rng.uniform(0.5, 3.0)), float(rng.uniform(0.0, 0.3)),
                     float(rng.uniform(0.0, 2.0 * math.pi))))
+
+
+def blade(edge_point, speed, half_width=0.0005, node_radius=0.005, flat_normal=(1.0, 0.0, 0.0),
+          direction=(0.0, -1.0, 0.0)):
+    """A kinematic blade (blx_blade): 1 mm thick by default, moving down."""
+    b = abi.Blade()
+    b.enabled = 1
+    b.edge_point[:] = edge_point
+    b.flat_normal[:] = flat_normal
+    b.direction[:] = direction
+    b.speed = speed
+    b.half_width = half_width
+    b.node_radius = node_radius
+    return b
+
+
+def plank_cut(nx, ny, nz, cell=0.01, node_mass=1.0e-3, speed=2.0, dt=1.0 / 2880.0, substeps=12 * 600, seed=971,
+              yield_stretch=0.004, hardening=0.1, weibull_k=4.0, weibull_scale=0.02, ledge_cells=16,
+              name="plank_cut"):
+    """A plank on two ledges under its ends, cut at mid-span by a 1 mm blade
+    moving down at `speed`: plastic rest-length update (yield, hardening)
+    before breaking at Weibull(k, scale) tensile strain; gravity; dt 1/2880 s
+    (12 substeps of a 1/240 s step)."""
+    pos, a, b, L = box26(nx, ny, nz, cell)
+    N = pos.shape[0]
+    c = log_uniform_compliance(seed)(a, b)
+    e = weibull_strain(seed + 1, weibull_k, weibull_scale)(a, b)
+    asset = Asset(reference_position=pos, mass=np.full(N, node_mass), bond_a=a, bond_b=b, rest_length=L,
+                  compliance=c, threshold=_thresholds(e, e))
+    bottom = pos[:, 1].min() - 0.5 * cell
+    top = pos[:, 1].max() + 0.5 * cell
+    half_x = 0.5 * nx * cell
+    span = ledge_cells * cell
+    # Two ledge footprints under the plank's ends (tangent x, bitangent z).
+    feet = [(-half_x + 0.5 * span, 0.0, 0.5 * span, INF), (half_x - 0.5 * span, 0.0, 0.5 * span, INF)]
+    st = settings(dt, gravity=(0.0, -9.81, 0.0), planes=[ledge_plane(bottom, 0.5 * cell, feet)], cell=cell,
+                  yield_stretch=yield_stretch, hardening=hardening)
+    sc = Scene(name=name, dims=(nx, ny, nz), cell=cell, asset=asset, slabs=box_slabs(nx, ny, nz, 1, 1),
+               settings=st, sphere=sphere_state(center=(0.0, 100.0, 0.0)), substeps=substeps)
+    sc.info = dict(blade=blade((0.0, top + 0.002, 0.0), speed, node_radius=0.5 * cell))
+    return sc
+
+
+def config5(substeps=12 * 600):
+    """256 x 32 x 32 plank, plastic (0.4%, H 0.1), Weibull(4, 2%), blade at 2 m/s."""
+    return plank_cut(256, 32, 32, substeps=substeps, name="c5_plank_cut_256x32x32")

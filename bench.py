@@ -21,7 +21,13 @@ Printed JSON carries, besides the rate:
                 MI355X's 8 TB/s HBM peak;
   step_roofline the whole substep at the algorithmic bytes per bond-solve;
   cpu_baseline  the reference's threaded CPU lane (oracle/_ref) on this host's
-                cores over a bounded window of the same workload.
+                cores over a bounded window of the same workload;
+  batched       512 cantilevers of uneven sizes in one batch against the plain
+                backend on a sample of the same trials (trials/s, speed-up);
+  served        a Poisson stream of marble trials admitted into a running
+                batch (trials/s, p50/p99 latency);
+  cut           a 256 x 32 x 32 plastic plank cut by a kinematic blade, with
+                fragment labels after every 12-substep step.
 """
 from __future__ import annotations
 
@@ -285,6 +291,60 @@ def copy_probe(nbytes, repeats):
             "gbps_read_plus_write": 2 * nbytes / secs.value / 1e9}
 
 
+def cut_leg(args):
+    """The cut form: a 256 x 32 x 32 plank on two ledges, plastic, cut at
+    mid-span by a 1 mm kinematic blade moving down at 2 m/s. Every step is 12
+    substeps followed by fragment labels computed on the device."""
+    from banjo_amd import abi, scenes
+    from banjo_amd.lattice import control, product
+    from banjo_amd.runner import prepare
+
+    hip = product()
+    steps, per = args.cut_steps, 12
+    t0 = time.perf_counter()
+    sc = scenes.config5(substeps=steps * per)
+    sched, state = prepare(hip, sc)
+    t_build = time.perf_counter() - t0
+    be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+    be.upload(state, sc.settings, sc.sphere)
+    be.set_blade(sc.info["blade"])
+    t_sim = t_label = 0.0
+    fragments = []
+    st = None
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        st = be.run(control(per))
+        t1 = time.perf_counter()
+        _, count = be.fragment_labels()
+        t_label += time.perf_counter() - t1
+        t_sim += t1 - t0
+        fragments.append(count)
+    # The last step's labels against the host library's union-find over the same bonds.
+    import numpy as np
+    labels, _ = be.fragment_labels()
+    out = state.copy()
+    be.download(out)
+    alive = np.empty(sc.bond_count, np.uint8)
+    alive[sched.bond_order] = out.alive
+    host, _ = hip.components(sc.node_count, sc.asset.bond_a, sc.asset.bond_b, alive)
+    be.close()
+    B = sc.bond_count
+    total = t_sim + t_label
+    return {"nodes": sc.node_count, "bonds": B, "steps": steps, "substeps": st.total_steps,
+            "seconds": total, "simulate_s": t_sim, "label_s": t_label,
+            "label_ms_per_step": t_label / steps * 1e3,
+            "bond_solves_per_s": B * st.total_steps / total,
+            "bond_solves_per_s_without_labels": B * st.total_steps / t_sim,
+            "broken_bonds": st.broken_bonds, "broken_fraction": st.broken_bonds / B,
+            "plastic_work_j": st.plastic_work_j, "fragments_final": fragments[-1],
+            "fragments_max": max(fragments), "labels_match_host": bool(np.array_equal(labels, host)),
+            "host_build_s": t_build,
+            "workload": "plank 256x32x32, 26-neighbour, cell 1 cm, on two 16-cell ledges, gravity; plastic rest "
+                        "length (yield 0.4%, hardening 0.1), Weibull(4, 2%) break strain; 1 mm blade at 2 m/s "
+                        "down through mid-span; dt 1/2880 s, 12 substeps per step, device fragment labels after "
+                        "every step"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,9 +354,10 @@ def main():
     ap.add_argument("--profile-substeps", type=int, default=1200)
     ap.add_argument("--cpu-substeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain,batched,served",
+    ap.add_argument("--forms", default="plain,batched,served,cut",
                     help="comma list of plain (the headline impact), batched (many uneven lattices), "
-                         "served (a stream of trials)")
+                         "served (a stream of trials), cut (a plank cut by a blade, labels every step)")
+    ap.add_argument("--cut-steps", type=int, default=600)
     ap.add_argument("--batch-trials", type=int, default=512)
     ap.add_argument("--batch-substeps", type=int, default=8 * 240)
     ap.add_argument("--batch-steps-per-launch", type=int, default=0)
@@ -325,6 +386,8 @@ def main():
             out["batched"] = batched_leg(args)
         if "served" in forms:
             out["served"] = served_leg(args)
+        if "cut" in forms:
+            out["cut"] = cut_leg(args)
         print(json.dumps(out))
         return
     t_build = time.perf_counter()
@@ -375,6 +438,8 @@ def main():
         out["batched"] = batched_leg(args)
     if "served" in forms:
         out["served"] = served_leg(args)
+    if "cut" in forms:
+        out["cut"] = cut_leg(args)
     print(json.dumps(out))
 
 

@@ -320,6 +320,29 @@ int blx_backend_clear_frames(blx_backend *backend);
 int blx_backend_fragment_labels(blx_backend *backend, uint32_t *label,
                                 uint32_t *component_count);
 
+/* ---- kinematic blade (no counterpart in the reference's lattice API) ------
+ *
+ * A thin blade moving at constant velocity through the lattice, for cutting
+ * workloads. The reference's blade lives in its rigid-body world
+ * (docs/cutting-model.md) and has no lattice-lane form, so this is an
+ * extension of the support planes (LatticePhysics.hpp:1190-1276): at the same
+ * point of the substep (after each constraint iteration, before the velocity
+ * update) every node whose contact sphere overlaps the blade is moved out of
+ * it by the shortest way, across a flat or ahead of the edge. The edge at
+ * substep k is at edge_point + direction * speed * dt * (k + 1). */
+typedef struct blx_blade {
+    uint8_t enabled;
+    double edge_point[3];   /* a point on the edge at time 0 */
+    double flat_normal[3];  /* unit normal of the blade's flats */
+    double direction[3];    /* unit direction of travel, perpendicular to flat_normal */
+    double speed;           /* m/s along direction */
+    double half_width;      /* half the thickness across the flats, m */
+    double node_radius;     /* contact radius of a node against the blade, m */
+} blx_blade;
+
+/* Sets or clears (enabled = 0) the blade of a backend between runs. */
+int blx_backend_set_blade(blx_backend *backend, const blx_blade *blade);
+
 /* ---- batches of lattices (no counterpart in the reference's API) ---------
  *
  * A batch advances many independent lattices of any sizes together: each
@@ -365,6 +388,8 @@ int blx_batch_sphere(blx_batch *batch, uint32_t slot, blx_sphere *sphere);
 /* LatticeBackend::firstFailureBonds for one slot (see blx_backend_first_failure_bonds). */
 int blx_batch_first_failure_bonds(const blx_batch *batch, uint32_t slot, uint32_t *out, uint64_t capacity,
                                   uint64_t *count);
+/* The blade of every slot of the batch (shared, like the step settings). */
+int blx_batch_set_blade(blx_batch *batch, const blx_blade *blade);
 /* The slot leaves the batch; its device memory is kept for reuse. */
 int blx_batch_release(blx_batch *batch, uint32_t slot);
 /* Device bytes the slot holds. */

@@ -253,6 +253,10 @@ template <class R> struct Settings {
     u32 plane_count; Plane<R> planes[BLX_MAX_SUPPORT_PLANES];
 };
 template <class R> struct Sphere { V3<R> center, velocity, omega; R radius, mass, inertia; };
+// The kinematic blade of the HIP library's extension (include/banjo_lattice.h,
+// blx_blade): no reference counterpart, restated here so that its use is
+// checked against a serial lane.
+template <class R> struct Blade { bool enabled; V3<R> c0, f, m; R speed, hw, r; };
 
 template <class R> Settings<R> convertSettings(const blx_settings &s) {
     Settings<R> o{};
@@ -323,6 +327,20 @@ public:
     }
 
     // ExternalLoads.hpp:28-50: validate everything, then replace.
+    void setBlade(const blx_blade &b) {
+        Blade<R> o{};
+        o.enabled = b.enabled != 0;
+        if (o.enabled) {
+            o.c0 = cv<R>(b.edge_point);
+            o.f = cv<R>(b.flat_normal);
+            o.m = cv<R>(b.direction);
+            o.speed = R(b.speed);
+            o.hw = R(b.half_width);
+            o.r = R(b.node_radius);
+        }
+        blade = o;
+    }
+
     void setForces(const double *f, u64 substeps) {
         if (!uploaded) throw logic("external forces before upload");
         if (f == nullptr) {
@@ -411,6 +429,7 @@ private:
     std::vector<R> damage, lambda, plastic_ext, plastic_strain, prev_t, prev_c, prev_s;
     u32 rank_deficient{};
     Settings<R> S{};
+    Blade<R> blade{};
     Sphere<R> sphere{};
     double origin[3]{};
     std::vector<Load> loads;
@@ -767,6 +786,21 @@ private:
     void nodeSupportProject(u32 i) {
         for (u32 p = 0; p < S.plane_count; ++p)
             if (project(S.planes[p], i, approach[std::size_t(p) * N + i]) > R(0)) engaged[std::size_t(p) * N + i] = 1;
+        if (blade.enabled) projectBlade(i);
+    }
+    // The shortest move of a node out of the blade, across a flat or ahead of
+    // the edge; the edge at substep k is at c0 + m * speed * dt * (k + 1).
+    void projectBlade(u32 i) {
+        const Blade<R> &bl = blade;
+        const R travel = bl.speed * (S.dt * R(status.total_steps + 1));
+        const V3<R> c = bl.c0 + travel * bl.m;
+        const V3<R> rel = position(i) - c;
+        const R a = dot(rel, bl.f), e = dot(rel, bl.m);
+        const R reach = bl.hw + bl.r;
+        if (!(absR(a) < reach) || !(e < bl.r)) return;
+        const R flat = reach - absR(a), edge = bl.r - e;
+        const V3<R> d = edge < flat ? edge * bl.m : (a < R(0) ? -flat : flat) * bl.f;
+        st3(u, i, ld3(u, i) + d);
     }
     void nodeSupportVelocity(u32 i) {
         for (u32 p = 0; p < S.plane_count; ++p)
@@ -1226,6 +1260,9 @@ int orc_backend_destroy(void *h) { return guard([&] { delete static_cast<Handle 
 
 int orc_backend_upload(void *h, const blx_state_view *st, const blx_settings *s, const blx_sphere *sp) {
     return guard([&] { ORC_DISPATCH(h, b.upload(*st, *s, *sp)); });
+}
+int orc_backend_set_blade(void *h, const blx_blade *bl) {
+    return guard([&] { ORC_DISPATCH(h, b.setBlade(*bl)); });
 }
 int orc_backend_set_external_forces(void *h, const double *f, uint64_t substeps) {
     return guard([&] { ORC_DISPATCH(h, b.setForces(f, substeps)); });

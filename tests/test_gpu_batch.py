@@ -124,3 +124,14 @@ def test_batch_in_double(gpu):
 def _control(n):
     from banjo_amd.lattice import control
     return control(n)
+
+
+def test_batch_blade_cuts_equal_alone(gpu):
+    # Same length and height (so the same ledges and blade), different widths and seeds.
+    scs = [scenes.plank_cut(20, 6, nz, speed=4.0, substeps=90, ledge_cells=4, seed=s, name=f"plank_20x6x{nz}")
+           for nz, s in [(4, 31), (6, 32), (9, 33)]]
+    batched = run_batch(gpu, scs)
+    for sc, b in zip(scs, batched):
+        a = alone(gpu, sc)
+        assert a["status"].broken_bonds > 0
+        assert_same(a, b)

@@ -35,6 +35,9 @@ FEATURES = {
     "plastic_double_4slabs": (lambda: scenes.plastic_impact(blocks=4, substeps=80), D),
     "bridge_ledges_float": (lambda: scenes.bridge_impact(), F),
     "bridge_ledges_3slabs_double": (lambda: scenes.bridge_impact(blocks=3, substeps=100), D),
+    # plank on two ledges, plastic, cut by a kinematic blade (blx_blade)
+    "blade_cut_float": (lambda: scenes.plank_cut(24, 6, 6, speed=4.0, substeps=120, ledge_cells=4), F),
+    "blade_cut_double": (lambda: scenes.plank_cut(20, 6, 5, speed=4.0, substeps=90, ledge_cells=4), D),
 }
 
 
