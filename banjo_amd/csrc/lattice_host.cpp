@@ -279,6 +279,53 @@ std::uint32_t componentsHost(std::uint32_t N, std::uint32_t B, const std::uint32
     return static_cast<std::uint32_t>(roots.size());
 }
 
+// The fastest nodal frequency bounds the stable substep at 2 / omega
+// (FastLattice.cpp:30-45); stiffnesses sum in adjacency order.
+double substepLimit(const blx_state_view &s) {
+    if (!s.adj_offsets) return 0.0;
+    double omega = 0.0;
+    for (std::uint32_t i = 0; i < s.node_count; ++i) {
+        const double m = s.mass[i];
+        if (!(m > 0.0)) continue;
+        double k = 0.0;
+        for (std::uint32_t q = s.adj_offsets[i]; q < s.adj_offsets[i + 1]; ++q) {
+            const std::uint32_t j = s.adj_bonds[q];
+            if (j >= s.bond_count) throw Invalid("adjacency names a bond out of range");
+            if (s.alive[j] && s.compliance[j] > 0.0) k += 1.0 / s.compliance[j];
+        }
+        if (k > 0.0) omega = std::max(omega, std::sqrt(k / m));
+    }
+    return omega > 0.0 ? 2.0 / omega : 0.0;
+}
+
+// Stored elastic energy of the live bonds in state order (FastLattice.cpp:199-214):
+// the bond vector is formed as ((x0_b + u_b) - x0_a) - u_a per axis.
+double elasticEnergy(const blx_state_view &s) {
+    double sum = 0.0;
+    for (std::uint32_t j = 0; j < s.bond_count; ++j) {
+        const double c = s.compliance[j];
+        if (!s.alive[j] || !(c > 0.0)) continue;
+        const std::uint32_t a = s.bond_a[j], b = s.bond_b[j];
+        if (a >= s.node_count || b >= s.node_count) throw Invalid("bond end out of range");
+        double d[3];
+        for (int k = 0; k < 3; ++k) d[k] = s.x0[3 * b + k] + s.u[3 * b + k] - s.x0[3 * a + k] - s.u[3 * a + k];
+        const double stretch = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - s.rest_length[j] -
+                               s.plastic_extension[j];
+        sum += 0.5 * stretch * stretch / c;
+    }
+    return sum;
+}
+
+// Kinetic energy in node order (FastLattice.cpp:216-224).
+double kineticEnergy(const blx_state_view &s) {
+    double sum = 0.0;
+    for (std::uint32_t i = 0; i < s.node_count; ++i) {
+        const double *v = s.v + 3 * size_t(i);
+        sum += 0.5 * s.mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    return sum;
+}
+
 } // namespace blx
 
 extern "C" {
@@ -318,6 +365,27 @@ int blx_box_slabs(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t horizon, uint3
 int blx_components_host(uint32_t N, uint32_t B, const uint32_t *ba, const uint32_t *bb, const uint8_t *alive,
                         uint32_t *label, uint32_t *count) {
     return blx::guard([&] { *count = blx::componentsHost(N, B, ba, bb, alive, label); });
+}
+
+int blx_state_substep_limit(const blx_state_view *state, double *seconds) {
+    return blx::guard([&] {
+        if (!state || !seconds) throw blx::Invalid("null argument");
+        *seconds = blx::substepLimit(*state);
+    });
+}
+
+int blx_state_elastic_energy(const blx_state_view *state, double *joules) {
+    return blx::guard([&] {
+        if (!state || !joules) throw blx::Invalid("null argument");
+        *joules = blx::elasticEnergy(*state);
+    });
+}
+
+int blx_state_kinetic_energy(const blx_state_view *state, double *joules) {
+    return blx::guard([&] {
+        if (!state || !joules) throw blx::Invalid("null argument");
+        *joules = blx::kineticEnergy(*state);
+    });
 }
 
 } // extern "C"

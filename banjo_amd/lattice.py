@@ -252,6 +252,18 @@ class LatticeLibrary:
                                  C.byref(count)), name)
         return label, count.value
 
+    def state_metrics(self, state: "State"):
+        """(substep limit s, elastic energy J, kinetic energy J) of a state:
+        latticeStateSubstepLimit / ElasticEnergy / KineticEnergy
+        (src/fastlattice/FastLattice.cpp:30-45,199-224)."""
+        sv = state.view()
+        out = []
+        for name in ("state_substep_limit", "state_elastic_energy", "state_kinetic_energy"):
+            x = C.c_double()
+            self.check(self.fn(name)(C.byref(sv), C.byref(x)), name)
+            out.append(x.value)
+        return tuple(out)
+
     def backend(self, precision, schedule: Schedule, *extra):
         return Backend(self, precision, schedule, *extra)
 

@@ -273,6 +273,16 @@ int blx_components_host(uint32_t node_count, uint32_t bond_count, const uint32_t
                         const uint32_t *bond_b, const uint8_t *alive, uint32_t *label,
                         uint32_t *component_count);
 
+/* latticeStateSubstepLimit (FastLattice.cpp:30-45): 2 / the fastest nodal
+ * frequency sqrt(sum of 1/compliance over a node's live bonds / mass), over
+ * nodes of positive mass; 0 when no node has a stiffness (or no adjacency). */
+int blx_state_substep_limit(const blx_state_view *state, double *seconds);
+/* latticeStateElasticEnergy (FastLattice.cpp:199-214): sum over live bonds of
+ * positive compliance of (|x_b - x_a| - L - plastic extension)^2 / (2 c). */
+int blx_state_elastic_energy(const blx_state_view *state, double *joules);
+/* latticeStateKineticEnergy (FastLattice.cpp:216-224): sum of m |v|^2 / 2. */
+int blx_state_kinetic_energy(const blx_state_view *state, double *joules);
+
 /* ---- device backend (LatticeBackend, FastLattice.hpp:255-369) ----------- */
 
 typedef struct blx_backend blx_backend;

@@ -11,6 +11,9 @@
 //                           (FastLattice.hpp:371-381) and LatticeBackend methods
 //   ref_components       -> findConnectedComponents (ConnectedComponents.cpp:51-87)
 //   ref_box_slabs        -> boxLatticeSlabs         (BoxLattice.cpp:175-195)
+//   ref_state_substep_limit, ref_state_elastic_energy, ref_state_kinetic_energy
+//                        -> latticeStateSubstepLimit / ElasticEnergy / KineticEnergy
+//                           (FastLattice.cpp:30-45,199-224)
 
 #include "../include/banjo_lattice.h"
 
@@ -269,6 +272,15 @@ int ref_box_slabs(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t horizon, uint3
     });
 }
 
+int ref_state_substep_limit(const blx_state_view *v, double *out) {
+    return guard([&] { *out = latticeStateSubstepLimit(stateFromView(*v)); });
+}
+int ref_state_elastic_energy(const blx_state_view *v, double *out) {
+    return guard([&] { *out = latticeStateElasticEnergy(stateFromView(*v)); });
+}
+int ref_state_kinetic_energy(const blx_state_view *v, double *out) {
+    return guard([&] { *out = latticeStateKineticEnergy(stateFromView(*v)); });
+}
 int ref_components(uint32_t N, uint32_t B, const uint32_t *ba, const uint32_t *bb, const uint8_t *alive, uint32_t *label, uint32_t *count) {
     return guard([&] {
         LatticeAsset asset;
