@@ -1250,9 +1250,11 @@ private:
         for (uint32_t k : run)
             for (uint32_t t = 0; t < slots_[k]->node_tiles; ++t) all.push_back(WgMap{k, t, 0, 0});
         end(node_tiles_map_);
+        // End-sample spans: up to kEndSpan consecutive bond tiles per workgroup.
         begin(bond_tiles_map_);
         for (uint32_t k : run)
-            for (uint32_t t = 0; t < slots_[k]->bond_tiles; ++t) all.push_back(WgMap{k, t, 0, 0});
+            for (uint32_t t = 0; t < slots_[k]->bond_tiles; t += kEndSpan)
+                all.push_back(WgMap{k, t, std::min(kEndSpan, slots_[k]->bond_tiles - t), 0});
         end(bond_tiles_map_);
         begin(list_map_);
         for (uint32_t k : run) {

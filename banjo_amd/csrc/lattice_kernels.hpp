@@ -33,6 +33,7 @@
 namespace blx {
 
 constexpr int kWG = 256;
+constexpr u32 kEndSpan = 1;  // bond tiles per end-sample workgroup (2 and 4 measured no faster)
 constexpr u32 kCap = 1024;        // striker candidates per block (LatticePhysics.hpp:105)
 constexpr int kFinishThreads = 1024;
 
@@ -172,7 +173,7 @@ template <class R> __device__ __forceinline__ const WgMap *tileEntry(const Batch
 // Nodes.
 
 template <class R, bool DIRECT> __device__ __forceinline__ V3<R> nodePos(const DevLattice<R> &L, u32 i, const R4<R> &u) {
-    if (DIRECT) return xyz(L.x04[i]) + xyz(u);
+    if (DIRECT) return xyz(ldg(&L.x04[i])) + xyz(u);
     return xyz(u);
 }
 
@@ -185,17 +186,24 @@ template <class R> __device__ void rinvNode(const DevLattice<R> &L, u32 i, u32 *
     const u32 N = L.N, D = L.D;
     R cov[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     u32 live = 0;
-    for (u32 k = 0; k < D; ++k) {
-        const u32 key = L.nbr_key[size_t(k) * N + i];
-        if (key == kNone) break;
-        if (key & kDeadSlot) continue;
-        const R4<R> rw = L.spal[key];
-        const R w = rw.w;
-        if (w <= R(0)) continue;
-        const R rr[3] = {rw.x, rw.y, rw.z};
-        for (int r = 0; r < 3; ++r)
-            for (int q = 0; q < 3; ++q) cov[r * 3 + q] += w * rr[r] * rr[q];
-        ++live;
+    // Slots in chunks of 8: the chunk's keys, then its palette entries, are
+    // loaded before the sums, which stay in slot order.
+    constexpr u32 kChunk = 8;
+    for (u32 k0 = 0; k0 < D; k0 += kChunk) {
+        u32 key[kChunk];
+        for (u32 q = 0; q < kChunk; ++q) key[q] = k0 + q < D ? L.nbr_key[size_t(k0 + q) * N + i] : kNone;
+        R4<R> rw[kChunk];
+        for (u32 q = 0; q < kChunk; ++q) rw[q] = key[q] & kDeadSlot ? R4<R>{R(0), R(0), R(0), R(0)} : L.spal[key[q]];
+        for (u32 q = 0; q < kChunk; ++q) {
+            if (key[q] & kDeadSlot) continue;  // dead or absent
+            const R w = rw[q].w;
+            if (w <= R(0)) continue;
+            const R rr[3] = {rw[q].x, rw[q].y, rw[q].z};
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) cov[r * 3 + c] += w * rr[r] * rr[c];
+            ++live;
+        }
+        if (key[kChunk - 1] == kNone) break;
     }
     u8 valid = 0;
     if (live >= 1) {
@@ -220,9 +228,9 @@ template <class R> __device__ void rinvNode(const DevLattice<R> &L, u32 i, u32 *
 // The nodal strain from the cached inverse (LatticePhysics.hpp:833-906).
 template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &L, u32 i) {
     const u32 N = L.N, D = L.D;
-    if (!L.valid[i]) return;
+    if (!ldg(&L.valid[i])) return;
     R acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const V3<R> xi = nodePos<R, DIRECT>(L, i, L.u4[i]);
+    const V3<R> xi = nodePos<R, DIRECT>(L, i, ldg(&L.u4[i]));
     // Slots in chunks of 8: every load of a chunk is issued before the sums,
     // which stay in slot order.
     constexpr u32 kChunk = 8;
@@ -233,11 +241,12 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
         u32 other[kChunk];
         for (u32 q = 0; q < kChunk; ++q) {
             const bool live = !(key[q] & kDeadSlot);
-            rw[q] = live ? L.spal[key[q]] : R4<R>{R(0), R(0), R(0), R(0)};
-            other[q] = live ? u32(int(i) + L.sdelta[key[q]]) : i;
+            rw[q] = live ? ldg(&L.spal[key[q]]) : R4<R>{R(0), R(0), R(0), R(0)};
+            other[q] = live ? u32(int(i) + ldg(&L.sdelta[key[q]])) : i;
         }
         R4<R> xo4[kChunk];
-        for (u32 q = 0; q < kChunk; ++q) xo4[q] = !(key[q] & kDeadSlot) ? L.u4[other[q]] : R4<R>{R(0), R(0), R(0), R(0)};
+        for (u32 q = 0; q < kChunk; ++q)
+            xo4[q] = !(key[q] & kDeadSlot) ? ldg(&L.u4[other[q]]) : R4<R>{R(0), R(0), R(0), R(0)};
         for (u32 q = 0; q < kChunk; ++q) {
             if (key[q] & kDeadSlot) continue;
             const R w = rw[q].w;
@@ -251,11 +260,11 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
         if (key[kChunk - 1] == kNone) break;
     }
     R inv[9], un[3], e[6];
-    for (int k = 0; k < 9; ++k) inv[k] = L.rinv[size_t(k) * N + i];
-    for (int k = 0; k < 3; ++k) un[k] = L.unm[size_t(k) * N + i];
+    for (int k = 0; k < 9; ++k) inv[k] = ldg(&L.rinv[size_t(k) * N + i]);
+    for (int k = 0; k < 3; ++k) un[k] = ldg(&L.unm[size_t(k) * N + i]);
     greenStrain(acc, inv, un, e);
-    L.sa[i] = R4<R>{e[0], e[1], e[2], e[3]};
-    L.sb[i] = R4<R>{e[4], e[5], R(1), R(0)};
+    stg(&L.sa[i], R4<R>{e[0], e[1], e[2], e[3]});
+    stg(&L.sb[i], R4<R>{e[4], e[5], R(1), R(0)});
 }
 
 // Which nodes a k_strain launch covers.
@@ -372,9 +381,9 @@ __device__ void sampleBond(const DevLattice<R> &L, u32 a, u32 b, u32 p, const Ed
         const V3<R> dir{g.d.y, g.d.z, g.d.w};
         const u32 ends[2] = {a, b};
         for (int q = 0; q < 2; ++q) {
-            const R4<R> sb = L.sb[ends[q]];
+            const R4<R> sb = ldg(&L.sb[ends[q]]);
             if (sb.z == R(0)) continue;
-            const R4<R> sa = L.sa[ends[q]];
+            const R4<R> sa = ldg(&L.sa[ends[q]]);
             R nn, sh;
             resolve(sa, sb, dir, nn, sh);
             t = maxR(t, nn);
@@ -880,28 +889,27 @@ __global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags)
     const WgMap *m = tileEntry(B);
     if (!m) return;
     const DevLattice<R> L = B.L[m->lat];
-    if (L.ctl->stop) return;
     const u32 j = m->a + threadIdx.x;
     if (j >= m->b) return;
     const bool first = flags & kSolveFirst;
-    // The record is loaded with the alive flag rather than after it, so the
-    // node gathers wait on one memory round trip, not two.
+    // The record is loaded with the alive flag and the run's stop flag, so the
+    // node gathers wait on one memory round trip, not two or three.
     const BondRec<R> r = ldStream(&L.rec[j]);
     const u8 live = ldStream(&L.alive[j]);
-    const R lam = first ? R(0) : L.lambda[j];
-    if (!live) return;
-    const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
-    const EdgePal<R> &g = L.epal[r.p & ~kNoDir];
+    const R lam = first ? R(0) : ldg(&L.lambda[j]);
+    if (L.ctl->stop || !live) return;
+    const R4<R> ua = ldg(&L.u4[r.a]), ub = ldg(&L.u4[r.b]);
+    const EdgePal<R> g = ldg(&L.epal[r.p & ~kNoDir]);
     const R4<R> ge = g.e;
     const V3<R> du = xyz(ub) - xyz(ua);
     const V3<R> e = xyz(ge);
-    const V3<R> delta = DIRECT ? (xyz(L.x04[r.b]) + xyz(ub)) - (xyz(L.x04[r.a]) + xyz(ua)) : e + du;
+    const V3<R> delta = DIRECT ? (xyz(ldg(&L.x04[r.b])) + xyz(ub)) - (xyz(ldg(&L.x04[r.a])) + xyz(ua)) : e + du;
     const R len = length(delta);
     if (len <= R(1.0e-12)) return;
     const R orig = ge.w;
     R C;
     if (PLASTIC) {
-        const R plastic = L.pext[j];
+        const R plastic = ldg(&L.pext[j]);
         const R rest = orig + plastic;
         C = DIRECT ? len - rest : (g.d.x + R(2) * dot(e, du) + dot(du, du) - plastic * (rest + orig)) / (len + rest);
     } else {
@@ -913,7 +921,7 @@ __global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags)
     const R alpha = r.c / (S.dt * S.dt);
     const R dl = (-C - alpha * lam) / (wa + wb + alpha);
     // The multiplier is read only by a later iteration of the same substep.
-    if (flags & kSolveKeepLambda) L.lambda[j] = lam + dl;
+    if (flags & kSolveKeepLambda) stg(&L.lambda[j], lam + dl);
     stStream(&L.u4[r.a], with(xyz(ua) - wa * dl * dir, ua.w));
     stStream(&L.u4[r.b], with(xyz(ub) + wb * dl * dir, ub.w));
 }
@@ -1037,18 +1045,28 @@ template <class R> __global__ __launch_bounds__(kWG) void k_support_velocity(Bat
 // 2 runs, those with neither end affected, and after it the rest.
 enum BondCover : int { kBondAll = 0, kBondUnaffected = 2, kBondAffected = 3 };
 
+// The streamed part of a bond's end sample: its record, previous peaks and
+// damage, and alive flag. A workgroup loads these for every bond of its span
+// before it starts on the first, so the span's first memory round trips overlap.
+template <class R> struct EndLoad {
+    BondNode r;
+    R4<R> pv;
+    u32 live;
+};
+template <class R> __device__ __forceinline__ EndLoad<R> loadEnd(const DevLattice<R> &L, u32 o) {
+    return {ldStream(&L.bnd[o]), ldStream(&L.pvd[o]), u32(ldStream(&L.alive_n[o]))};
+}
+
 // One bond's end sample, damage and break. Returns through the references the
 // bond's contributions to the running maxima, its plastic work and the nodes
 // it claims for the dirty list.
 template <class R, bool DIRECT, bool PLASTIC, int COVER>
-__device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step<R> &S, u32 o, double m0, u32 stamp,
-                                              float &mt, float &mc, float &ms, float &mp, double &pw, u32 &claim_a,
-                                              u32 &claim_b) {
-    // The bond's streamed data is loaded together with its alive flag, so the
-    // node gathers wait on one memory round trip.
-    const BondNode r = ldStream(&L.bnd[o]);
-    R4<R> pv = ldStream(&L.pvd[o]);
-    u8 live = ldStream(&L.alive_n[o]);
+__device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step<R> &S, u32 o, const EndLoad<R> &in,
+                                              double m0, u32 stamp, float &mt, float &mc, float &ms, float &mp,
+                                              double &pw, u32 &claim_a, u32 &claim_b) {
+    const BondNode r = in.r;
+    R4<R> pv = in.pv;
+    u32 live = in.live;
     Thresholds<R> th{};
     if (L.tensile_only) {
         const Tensile<R> t2 = ldStream(&L.thr2[o]);
@@ -1057,11 +1075,11 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
     } else {
         th = L.thr[o];
     }
-    if (COVER == kBondUnaffected && live && (L.affected[r.a] == stamp || L.affected[r.b] == stamp)) live = 0;
+    if (COVER == kBondUnaffected && live && (ldg(&L.affected[r.a]) == stamp || ldg(&L.affected[r.b]) == stamp)) live = 0;
     if (!live) return;
     const u32 j = r.j;
-    const R4<R> ua = L.u4[r.a], ub = L.u4[r.b];
-    const EdgePal<R> g = L.epal[r.p & ~kNoDir];
+    const R4<R> ua = ldg(&L.u4[r.a]), ub = ldg(&L.u4[r.b]);
+    const EdgePal<R> g = ldg(&L.epal[r.p & ~kNoDir]);
     const R restL = g.e.w;
     R plastic = R(0);
     if (PLASTIC) {
@@ -1140,6 +1158,7 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
 __device__ __forceinline__ void appendDirty(u32 *list, u32 *count, u32 claim_a, u32 claim_b) {
     const u32 lane = threadIdx.x & 63U;
     const u32 mine = (claim_a != kNone) + (claim_b != kNone);
+    if (!__ballot(mine != 0)) return;  // the common case: no break in this wave
     u32 x = mine;
     for (u32 q = 1; q < 64; q <<= 1) {
         const u32 y = __shfl_up(x, q);
@@ -1174,9 +1193,11 @@ __device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, f
         float m = 0.f;
         for (int w = 0; w < kWG / 64; ++w) m = fmaxf(m, s_max[threadIdx.x][w]);
         u32 *dst = threadIdx.x < 3 ? &ctl->peak_bits[threadIdx.x] : &ctl->pstretch_bits;
-        // The running maxima rarely move: read first, so tens of thousands of
-        // workgroups do not queue atomics on the same four words.
-        if (m > 0.f && __float_as_uint(m) > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        // The running maxima rarely move: read first, so thousands of
+        // workgroups do not queue atomics on the same four words. The read goes
+        // through the caches: a maximum only grows, so a stale value is never
+        // above the current one and never skips a needed update.
+        if (m > 0.f && __float_as_uint(m) > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT)) {
             atomicMax(dst, __float_as_uint(m));
         }
     }
@@ -1189,6 +1210,15 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
     if (!m) return;
     const DevLattice<R> L = B.L[m->lat];
     DevCtl *ctl = L.ctl;
+    // The span's streamed loads go out with the run state's, before the stop test.
+    EndLoad<R> in[kEndSpan];
+    if (COVER != kBondAffected) {
+#pragma unroll
+        for (u32 q = 0; q < kEndSpan; ++q) {
+            const u32 o = (m->a + q) * kWG + threadIdx.x;
+            in[q] = q < m->b && o < L.B ? loadEnd(L, o) : EndLoad<R>{};
+        }
+    }
     if (ctl->stop) return;
     __shared__ float s_max[4][kWG / 64];
     __shared__ double s_pw[kWG];
@@ -1213,7 +1243,8 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
                 if (o != kNone) {
                     const u32 a = L.bnd[o].a;
                     if (a == i || L.affected[a] != stamp)
-                        endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, m0, stamp, mt, mc, ms, mp, pw, claim_a, claim_b);
+                        endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, loadEnd(L, o), m0, stamp, mt, mc, ms, mp, pw,
+                                                                 claim_a, claim_b);
                 }
             }
             appendDirty(L.dirty_list, &ctl->dirty_count, claim_a, claim_b);
@@ -1221,12 +1252,14 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
         publishMaxima(ctl, mt, mc, ms, mp, s_max);
         return;
     }
-    const u32 tile = m->a;
-    {
-        const u32 o = tile * kWG + threadIdx.x;
+    // A span of up to kEndSpan consecutive tiles of kWG bonds (m->a, m->b).
+#pragma unroll
+    for (u32 q = 0; q < kEndSpan; ++q) {
+        if (q >= m->b) break;
+        const u32 tile = m->a + q, o = tile * kWG + threadIdx.x;
         u32 claim_a = kNone, claim_b = kNone;
         double pw = 0.0;
-        if (o < L.B) endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, m0, stamp, mt, mc, ms, mp, pw, claim_a, claim_b);
+        if (o < L.B) endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, in[q], m0, stamp, mt, mc, ms, mp, pw, claim_a, claim_b);
         appendDirty(L.dirty_list, &ctl->dirty_count, claim_a, claim_b);
         if (PLASTIC) {
             s_pw[threadIdx.x] = pw;
