@@ -228,15 +228,31 @@ template <class R> __device__ void rinvNode(const DevLattice<R> &L, u32 i, u32 *
 // The nodal strain from the cached inverse (LatticePhysics.hpp:833-906).
 template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &L, u32 i) {
     const u32 N = L.N, D = L.D;
-    if (!ldg(&L.valid[i])) return;
+#ifndef BLX_STRAIN_CHUNK
+#define BLX_STRAIN_CHUNK 8
+#endif
+#ifndef BLX_STRAIN_PREFETCH
+#define BLX_STRAIN_PREFETCH 1
+#endif
+    // Slots in chunks: every load of a chunk is issued before the sums, which
+    // stay in slot order, and the next chunk's keys are in flight meanwhile.
+    constexpr u32 kChunk = BLX_STRAIN_CHUNK;
+    // The node's flag, own displacement and first keys go out together.
+    const u8 valid = ldg(&L.valid[i]);
+    const R4<R> ui = ldg(&L.u4[i]);
+    u32 next[kChunk];
+    for (u32 q = 0; q < kChunk; ++q) next[q] = q < D ? ldStream(&L.nbr_key[size_t(q) * N + i]) : kNone;
+    if (!valid) return;
     R acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const V3<R> xi = nodePos<R, DIRECT>(L, i, ldg(&L.u4[i]));
-    // Slots in chunks of 8: every load of a chunk is issued before the sums,
-    // which stay in slot order.
-    constexpr u32 kChunk = 8;
+    const V3<R> xi = nodePos<R, DIRECT>(L, i, ui);
     for (u32 k0 = 0; k0 < D; k0 += kChunk) {
         u32 key[kChunk];
-        for (u32 q = 0; q < kChunk; ++q) key[q] = k0 + q < D ? ldStream(&L.nbr_key[size_t(k0 + q) * N + i]) : kNone;
+        for (u32 q = 0; q < kChunk; ++q) key[q] = next[q];
+        if (BLX_STRAIN_PREFETCH) {
+            if (key[kChunk - 1] != kNone)
+                for (u32 q = 0; q < kChunk; ++q)
+                    next[q] = k0 + kChunk + q < D ? ldStream(&L.nbr_key[size_t(k0 + kChunk + q) * N + i]) : kNone;
+        }
         R4<R> rw[kChunk];
         u32 other[kChunk];
         for (u32 q = 0; q < kChunk; ++q) {
@@ -258,6 +274,9 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
                 for (int c = 0; c < 3; ++c) acc[r * 3 + c] += w * cc[r] * rr[c];
         }
         if (key[kChunk - 1] == kNone) break;
+        if (!BLX_STRAIN_PREFETCH)
+            for (u32 q = 0; q < kChunk; ++q)
+                next[q] = k0 + kChunk + q < D ? ldStream(&L.nbr_key[size_t(k0 + kChunk + q) * N + i]) : kNone;
     }
     R inv[9], un[3], e[6];
     for (int k = 0; k < 9; ++k) inv[k] = ldg(&L.rinv[size_t(k) * N + i]);
@@ -361,8 +380,9 @@ __device__ __forceinline__ V3<R> bondVec(const DevLattice<R> &L, u32 a, u32 b, c
 // direction comes from the edge palette, where it was computed as
 // e * (1 / |e|) by the same expression.
 template <class R, bool DIRECT>
-__device__ void sampleBond(const DevLattice<R> &L, u32 a, u32 b, u32 p, const EdgePal<R> &g, const R4<R> &ua,
-                           const R4<R> &ub, R &t, R &c, R &s) {
+__device__ __forceinline__ void sampleBondWith(const DevLattice<R> &L, u32 a, u32 b, u32 p, const EdgePal<R> &g,
+                                               const R4<R> &ua, const R4<R> &ub, const R4<R> (&st)[4], R &t, R &c,
+                                               R &s) {
     const V3<R> delta = bondVec<R, DIRECT>(L, a, b, g.e, ua, ub);
     const R len = length(delta);
     const R rest = g.e.w;
@@ -379,18 +399,34 @@ __device__ void sampleBond(const DevLattice<R> &L, u32 a, u32 b, u32 p, const Ed
     s = R(0);
     if (!(p & kNoDir)) {
         const V3<R> dir{g.d.y, g.d.z, g.d.w};
-        const u32 ends[2] = {a, b};
         for (int q = 0; q < 2; ++q) {
-            const R4<R> sb = ldg(&L.sb[ends[q]]);
-            if (sb.z == R(0)) continue;
-            const R4<R> sa = ldg(&L.sa[ends[q]]);
+            // st holds (sa, sb) of end a, then of end b; sb.z is the valid flag.
+            if (st[2 * q + 1].z == R(0)) continue;
             R nn, sh;
-            resolve(sa, sb, dir, nn, sh);
+            resolve(st[2 * q], st[2 * q + 1], dir, nn, sh);
             t = maxR(t, nn);
             c = maxR(c, -nn);
             s = maxR(s, sh);
         }
     }
+}
+
+// The nodal strains a bond sample reads: (sa, sb) of end a, then of end b.
+template <class R> __device__ __forceinline__ void loadEndStrains(const DevLattice<R> &L, u32 a, u32 b, u32 p,
+                                                                  R4<R> (&st)[4]) {
+    if (p & kNoDir) return;
+    st[0] = ldg(&L.sa[a]);
+    st[1] = ldg(&L.sb[a]);
+    st[2] = ldg(&L.sa[b]);
+    st[3] = ldg(&L.sb[b]);
+}
+
+template <class R, bool DIRECT>
+__device__ void sampleBond(const DevLattice<R> &L, u32 a, u32 b, u32 p, const EdgePal<R> &g, const R4<R> &ua,
+                           const R4<R> &ub, R &t, R &c, R &s) {
+    R4<R> st[4] = {};
+    loadEndStrains(L, a, b, p, st);
+    sampleBondWith<R, DIRECT>(L, a, b, p, g, ua, ub, st, t, c, s);
 }
 
 // The edge palette's derived terms: |e| and the unit direction
@@ -1066,7 +1102,10 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
                                               double &pw, u32 &claim_a, u32 &claim_b) {
     const BondNode r = in.r;
     R4<R> pv = in.pv;
-    u32 live = in.live;
+    if (!in.live) return;
+    const u32 j = r.j;
+    // Every gather the bond needs goes out in one batch, before any test on
+    // what it returns: one memory round trip after the record, not three.
     Thresholds<R> th{};
     if (L.tensile_only) {
         const Tensile<R> t2 = ldStream(&L.thr2[o]);
@@ -1075,17 +1114,24 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
     } else {
         th = L.thr[o];
     }
-    if (COVER == kBondUnaffected && live && (ldg(&L.affected[r.a]) == stamp || ldg(&L.affected[r.b]) == stamp)) live = 0;
-    if (!live) return;
-    const u32 j = r.j;
+    u32 aff_a = 0, aff_b = 0;
+    if (COVER == kBondUnaffected) {
+        aff_a = ldg(&L.affected[r.a]);
+        aff_b = ldg(&L.affected[r.b]);
+    }
     const R4<R> ua = ldg(&L.u4[r.a]), ub = ldg(&L.u4[r.b]);
     const EdgePal<R> g = ldg(&L.epal[r.p & ~kNoDir]);
+    R4<R> st[4] = {};
+    loadEndStrains(L, r.a, r.b, r.p, st);
+    R plastic = R(0), ps = R(0);
+    if (PLASTIC) {
+        plastic = ldg(&L.pext[j]);
+        ps = ldg(&L.pstr[o]);
+    }
+    if (COVER == kBondUnaffected && (aff_a == stamp || aff_b == stamp)) return;
     const R restL = g.e.w;
-    R plastic = R(0);
     if (PLASTIC) {
         // bondPlasticReturn (LatticePhysics.hpp:1018-1039).
-        plastic = L.pext[j];
-        R ps = L.pstr[o];
         const R elastic = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
         const R mag = absR(elastic);
         const R h = S.hardening;
@@ -1105,7 +1151,7 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
         mp = fmaxf(mp, float(absR(plastic) / restL));
     }
     R t, c, s;
-    sampleBond<R, DIRECT>(L, r.a, r.b, r.p, g, ua, ub, t, c, s);
+    sampleBondWith<R, DIRECT>(L, r.a, r.b, r.p, g, ua, ub, st, t, c, s);
     const R pt = maxR(R(0), maxR(pv.x, t)), pc = maxR(R(0), maxR(pv.y, c)), pss = maxR(R(0), maxR(pv.z, s));
     mt = fmaxf(mt, float(pt));
     mc = fmaxf(mc, float(pc));
@@ -1340,6 +1386,8 @@ __device__ double fixedOrderSum(const double *part, u32 count, u32 stride, u32 o
 }
 
 constexpr u32 kEventBatch = 4096;
+constexpr u32 kGroupsPerThread = 4;  // summary words a finish thread reads at once
+constexpr u32 kFlagsInLds = 2048;    // flagged event words kept in LDS
 constexpr unsigned kBudgetSpent = 3U;  // the reference's max-steps exit reason
 
 // Position of the k-th (from 0) set bit of m; k < popcount(m).
@@ -1369,6 +1417,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     __shared__ float s_d[kEventBatch];
     __shared__ R s_e[kEventBatch];
     __shared__ u32 s_take;
+    __shared__ u32 s_flags[kFlagsInLds];
     const u32 tid = threadIdx.x, lane = tid & 63U;
     const u64 step = ctl->step;
     // Wave 0 carries the folds; its lanes all hold the same running values.
@@ -1378,26 +1427,39 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     const bool first_round = ctl->broken_bonds == 0;
     u64 ff = ctl->first_failure_count;
     const double flat = ctl->flat_fraction;
-    for (u32 gbase = 0; gbase < L.group_words; gbase += kFinishThreads) {
-        // Event words flagged in these summary words, in order, into flag_list.
-        const u32 g = gbase + tid;
-        u64 bits = 0;
-        if (g < L.group_words) {
-            bits = L.grp_bits[g];
-            if (bits) L.grp_bits[g] = 0;
+    for (u32 gbase = 0; gbase < L.group_words; gbase += kFinishThreads * kGroupsPerThread) {
+        // Event words flagged in these summary words, in order, into the flag
+        // list: in LDS when they fit (the usual case), else in flag_list. A
+        // thread reads its consecutive summary words in one round trip.
+        u64 bits[kGroupsPerThread];
+        u32 cnt = 0;
+#pragma unroll
+        for (u32 q = 0; q < kGroupsPerThread; ++q) {
+            const u32 g = gbase + tid * kGroupsPerThread + q;
+            bits[q] = g < L.group_words ? L.grp_bits[g] : 0ULL;
+        }
+#pragma unroll
+        for (u32 q = 0; q < kGroupsPerThread; ++q) {
+            if (bits[q]) L.grp_bits[gbase + tid * kGroupsPerThread + q] = 0;
+            cnt += u32(__popcll(bits[q]));
         }
         u32 nflag = 0;
-        u32 at = blockScan1024(u32(__popcll(bits)), s_waves, &nflag);
-        while (bits) {
-            L.flag_list[at++] = g * 64U + u32(__ffsll((long long)bits) - 1);
-            bits &= bits - 1;
+        u32 at = blockScan1024(cnt, s_waves, &nflag);
+        u32 *flags = nflag <= kFlagsInLds ? s_flags : L.flag_list;
+#pragma unroll
+        for (u32 q = 0; q < kGroupsPerThread; ++q) {
+            const u32 g = gbase + tid * kGroupsPerThread + q;
+            while (bits[q]) {
+                flags[at++] = g * 64U + u32(__ffsll((long long)bits[q]) - 1);
+                bits[q] &= bits[q] - 1;
+            }
         }
         __syncthreads();
         for (u32 fb = 0; fb < nflag;) {
             // As many whole event words as fit in the event batch; a word holds
             // at most 64 events, so at least one always fits.
             const u32 mcount = min(u32(kFinishThreads), nflag - fb);
-            const u32 wd = tid < mcount ? L.flag_list[fb + tid] : 0U;
+            const u32 wd = tid < mcount ? flags[fb + tid] : 0U;
             u64 evw = 0, bkw = 0;
             if (tid < mcount) {
                 evw = L.ev_bits[wd];
