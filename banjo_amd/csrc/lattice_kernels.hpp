@@ -1420,7 +1420,8 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     __shared__ u32 s_flags[kFlagsInLds];
     const u32 tid = threadIdx.x, lane = tid & 63U;
     const u64 step = ctl->step;
-    // Wave 0 carries the folds; its lanes all hold the same running values.
+    // Wave 0 carries the folds: its lanes all hold the same maximum damage and
+    // counts; lane 0 holds the removed-energy sum and its gain flag.
     double m = ctl->max_damage, removed = ctl->removed_energy_j;
     bool damage_gain = false, energy_gain = false;
     u32 broken_now = 0;
@@ -1498,12 +1499,13 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
             // Wave 0 folds the staged batch in bond order, 64 events at a time;
             // the sequential steps broadcast one lane's values with readlane.
             if (tid < 64) {
+                u32 nb = 0;  // broken bonds of this batch, their energies compacted into s_e[0, nb)
                 for (u32 c0 = 0; c0 < nev; c0 += 64) {
                     const u32 e = c0 + lane;
                     const bool ok = e < nev;
                     const u32 jr = ok ? s_j[e] : 0U;
                     const float dmg = ok ? s_d[e] : 0.f;
-                    const double en = ok ? double(s_e[e]) : 0.0;
+                    const R en = ok ? s_e[e] : R(0);
                     // max_damage rises only by more than 1e-6 (CpuLatticeBackend.cpp:462-465).
                     u64 cand = __ballot(ok && double(dmg) > m + 1.0e-6);
                     while (cand) {
@@ -1512,20 +1514,32 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
                         damage_gain = true;
                         cand = __ballot(ok && lane > l && double(dmg) > m + 1.0e-6);
                     }
-                    // Removed energy summed in bond order (CpuLatticeBackend.cpp:466-477).
+                    // Broken bonds: first-failure list and counts in bond order
+                    // (CpuLatticeBackend.cpp:466-477).
                     const u64 bmask = __ballot(ok && (jr >> 31));
                     if (first_round && ok && (jr >> 31))
                         L.first_fail[ff + __popcll(bmask & ((1ULL << lane) - 1ULL))] = jr & 0x7fffffffU;
                     ff += __popcll(bmask);
                     broken_now += u32(__popcll(bmask));
-                    u64 left = bmask;
-                    while (left) {
-                        const u32 l = u32(__ffsll((long long)left) - 1);
-                        left &= left - 1;
-                        const double el = laneValue(en, l);
-                        removed += el;
-                        if (el > flat * (removed > 1.0e-12 ? removed : 1.0e-12)) energy_gain = true;
+                    // In place: every lane has read its entry, and the write
+                    // index never passes the read index.
+                    if (ok && (jr >> 31)) s_e[nb + __popcll(bmask & ((1ULL << lane) - 1ULL))] = en;
+                    nb += u32(__popcll(bmask));
+                }
+                // Removed energy summed in bond order by one lane: the only
+                // dependence from one bond to the next is the addition.
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                if (lane == 0) {
+                    double sum = removed;
+                    bool gain = false;
+#pragma unroll 8
+                    for (u32 k = 0; k < nb; ++k) {
+                        const double el = double(s_e[k]);
+                        sum += el;
+                        gain |= el > flat * (sum > 1.0e-12 ? sum : 1.0e-12);
                     }
+                    removed = sum;
+                    energy_gain |= gain;
                 }
             }
             __syncthreads();
