@@ -570,7 +570,7 @@ public:
         o.ev_e = arena.reserve<R>(B);
         o.flag_list = arena.reserve<uint32_t>(event_words);
         o.tile_cand = arena.reserve<uint32_t>(node_tiles);
-        o.dirty_list = arena.reserve<uint32_t>(N);
+        o.dirty_list = arena.reserve<uint32_t>(2 * size_t(N));  // two halves, by substep parity
         o.dirty_stamp = arena.reserve<uint32_t>(N);
         o.rank_stamp = arena.reserve<uint32_t>(N);
         o.affected = arena.reserve<uint32_t>(N);
@@ -605,7 +605,8 @@ public:
         putv(o.sdelta, sdelta);
         putv(o.epal, epal);
         putv(o.nbb, sch.nbb);
-        putv(o.dirty_list, all);  // upload marks every node dirty
+        // Upload marks every node dirty, in the half substep 0's start sample reads.
+        put(o.dirty_list + size_t(N) * sizeof(uint32_t), all.data(), all.size() * sizeof(uint32_t));
         // Palette directions on the device, then the bond arrays that carry
         // the no-direction flag.
         hipLaunchKernelGGL((k_edge_palette<R>), dim3(std::max(1U, std::min(4096U, (np + kWG - 1) / kWG))), dim3(kWG), 0,
@@ -665,8 +666,8 @@ public:
         // Control block and sphere. A slot stays stopped until it is armed.
         DevCtl c{};
         c.first_failure_step = c.last_failure_step = c.last_energy_gain_step = ~0ULL;
-        c.dirty_start = 1;
-        c.dirty_count = N;
+        c.dirty_start[1] = 1;  // the half of parity 1: read by substep 0 and the upload's k_rinv
+        c.dirty_count[1] = N;
         c.rank_slot = 1;  // booked by the first substep's k_finish
         c.rinv_stamp = 0xffffffffU;
         c.flat_fraction = 1.0e-3;
@@ -699,7 +700,7 @@ public:
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&early_, hipStreamNonBlocking));
-        for (hipEvent_t *e : {&ev_bond_end_, &ev_rinv_, &ev_pass1_, &ev_velocity_, &ev_early_})
+        for (hipEvent_t *e : {&ev_bond_end_, &ev_rinv_, &ev_pass1_, &ev_velocity_, &ev_early_, &ev_predict_, &ev_start_})
             HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming));
         HIP_OK(hipEventCreate(&ev0_));
         HIP_OK(hipEventCreate(&ev1_));
@@ -718,7 +719,7 @@ public:
         (void)hipSetDevice(device_);
         for (hipStream_t s : {stream_, side_, early_})
             if (s) (void)hipStreamSynchronize(s);
-        for (hipEvent_t e : {ev_bond_end_, ev_rinv_, ev_pass1_, ev_velocity_, ev_early_, ev0_, ev1_})
+        for (hipEvent_t e : {ev_bond_end_, ev_rinv_, ev_pass1_, ev_velocity_, ev_early_, ev_predict_, ev_start_, ev0_, ev1_})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
         for (hipStream_t s : {early_, side_, stream_})
@@ -930,6 +931,8 @@ public:
             const uint64_t n = std::min<uint64_t>(profilingBatch(batch_len), substeps - done);
             HIP_OK(hipEventRecord(ev0_, stream_));
             for (uint64_t q = 0; q < n; ++q) enqueueSubstep();
+            // side_ ends each batch joined: its last k_rinv is done when the batch is.
+            HIP_OK(hipStreamWaitEvent(stream_, ev_rinv_, 0));
             HIP_OK(hipEventRecord(ev1_, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
             float ms = 0.f;
@@ -1186,6 +1189,7 @@ private:
     // overlap the folds; early_ the end sample of the nodes and bonds striker
     // pass 2 cannot reach, which overlaps pass 2.
     hipStream_t stream_ = nullptr, side_ = nullptr, early_ = nullptr;
+    hipEvent_t ev_predict_ = nullptr, ev_start_ = nullptr;
     hipEvent_t ev_bond_end_ = nullptr, ev_rinv_ = nullptr, ev_pass1_ = nullptr, ev_velocity_ = nullptr,
                ev_early_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -1429,14 +1433,26 @@ private:
                        small = batch(small_map_), lats = batch(lattice_map_);
         const dim3 ng = tiles(node_tiles_map_), bg = tiles(bond_tiles_map_), lg = entries(list_map_),
                    sg = entries(small_map_), fg = entries(lattice_map_);
-        BLX_LAUNCH(KStrainStart, 1, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainDirty>), lg, dim3(kWG), 0, stream_, list));
-        BLX_LAUNCH(KBondStart, 2, hipLaunchKernelGGL((k_bond_start<R, DIRECT>), lg, dim3(kWG), 0, stream_, list));
         profBegin(KPredict, 3);
         if (sphere && any_loads_) hipLaunchKernelGGL((k_predict<R, true, true, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
         else if (sphere) hipLaunchKernelGGL((k_predict<R, true, false, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
         else if (any_loads_) hipLaunchKernelGGL((k_predict<R, false, true, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
         else hipLaunchKernelGGL((k_predict<R, false, false, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
         profEnd();
+        // The start sample of the nodes and bonds the last failure round made
+        // dirty reads the positions at substep start from up4, so it runs on
+        // side_ after k_predict, behind the previous substep's k_rinv on the
+        // same stream, and overlaps the striker pass and the sweep; nothing
+        // before the end sample reads what it writes.
+        HIP_OK(hipEventRecord(ev_predict_, stream_));
+        HIP_OK(hipStreamWaitEvent(side_, ev_predict_, 0));
+        profBegin(KStrainStart, 1, side_);
+        hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainDirty>), lg, dim3(kWG), 0, side_, list);
+        profEnd(side_);
+        profBegin(KBondStart, 2, side_);
+        hipLaunchKernelGGL((k_bond_start<R, DIRECT>), lg, dim3(kWG), 0, side_, list);
+        profEnd(side_);
+        HIP_OK(hipEventRecord(ev_start_, side_));
         if (sphere) BLX_LAUNCH(KPass1, 4, hipLaunchKernelGGL((k_pass<R, 1>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
         if (split) {
             HIP_OK(hipEventRecord(ev_pass1_, stream_));
@@ -1468,6 +1484,7 @@ private:
             if ((S.plane_count > 0 || S.blade.enabled) && it + 1 < iterations)
                 BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
+        HIP_OK(hipStreamWaitEvent(stream_, ev_start_, 0));
         profBegin(KVelocity, 8);
         if (S.plane_count > 0 || S.blade.enabled)
             hipLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nodes, S);
@@ -1501,7 +1518,9 @@ private:
             else hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAll>), bg, dim3(kWG), 0, stream_, bonds, S);
             profEnd();
         }
-        // The dirty nodes' inverse rest covariances overlap the ordered folds.
+        // The dirty nodes' inverse rest covariances overlap the ordered folds
+        // and the next substep's k_predict; the next start sample follows them
+        // on side_.
         HIP_OK(hipEventRecord(ev_bond_end_, stream_));
         HIP_OK(hipStreamWaitEvent(side_, ev_bond_end_, 0));
         profBegin(KRinv, 1, side_);
@@ -1514,7 +1533,6 @@ private:
         else if (plastic) hipLaunchKernelGGL((k_finish<R, false, true>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
         else hipLaunchKernelGGL((k_finish<R, false, false>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
         profEnd();
-        HIP_OK(hipStreamWaitEvent(stream_, ev_rinv_, 0));
         HIP_OK(hipGetLastError());
     }
 

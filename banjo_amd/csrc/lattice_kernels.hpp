@@ -44,7 +44,7 @@ struct DevCtl {
     u64 step;            // RunStatus::total_steps
     u32 stop;            // nonzero: the remaining substeps of the batch are no-ops
     u32 error;           // bit 0: external kick overflowed backend precision
-    u32 dirty_start;     // recompute the start sample before the next substep
+    u32 dirty_start[2];  // by substep parity: recompute the start sample of the next substep
     u32 loads_active;
     u64 load_remaining;
     u64 first_failure_step, last_failure_step;
@@ -63,7 +63,7 @@ struct DevCtl {
     u32 pstretch_bits;
     u32 rank_deficient;
     u32 exit_reason;
-    u32 dirty_count;     // entries in dirty_list
+    u32 dirty_count[2];  // by substep parity: entries in that half of dirty_list
     u32 rank_slot;       // parity slot k_rinv counts into (set by k_bond_end)
     u32 rank_pending[2]; // rank-deficient recomputations not yet booked
     u32 rinv_stamp;      // break round the dirty list belongs to
@@ -131,7 +131,11 @@ template <class R> struct DevLattice {
     R *ev_e;          // stored energy removed by a break
     u32 *flag_list;   // scratch: flagged groups in order
     u32 *tile_cand;   // candidates per node tile
-    u32 *dirty_list;  // nodes a break made dirty, each once per substep (unordered)
+    // Nodes a break made dirty, each once per substep (unordered), in two halves
+    // of N by substep parity: the end sample of substep t fills half t & 1, and
+    // the start sample of substep t + 1 reads it while substep t + 1's end
+    // sample fills the other half.
+    u32 *dirty_list;
     u32 *dirty_stamp; // substep stamp of a node's entry in dirty_list
     u32 *rank_stamp;  // break round a node was last counted rank-deficient in
     u32 *affected;    // substep stamp: striker pass 2 can move the node or a neighbour
@@ -226,7 +230,9 @@ template <class R> __device__ void rinvNode(const DevLattice<R> &L, u32 i, u32 *
 }
 
 // The nodal strain from the cached inverse (LatticePhysics.hpp:833-906).
-template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &L, u32 i) {
+// PREV takes the positions from up4, the displacement at substep start: the
+// start sample runs after k_predict, which copies u to up4 before it moves u.
+template <class R, bool DIRECT, bool PREV> __device__ void strainNode(const DevLattice<R> &L, u32 i) {
     const u32 N = L.N, D = L.D;
 #ifndef BLX_STRAIN_CHUNK
 #define BLX_STRAIN_CHUNK 8
@@ -239,7 +245,8 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
     constexpr u32 kChunk = BLX_STRAIN_CHUNK;
     // The node's flag, own displacement and first keys go out together.
     const u8 valid = ldg(&L.valid[i]);
-    const R4<R> ui = ldg(&L.u4[i]);
+    const R4<R> *pos = PREV ? L.up4 : L.u4;
+    const R4<R> ui = ldg(&pos[i]);
     u32 next[kChunk];
     for (u32 q = 0; q < kChunk; ++q) next[q] = q < D ? ldStream(&L.nbr_key[size_t(q) * N + i]) : kNone;
     if (!valid) return;
@@ -262,7 +269,7 @@ template <class R, bool DIRECT> __device__ void strainNode(const DevLattice<R> &
         }
         R4<R> xo4[kChunk];
         for (u32 q = 0; q < kChunk; ++q)
-            xo4[q] = !(key[q] & kDeadSlot) ? ldg(&L.u4[other[q]]) : R4<R>{R(0), R(0), R(0), R(0)};
+            xo4[q] = !(key[q] & kDeadSlot) ? ldg(&pos[other[q]]) : R4<R>{R(0), R(0), R(0), R(0)};
         for (u32 q = 0; q < kChunk; ++q) {
             if (key[q] & kDeadSlot) continue;
             const R w = rw[q].w;
@@ -307,10 +314,12 @@ __global__ __launch_bounds__(kWG) void k_strain(Batch<R> B) {
         const WgMap m = B.map[blockIdx.x];
         const DevLattice<R> L = B.L[m.lat];
         if (L.ctl->stop) return;
-        if (COVER == kStrainDirty && !L.ctl->dirty_start) return;
-        const u32 count = COVER == kStrainDirty ? L.ctl->dirty_count : L.ctl->affected_count;
-        const u32 *list = COVER == kStrainDirty ? L.dirty_list : L.affected_list;
-        for (u32 k = m.a * kWG + threadIdx.x; k < count; k += m.b * kWG) strainNode<R, DIRECT>(L, list[k]);
+        const u32 half = u32(L.ctl->step & 1U) ^ 1U;  // the previous substep's half
+        if (COVER == kStrainDirty && !L.ctl->dirty_start[half]) return;
+        const u32 count = COVER == kStrainDirty ? L.ctl->dirty_count[half] : L.ctl->affected_count;
+        const u32 *list = COVER == kStrainDirty ? L.dirty_list + size_t(half) * L.N : L.affected_list;
+        for (u32 k = m.a * kWG + threadIdx.x; k < count; k += m.b * kWG)
+            strainNode<R, DIRECT, COVER == kStrainDirty>(L, list[k]);
         return;
     }
     const WgMap *m = tileEntry(B);
@@ -320,7 +329,7 @@ __global__ __launch_bounds__(kWG) void k_strain(Batch<R> B) {
     const u32 i = m->a * kWG + threadIdx.x;
     if (i >= L.N) return;
     if (COVER == kStrainUnaffected && L.affected[i] == u32(L.ctl->step) + 1U) return;
-    strainNode<R, DIRECT>(L, i);
+    strainNode<R, DIRECT, false>(L, i);
 }
 
 // The nodes striker pass 2 can reach this substep: its candidates (the
@@ -360,10 +369,13 @@ template <class R> __global__ __launch_bounds__(kWG) void k_mark_affected(Batch<
 template <class R> __global__ __launch_bounds__(kWG) void k_rinv(Batch<R> B) {
     const WgMap m = B.map[blockIdx.x];
     const DevLattice<R> L = B.L[m.lat];
-    const u32 count = L.ctl->dirty_count;
-    u32 *counter = &L.ctl->rank_pending[L.ctl->rank_slot];
+    // rank_slot is the parity of the substep whose end sample filled the list.
+    const u32 half = L.ctl->rank_slot;
+    const u32 count = L.ctl->dirty_count[half];
+    const u32 *list = L.dirty_list + size_t(half) * L.N;
+    u32 *counter = &L.ctl->rank_pending[half];
     const u32 stamp = L.ctl->rinv_stamp;
-    for (u32 k = m.a * kWG + threadIdx.x; k < count; k += m.b * kWG) rinvNode<R>(L, L.dirty_list[k], counter, stamp);
+    for (u32 k = m.a * kWG + threadIdx.x; k < count; k += m.b * kWG) rinvNode<R>(L, list[k], counter, stamp);
 }
 
 // ---------------------------------------------------------------------------
@@ -451,15 +463,18 @@ template <class R> __global__ void k_edge_palette(EdgePal<R> *pal, u32 *no_dir, 
 template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_start(Batch<R> B) {
     const WgMap m = B.map[blockIdx.x];
     const DevLattice<R> L = B.L[m.lat];
-    if (L.ctl->stop || !L.ctl->dirty_start) return;
-    const u64 pairs = u64(L.ctl->dirty_count) * L.D;
+    const u32 half = u32(L.ctl->step & 1U) ^ 1U;  // the previous substep's half
+    if (L.ctl->stop || !L.ctl->dirty_start[half]) return;
+    const u64 pairs = u64(L.ctl->dirty_count[half]) * L.D;
+    const u32 *list = L.dirty_list + size_t(half) * L.N;
     for (u64 q = u64(m.a) * kWG + threadIdx.x; q < pairs; q += u64(m.b) * kWG) {
-        const u32 i = L.dirty_list[q / L.D], k = u32(q % L.D);
+        const u32 i = list[q / L.D], k = u32(q % L.D);
         const u32 o = L.nbr_bond[size_t(k) * L.N + i];
         if (o == kNone || !L.alive_n[o]) continue;
         const BondNode r = L.bnd[o];
         R t, c, s;
-        sampleBond<R, DIRECT>(L, r.a, r.b, r.p, L.epal[r.p & ~kNoDir], L.u4[r.a], L.u4[r.b], t, c, s);
+        // Positions at substep start (up4): this runs after k_predict.
+        sampleBond<R, DIRECT>(L, r.a, r.b, r.p, L.epal[r.p & ~kNoDir], L.up4[r.a], L.up4[r.b], t, c, s);
         R4<R> pv = L.pvd[o];
         pv.x = maxR(R(0), t);
         pv.y = maxR(R(0), c);
@@ -492,8 +507,9 @@ __global__ __launch_bounds__(kWG) void k_predict(Batch<R> B, Step<R> S) {
     if (ctl->stop) return;
     const u32 tile = m->a;
     if (tile == 0 && threadIdx.x == 0) {
-        ctl->dirty_start = 0;
-        ctl->dirty_count = 0;
+        const u32 half = u32(ctl->step & 1U);  // this substep's half, which its end sample fills
+        ctl->dirty_start[half] = 0;
+        ctl->dirty_count[half] = 0;
         ctl->affected_count = 0;
     }
     const bool loads = LOADS && ctl->load_remaining > 0;
@@ -1270,6 +1286,7 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
     __shared__ double s_pw[kWG];
     const double m0 = ctl->max_damage;
     const u32 stamp = u32(ctl->step) + 1U;
+    const u32 half = u32(ctl->step & 1U);
     if (COVER != kBondAffected && m->a == 0 && threadIdx.x == 0) {
         ctl->rank_slot = u32(ctl->step & 1U);
         ctl->rinv_stamp = stamp;
@@ -1293,7 +1310,7 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
                                                                  claim_a, claim_b);
                 }
             }
-            appendDirty(L.dirty_list, &ctl->dirty_count, claim_a, claim_b);
+            appendDirty(L.dirty_list + size_t(half) * L.N, &ctl->dirty_count[half], claim_a, claim_b);
         }
         publishMaxima(ctl, mt, mc, ms, mp, s_max);
         return;
@@ -1306,7 +1323,7 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
         u32 claim_a = kNone, claim_b = kNone;
         double pw = 0.0;
         if (o < L.B) endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, in[q], m0, stamp, mt, mc, ms, mp, pw, claim_a, claim_b);
-        appendDirty(L.dirty_list, &ctl->dirty_count, claim_a, claim_b);
+        appendDirty(L.dirty_list + size_t(half) * L.N, &ctl->dirty_count[half], claim_a, claim_b);
         if (PLASTIC) {
             s_pw[threadIdx.x] = pw;
             __syncthreads();
@@ -1590,7 +1607,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     sphereSupport(S, sp, ev);
     ctl->ball_support_events = ev;
     *L.sphere = sp;
-    if (any) ctl->dirty_start = 1;
+    if (any) ctl->dirty_start[step & 1U] = 1;
     // Run loop bookkeeping (CpuLatticeBackend.cpp:279-298).
     const u64 total = step + 1;
     ctl->step = total;
