@@ -98,6 +98,13 @@ class LoadLedger(C.Structure):
                 ("angular_impulse_kg_m2_s", f64 * 3), ("work_j", f64)]
 
 
+class NodeContactAccumulators(C.Structure):
+    """NodeContactAccumulators, src/fastlattice/LatticePhysics.hpp:204-222."""
+    _fields_ = [("contacts", u64), ("pair_tests", u64), ("rebuilds", u64), ("pairs_listed", u64),
+                ("pair_overflow", u64), ("dissipated_kinetic_energy_j", f64), ("maximum_overlap_m", f64),
+                ("maximum_position_correction_m", f64), ("momentum_residual", f64 * 3)]
+
+
 class RunStatus(C.Structure):
     """RunStatus, src/fastlattice/FastLattice.hpp:182-245 (lattice-lane fields)."""
     _fields_ = [("total_steps", u64), ("first_failure_step", u64), ("last_failure_step", u64),
@@ -109,7 +116,8 @@ class RunStatus(C.Structure):
                 ("kernel_seconds", f64), ("wall_seconds", f64), ("frames_captured", u32),
                 ("phase_seconds", f64 * PHASE_COUNT), ("max_tensile_stretch", f32),
                 ("max_compressive_strain", f32), ("max_shear_strain", f32),
-                ("rank_deficient_nodes", u32), ("plastic_work_j", f64), ("max_plastic_stretch", f32)]
+                ("rank_deficient_nodes", u32), ("plastic_work_j", f64), ("max_plastic_stretch", f32),
+                ("node_contact", NodeContactAccumulators)]
 
 
 class AssetView(C.Structure):

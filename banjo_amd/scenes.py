@@ -140,6 +140,54 @@ def settings(dt, gravity=(0.0, 0.0, 0.0), iterations=1, planes=(), sphere=False,
     return s
 
 
+def contact_bucket_mask(node_count):
+    """latticeContactBucketMask (LatticePhysics.hpp:239-243): the smallest power of
+    two at least twice the node count (64 at least, 2^22 at most), minus one."""
+    size = 64
+    while size < 2 * node_count and size < (1 << 22):
+        size <<= 1
+    return size - 1
+
+
+def with_node_contact(s, cell, node_count, mode=2):
+    """Node-node contact settings as the reference's contact tests set them
+    (fast_lattice_tests.cpp:745-753): radius half a cell, skin a quarter."""
+    s.node_contact.mode = mode
+    s.node_contact.radius = 0.5 * cell
+    s.node_contact.skin = 0.25 * cell
+    s.node_contact.margin = 1.0e-5
+    s.node_contact.restitution = 0.0
+    s.node_contact.restitution_speed_threshold = 0.5
+    s.node_contact.static_friction = 0.5
+    s.node_contact.dynamic_friction = 0.4
+    s.node_contact.bucket_mask = contact_bucket_mask(node_count)
+    return s
+
+
+def two_cells(cell=0.01, separation=0.02, bonded=False, mode=2, mass_ratio=1.0, v=(4.0, -1.0), substeps=4000,
+              compliance=1.0e-6, node_mass=1.0e-3):
+    """Two free cells on the x axis closing on each other, optionally joined by
+    a bond at rest; dt 1e-6 s, no gravity, absolute-position arithmetic
+    (the pair of fast_lattice_tests.cpp:690-755)."""
+    pos = np.array([[-0.5 * separation, 0.0, 0.0], [0.5 * separation, 0.0, 0.0]])
+    m = np.array([node_mass, node_mass * mass_ratio])
+    if bonded:
+        a, b = np.array([0], np.uint32), np.array([1], np.uint32)
+        L, c = np.array([separation]), np.array([compliance])
+        thr = np.array([[1.0, 2.0, 1.0, 2.0, 1.0, 2.0]])
+    else:
+        a = b = np.zeros(0, np.uint32)
+        L = c = np.zeros(0)
+        thr = np.zeros((0, 6))
+    vel = np.array([[v[0], 0.0, 0.0], [v[1], 0.0, 0.0]])
+    asset = Asset(reference_position=pos, mass=m, bond_a=a, bond_b=b, rest_length=L, compliance=c, threshold=thr,
+                  velocity=vel)
+    st = with_node_contact(settings(1.0e-6, direct=True, cell=cell), cell, 2, mode)
+    return Scene(name=f"two_cells_{'bonded' if bonded else 'free'}", dims=(2, 1, 1), cell=cell, asset=asset,
+                 slabs=np.array([0, 2], np.uint32), settings=st, sphere=sphere_state(center=(0.0, 100.0, 0.0)),
+                 substeps=substeps)
+
+
 def sphere_state(center=(0, 0, 0), velocity=(0, 0, 0), radius=1.0, mass=1.0):
     sp = abi.Sphere()
     sp.center[:] = center

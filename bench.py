@@ -27,7 +27,10 @@ Printed JSON carries, besides the rate:
   served        a Poisson stream of marble trials admitted into a running
                 batch (trials/s, p50/p99 latency);
   cut           a 256 x 32 x 32 plastic plank cut by a kinematic blade, with
-                fragment labels after every 12-substep step.
+                fragment labels after every 12-substep step;
+  c1            the 16^3 cantilever of config 1, run whole.
+The CPU baseline also carries, under "more", the reference's threaded and
+serial CPU lanes on config 1, config 2 and the plank.
 """
 from __future__ import annotations
 
@@ -274,10 +277,48 @@ def cpu_leg(args, sc):
                 break
     except OSError:
         pass
-    return {"value": value, "unit": "bond-solves/s", "cores": cores, "kind": kind,
-            "sample": f"first {st.total_steps} substeps of the same 128x64x128 impact from t=0, float, "
-                      f"{'ParallelCpuLatticeBackend' if kind == 'reference' else 'serial oracle'} with {cores} "
-                      f"thread(s); {seconds:.2f} s; host {cpu_model}".strip()}
+    out = {"value": value, "unit": "bond-solves/s", "cores": cores, "kind": kind,
+           "sample": f"first {st.total_steps} substeps of the same 128x64x128 impact from t=0, float, "
+                     f"{'ParallelCpuLatticeBackend' if kind == 'reference' else 'serial oracle'} with {cores} "
+                     f"thread(s); {seconds:.2f} s; host {cpu_model}".strip()}
+    if kind == "reference" and not args.cpu_quick:
+        out["more"] = cpu_more(lib, sc, threads)
+    return out
+
+
+def cpu_more(lib, c2, threads):
+    """The reference's CPU lane on the other workloads, threaded and serial:
+    config 1 in full, config 2 and the plank over fixed windows."""
+    from banjo_amd import abi, scenes
+    from banjo_amd.lattice import control
+    from banjo_amd.runner import prepare
+
+    def rate(sc, kind, n, precision=abi.PRECISION_FLOAT):
+        sched, state = prepare(lib, sc)
+        be = lib.backend(precision, sched, kind, threads if kind == 1 else 1)
+        be.upload(state, sc.settings, sc.sphere)
+        if sc.forces is not None:
+            be.set_external_forces(sc.forces, sc.force_substeps)
+        t0 = time.perf_counter()
+        st = be.run(control(n))
+        secs = time.perf_counter() - t0
+        be.close()
+        return {"value": sc.bond_count * st.total_steps / secs, "substeps": st.total_steps, "seconds": secs,
+                "threads": threads if kind == 1 else 1}
+
+    c1 = scenes.config1()
+    plank = scenes.config5(substeps=100)
+    return {
+        "c1_full_threaded": rate(c1, 1, c1.substeps),
+        "c1_serial": rate(c1, 0, 128),
+        "c2_serial": rate(c2, 0, 1),
+        "plank_threaded": rate(plank, 1, 100),
+        "plank_serial": rate(plank, 0, 3),
+        "note": "config 1: 16^3 cantilever, 1,920 substeps (serial: first 128); config 2: the 128x64x128 impact "
+                "(serial: first substep); plank: the 256x32x32 plastic plank on its ledges without the blade "
+                "(the reference has none), first 100 substeps (serial: 3); float, bond-solves/s; the first "
+                "substep after upload samples every node, so short serial windows read low",
+    }
 
 
 def copy_probe(nbytes, repeats):
@@ -289,6 +330,32 @@ def copy_probe(nbytes, repeats):
               "hbm_copy_probe")
     return {"copy_bytes": nbytes, "read_bytes": nbytes, "write_bytes": nbytes, "seconds": secs.value,
             "gbps_read_plus_write": 2 * nbytes / secs.value / 1e9}
+
+
+def c1_leg(args):
+    """Config 1 on the GPU: the 16^3 cantilever, 1,920 substeps, run whole."""
+    from banjo_amd import abi, scenes
+    from banjo_amd.lattice import control, product
+    from banjo_amd.runner import prepare
+
+    hip = product()
+    sc = scenes.config1()
+    sched, state = prepare(hip, sc)
+    be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+    out = {}
+    for label in ("warm", "timed"):
+        be.upload(state, sc.settings, sc.sphere)
+        be.set_external_forces(sc.forces, sc.force_substeps)
+        t0 = time.perf_counter()
+        st = be.run(control(sc.substeps))
+        out[label] = (time.perf_counter() - t0, st)
+    be.close()
+    secs, st = out["timed"]
+    return {"nodes": sc.node_count, "bonds": sc.bond_count, "substeps": st.total_steps, "seconds": secs,
+            "bond_solves_per_s": sc.bond_count * st.total_steps / secs, "broken_bonds": st.broken_bonds,
+            "workload": "16^3 cantilever clamped at x=0, 5 N tip load and gravity, c 1e-6, break strain 1%, "
+                        "dt 1/1920 s, 1,920 substeps; a lattice this small is launch-bound (about 40 launches "
+                        "per substep)"}
 
 
 def cut_leg(args):
@@ -354,9 +421,11 @@ def main():
     ap.add_argument("--profile-substeps", type=int, default=1200)
     ap.add_argument("--cpu-substeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain,batched,served,cut",
+    ap.add_argument("--cpu-quick", action="store_true", help="only the config-2 window of the CPU baseline")
+    ap.add_argument("--forms", default="plain,batched,served,cut,c1",
                     help="comma list of plain (the headline impact), batched (many uneven lattices), "
-                         "served (a stream of trials), cut (a plank cut by a blade, labels every step)")
+                         "served (a stream of trials), cut (a plank cut by a blade, labels every step), "
+                         "c1 (the 16^3 cantilever in full)")
     ap.add_argument("--cut-steps", type=int, default=600)
     ap.add_argument("--batch-trials", type=int, default=512)
     ap.add_argument("--batch-substeps", type=int, default=8 * 240)
@@ -388,6 +457,8 @@ def main():
             out["served"] = served_leg(args)
         if "cut" in forms:
             out["cut"] = cut_leg(args)
+        if "c1" in forms:
+            out["c1"] = c1_leg(args)
         print(json.dumps(out))
         return
     t_build = time.perf_counter()
@@ -440,6 +511,8 @@ def main():
         out["served"] = served_leg(args)
     if "cut" in forms:
         out["cut"] = cut_leg(args)
+    if "c1" in forms:
+        out["c1"] = c1_leg(args)
     print(json.dumps(out))
 
 

@@ -145,6 +145,16 @@ typedef struct blx_load_ledger {
     double work_j;
 } blx_load_ledger;
 
+/* NodeContactAccumulators (LatticePhysics.hpp:204-222): the node-node
+ * contact ledger, summed in pair order. */
+typedef struct blx_node_contact_accumulators {
+    uint64_t contacts, pair_tests, rebuilds, pairs_listed, pair_overflow;
+    double dissipated_kinetic_energy_j;
+    double maximum_overlap_m;
+    double maximum_position_correction_m;
+    double momentum_residual[3];
+} blx_node_contact_accumulators;
+
 typedef struct blx_run_status {
     uint64_t total_steps;
     uint64_t first_failure_step;   /* UINT64_MAX: none */
@@ -170,6 +180,7 @@ typedef struct blx_run_status {
     uint32_t rank_deficient_nodes;
     double plastic_work_j;
     float max_plastic_stretch;
+    blx_node_contact_accumulators node_contact;
 } blx_run_status;
 
 /* LatticeAsset bonds + ActiveMatter nodes and bond state, asset order. */

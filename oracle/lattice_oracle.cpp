@@ -18,11 +18,12 @@
 //   run loop          CpuLatticeBackend.cpp:255-304
 //   external forces   ExternalLoads.hpp:28-50, 79-148
 //   fragments         fracture/ConnectedComponents.cpp:14-87
+//   node contact      LatticePhysics.hpp:1430-1667, CpuLatticeBackend.cpp:424-437
 //   box stencil       matter/BoxLattice.cpp:49-171 (Moore stencil in place of
 //                     the Euclidean horizon)
 //
-// Not restated (the entry points refuse them): node-node contact, plate
-// bending, the Verlet integrator, wrenches, reversible trials.
+// Not restated (the entry points refuse them): plate bending, the Verlet
+// integrator, wrenches, reversible trials.
 
 #include "../include/banjo_lattice.h"
 
@@ -291,7 +292,10 @@ public:
 
     void upload(const blx_state_view &st, const blx_settings &settings, const blx_sphere &sp) {
         if (settings.bond_integrator != BLX_BOND_XPBD) throw invalid("the oracle restates the XPBD integrator only");
-        if (settings.node_contact.mode != BLX_NODE_CONTACT_OFF) throw invalid("the oracle does not restate node-node contact");
+        if (settings.node_contact.mode > BLX_NODE_CONTACT_ON) throw invalid("unknown node contact mode");
+        if (settings.node_contact.mode != BLX_NODE_CONTACT_OFF &&
+            (settings.node_contact.bucket_mask & (settings.node_contact.bucket_mask + 1U)) != 0U)
+            throw invalid("node contact bucket_mask must be a power of two minus one");
         if (settings.plate_half_thickness != 0.0) throw invalid("the oracle does not restate plate bending");
         if (settings.plane_count > BLX_MAX_SUPPORT_PLANES) throw invalid("too many support planes");
         if (st.node_count != sch_.node_count || st.bond_count != sch_.bond_count) throw invalid("state does not match the schedule");
@@ -322,6 +326,17 @@ public:
         for (int k = 0; k < 3; ++k) origin[k] = st.origin[k];
         loads.clear(); load_remaining = 0;
         dirty_start = true;
+        const blx_node_contact_settings &nc = settings.node_contact;
+        NC = {nc.mode, R(nc.radius), R(nc.skin), R(nc.margin), R(nc.restitution), R(nc.restitution_speed_threshold),
+              R(nc.static_friction), R(nc.dynamic_friction), nc.bucket_mask};
+        if (NC.mode != BLX_NODE_CONTACT_OFF) {
+            const std::size_t buckets = std::size_t(NC.mask) + 1;
+            node_cell.assign(n3, 0); build_u.assign(n3, R(0));
+            bucket_begin.assign(buckets + 1, 0U); bucket_cursor.assign(buckets, 0U); bucket_nodes.assign(N, 0U);
+            pair_other.assign(std::size_t(kPairs) * N, 0U); pair_bond.assign(std::size_t(kPairs) * N, kNoBond);
+            pair_fill.assign(N, 0U); pair_node_list.assign(N, 0U); pair_node_count = 0;
+        }
+        contact_rebuild = true;
         first_failure.clear(); frames.clear();
         uploaded = true;
     }
@@ -437,6 +452,176 @@ private:
     double flat_fraction{1.0e-3};
     bool dirty_start{true};
     bool uploaded{false};
+
+    // Node-node contact (LatticePhysics.hpp:1430-1650): settings in backend
+    // precision, the broad phase's cells, hash and pair lists, and the flag that
+    // forces a rebuild after upload and after every failure round.
+    static constexpr u32 kPairs = 24;  // kMaxPairsPerNode (LatticePhysics.hpp:179)
+    struct NodeContact { u8 mode; R radius, skin, margin, restitution, rest_thr, sf, df; u32 mask; } NC{};
+    std::vector<std::int32_t> node_cell;
+    std::vector<R> build_u;
+    std::vector<u32> bucket_begin, bucket_cursor, bucket_nodes, pair_other, pair_bond, pair_fill, pair_node_list;
+    u32 pair_node_count{};
+    bool contact_rebuild{true};
+
+    R ncCutoff() const { return R(2) * NC.radius + NC.skin; }
+    // Teschner's spatial hash (LatticePhysics.hpp:1447-1456).
+    static u32 ncBucket(std::int32_t ix, std::int32_t iy, std::int32_t iz, u32 mask) {
+        return ((u32(ix) * 73856093U) ^ (u32(iy) * 19349663U) ^ (u32(iz) * 83492791U)) & mask;
+    }
+    // Step 1 (LatticePhysics.hpp:1460-1468).
+    void ncStoreCell(u32 i) {
+        const R inv = R(1) / ncCutoff();
+        const V3<R> p = position(i);
+        node_cell[3 * std::size_t(i)] = std::int32_t(std::floor(p.x * inv));
+        node_cell[3 * std::size_t(i) + 1] = std::int32_t(std::floor(p.y * inv));
+        node_cell[3 * std::size_t(i) + 2] = std::int32_t(std::floor(p.z * inv));
+        st3(build_u, i, ld3(u, i));
+    }
+    u32 ncBucketOf(u32 i) const {
+        return ncBucket(node_cell[3 * std::size_t(i)], node_cell[3 * std::size_t(i) + 1], node_cell[3 * std::size_t(i) + 2],
+                        NC.mask);
+    }
+    // Step 2 (LatticePhysics.hpp:1471-1489): counting sort into the buckets, in node order.
+    void ncHashBuild() {
+        const u32 buckets = NC.mask + 1U;
+        std::fill(bucket_begin.begin(), bucket_begin.end(), 0U);
+        for (u32 i = 0; i < N; ++i) if (mass[i] > R(0)) ++bucket_begin[ncBucketOf(i) + 1U];
+        for (u32 b = 0; b < buckets; ++b) bucket_begin[b + 1U] += bucket_begin[b];
+        for (u32 b = 0; b < buckets; ++b) bucket_cursor[b] = bucket_begin[b];
+        for (u32 i = 0; i < N; ++i) if (mass[i] > R(0)) bucket_nodes[bucket_cursor[ncBucketOf(i)]++] = i;
+    }
+    // Step 3 (LatticePhysics.hpp:1494-1546): the pairs node i owns, over the 27
+    // cells around it, skipping pairs a live bond holds. Returns the overflow.
+    u32 ncGather(u32 i) {
+        pair_fill[i] = 0U;
+        if (mass[i] <= R(0)) return 0U;
+        const R cutoff = ncCutoff(), cutoff2 = cutoff * cutoff;
+        const V3<R> p = position(i);
+        const std::int32_t cx = node_cell[3 * std::size_t(i)], cy = node_cell[3 * std::size_t(i) + 1],
+                           cz = node_cell[3 * std::size_t(i) + 2];
+        u32 fill = 0, over = 0;
+        for (std::int32_t dz = -1; dz <= 1; ++dz)
+            for (std::int32_t dy = -1; dy <= 1; ++dy)
+                for (std::int32_t dx = -1; dx <= 1; ++dx) {
+                    const std::int32_t gx = cx + dx, gy = cy + dy, gz = cz + dz;
+                    const u32 b = ncBucket(gx, gy, gz, NC.mask);
+                    for (u32 k = bucket_begin[b]; k < bucket_begin[b + 1U]; ++k) {
+                        const u32 j = bucket_nodes[k];
+                        if (j <= i) continue;
+                        if (node_cell[3 * std::size_t(j)] != gx || node_cell[3 * std::size_t(j) + 1] != gy ||
+                            node_cell[3 * std::size_t(j) + 2] != gz)
+                            continue;
+                        if (length2(position(j) - p) > cutoff2) continue;
+                        u32 bond = kNoBond;
+                        bool held = false;
+                        for (u32 q = 0; q < D; ++q) {
+                            const std::size_t slot = std::size_t(q) * N + i;
+                            if (nbr_bond[slot] == kNoBond) break;
+                            if (nbr_other[slot] != j) continue;
+                            bond = nbr_bond[slot];
+                            held = nbr_alive[slot] != 0;
+                            break;
+                        }
+                        if (held) continue;
+                        if (fill >= kPairs) { ++over; continue; }
+                        pair_other[std::size_t(kPairs) * i + fill] = j;
+                        pair_bond[std::size_t(kPairs) * i + fill] = bond;
+                        ++fill;
+                    }
+                }
+        pair_fill[i] = fill;
+        return over;
+    }
+    // Step 4 (LatticePhysics.hpp:1551-1561): the owners in node order; returns the pairs.
+    u32 ncActiveList() {
+        u32 count = 0, pairs = 0;
+        for (u32 i = 0; i < N; ++i) {
+            if (pair_fill[i] == 0U) continue;
+            pair_node_list[count++] = i;
+            pairs += pair_fill[i];
+        }
+        pair_node_count = count;
+        return pairs;
+    }
+    // LatticePhysics.hpp:1566-1574.
+    bool ncStale() const {
+        const R half = R(0.5) * NC.skin, limit = half * half;
+        for (u32 i = 0; i < N; ++i)
+            if (length2(ld3(u, i) - ld3(build_u, i)) > limit) return true;
+        return false;
+    }
+    // One pair's response (LatticePhysics.hpp:1585-1648).
+    void ncPair(u32 a, u32 b, u32 bond, blx_node_contact_accumulators &acc) {
+        if (bond != kNoBond && alive[bond]) return;
+        const R ma = mass[a], mb = mass[b];
+        if (ma <= R(0) || mb <= R(0)) return;
+        ++acc.pair_tests;
+        const V3<R> r = position(b) - position(a);
+        const R distance = length(r);
+        const V3<R> normal = normalized(r);
+        const R gap = distance - R(2) * NC.radius;
+        if (gap < R(0) && -gap > R(acc.maximum_overlap_m)) acc.maximum_overlap_m = double(-gap);
+        if (NC.mode != BLX_NODE_CONTACT_ON) return;
+        V3<R> va = ld3(v, a), vb = ld3(v, b);
+        const R vn = dot(vb - va, normal);
+        if (gap > maxR(NC.margin, -vn * S.dt)) return;
+        const R inverse_a = R(1) / ma, inverse_b = R(1) / mb;
+        const R inverse_pair = inverse_a + inverse_b;
+        const R restitution = -vn > NC.rest_thr ? NC.restitution : R(0);
+        const R desired = gap > NC.margin ? -gap / S.dt : -restitution * minR(vn, R(0));
+        const R normal_impulse = maxR(R(0), (desired - vn) / inverse_pair);
+        if (normal_impulse > R(0)) {
+            const V3<R> va0 = va, vb0 = vb;
+            const R before = R(0.5) * ma * length2(va) + R(0.5) * mb * length2(vb);
+            auto apply = [&](V3<R> impulse) {
+                vb = vb + inverse_b * impulse;
+                va = va - inverse_a * impulse;
+            };
+            apply(normal_impulse * normal);
+            const V3<R> relative = vb - va;
+            const V3<R> tangent = relative - dot(relative, normal) * normal;
+            const R tangent_speed = length(tangent);
+            if (tangent_speed > R(1.0e-12)) {
+                const V3<R> direction = tangent / tangent_speed;
+                const R sticking = tangent_speed / inverse_pair;
+                const R friction = sticking <= NC.sf * normal_impulse ? sticking : minR(sticking, NC.df * normal_impulse);
+                apply(-friction * direction);
+            }
+            st3(v, a, va);
+            st3(v, b, vb);
+            acc.dissipated_kinetic_energy_j += double(before - (R(0.5) * ma * length2(va) + R(0.5) * mb * length2(vb)));
+            acc.momentum_residual[0] += double(ma * (va.x - va0.x) + mb * (vb.x - vb0.x));
+            acc.momentum_residual[1] += double(ma * (va.y - va0.y) + mb * (vb.y - vb0.y));
+            acc.momentum_residual[2] += double(ma * (va.z - va0.z) + mb * (vb.z - vb0.z));
+            ++acc.contacts;
+        }
+        if (gap < -NC.margin) {
+            const R correction = minR(-gap - NC.margin, R(0.2) * NC.radius);
+            const V3<R> shift = (correction / inverse_pair) * normal;
+            st3(u, a, ld3(u, a) - inverse_a * shift);
+            st3(u, b, ld3(u, b) + inverse_b * shift);
+            if (correction > R(acc.maximum_position_correction_m)) acc.maximum_position_correction_m = double(correction);
+        }
+    }
+    // CpuLatticeBackend.cpp:424-437: rebuild when forced or stale, then the
+    // serial narrow phase in pair order (LatticePhysics.hpp:1655-1667).
+    void nodeContact() {
+        blx_node_contact_accumulators &acc = status.node_contact;
+        if (contact_rebuild || ncStale()) {
+            for (u32 i = 0; i < N; ++i) ncStoreCell(i);
+            ncHashBuild();
+            for (u32 i = 0; i < N; ++i) acc.pair_overflow += ncGather(i);
+            acc.pairs_listed += ncActiveList();
+            ++acc.rebuilds;
+            contact_rebuild = false;
+        }
+        for (u32 k = 0; k < pair_node_count; ++k) {
+            const u32 a = pair_node_list[k];
+            for (u32 q = 0; q < pair_fill[a]; ++q)
+                ncPair(a, pair_other[std::size_t(kPairs) * a + q], pair_bond[std::size_t(kPairs) * a + q], acc);
+        }
+    }
 
     V3<R> position(u32 i) const { return ld3(x0, i) + ld3(u, i); }
     V3<R> bondVector(u32 j) const {
@@ -1042,6 +1227,7 @@ private:
             for (u32 i = 0; i < N; ++i) if (!candidate[i]) nodeSupportVelocity(i);
         }
         if (S.sphere_enabled) contactPass(2, status.contact);
+        if (NC.mode != BLX_NODE_CONTACT_OFF) nodeContact();
         return finish();
     }
 
@@ -1074,7 +1260,7 @@ private:
         status.rank_deficient_nodes = rank_deficient;
         sphere.center = sphere.center + S.dt * sphere.velocity;
         sphereSupport(status.contact);
-        if (any) dirty_start = true;
+        if (any) dirty_start = contact_rebuild = true;
         return any;
     }
 

@@ -205,6 +205,15 @@ void statusToC(const RunStatus &s, blx_run_status &c) {
     c.max_tensile_stretch = s.max_tensile_stretch; c.max_compressive_strain = s.max_compressive_strain;
     c.max_shear_strain = s.max_shear_strain; c.rank_deficient_nodes = s.rank_deficient_nodes;
     c.plastic_work_j = s.plastic_work_j; c.max_plastic_stretch = s.max_plastic_stretch;
+    const NodeContactAccumulators &n = s.node_contact;
+    c.node_contact.contacts = n.contacts; c.node_contact.pair_tests = n.pair_tests; c.node_contact.rebuilds = n.rebuilds;
+    c.node_contact.pairs_listed = n.pairs_listed; c.node_contact.pair_overflow = n.pair_overflow;
+    c.node_contact.dissipated_kinetic_energy_j = n.dissipated_kinetic_energy_j;
+    c.node_contact.maximum_overlap_m = n.maximum_overlap_m;
+    c.node_contact.maximum_position_correction_m = n.maximum_position_correction_m;
+    c.node_contact.momentum_residual[0] = n.momentum_residual_x;
+    c.node_contact.momentum_residual[1] = n.momentum_residual_y;
+    c.node_contact.momentum_residual[2] = n.momentum_residual_z;
 }
 
 struct Handle {
