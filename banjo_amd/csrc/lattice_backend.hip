@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <array>
@@ -191,6 +192,9 @@ template <class R> Step<R> convertStep(const blx_settings &s) {
     o.c_slack = R(s.contact.prefilter_slack);
     o.plane_count = s.plane_count;
     o.sphere_enabled = s.sphere_enabled;
+    o.nc = {s.node_contact.mode, s.node_contact.bucket_mask, R(s.node_contact.radius), R(s.node_contact.skin),
+            R(s.node_contact.margin), R(s.node_contact.restitution), R(s.node_contact.restitution_speed_threshold),
+            R(s.node_contact.static_friction), R(s.node_contact.dynamic_friction)};
     for (int p = 0; p < 2; ++p) {
         const blx_support_plane &in = s.planes[p];
         Plane<R> &pl = o.planes[p];
@@ -218,7 +222,9 @@ void validateSettings(const blx_settings &s) {
     if (s.bond_integrator == BLX_BOND_VELOCITY_VERLET)
         throw Invalid("the Verlet bond integrator is the serial double CPU reference only");
     if (s.bond_integrator != BLX_BOND_XPBD) throw Invalid("unknown bond integrator");
-    if (s.node_contact.mode != BLX_NODE_CONTACT_OFF) throw Invalid("node-node contact is not available on the HIP backend");
+    if (s.node_contact.mode > BLX_NODE_CONTACT_ON) throw Invalid("unknown node contact mode");
+    if (s.node_contact.mode != BLX_NODE_CONTACT_OFF && (s.node_contact.bucket_mask & (s.node_contact.bucket_mask + 1U)) != 0U)
+        throw Invalid("node contact bucket_mask must be a power of two minus one");
     if (s.plate_half_thickness != 0.0) throw Invalid("plate bending is not available on the HIP backend");
     if (s.audit_energy) throw Invalid("the kinetic energy audit is not available on the HIP backend");
     if (s.plane_count > BLX_MAX_SUPPORT_PLANES) throw Invalid("at most two support planes");
@@ -252,6 +258,9 @@ public:
     uint32_t N = 0, B = 0, D = 0, node_tiles = 0, bond_tiles = 0, group_words = 0;
     uint32_t slot_palette = 0, edge_palette = 0;
     bool tensile_only = false;
+    bool contact = false;        // node-node contact arrays allocated
+    uint32_t nc_table_mask = 0;  // cell table size minus one (a power of two >= 2N)
+    DevBuf nc_temp;              // rocPRIM scratch of the contact rebuild
     DevLattice<R> L{};
     DevCtl ctl_host{};
     blx_run_status status{};
@@ -274,6 +283,8 @@ public:
         size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
         size_t affected, affected_list, tile_plastic, load_part, fext, freq, cand_all, cand_tile, cand_lo, cand_cnt;
         size_t nbb, first_fail;
+        size_t nc_build, nc_key, nc_key_sorted, nc_node, nc_node_sorted, nc_table_key, nc_table_val;
+        size_t pair_fill, pair_other, pair_bond, pair_offset, flat_a, flat_b, flat_bond;
     } off{};
     size_t used = 0;  // arena bytes the arrays occupy
     Arena arena;
@@ -367,6 +378,21 @@ public:
         l.cand_cnt = arena.at<uint32_t>(o.cand_cnt);
         l.nbb = arena.at<uint32_t>(o.nbb);
         l.first_fail = arena.at<uint32_t>(o.first_fail);
+        l.nc_build = arena.at<R4<R>>(o.nc_build);
+        l.nc_key = arena.at<uint64_t>(o.nc_key);
+        l.nc_key_sorted = arena.at<uint64_t>(o.nc_key_sorted);
+        l.nc_node = arena.at<uint32_t>(o.nc_node);
+        l.nc_node_sorted = arena.at<uint32_t>(o.nc_node_sorted);
+        l.nc_table_key = arena.at<uint64_t>(o.nc_table_key);
+        l.nc_table_val = arena.at<uint2>(o.nc_table_val);
+        l.nc_table_mask = nc_table_mask;
+        l.pair_fill = arena.at<uint32_t>(o.pair_fill);
+        l.pair_other = arena.at<uint32_t>(o.pair_other);
+        l.pair_bond = arena.at<uint32_t>(o.pair_bond);
+        l.pair_offset = arena.at<uint32_t>(o.pair_offset);
+        l.flat_a = arena.at<uint32_t>(o.flat_a);
+        l.flat_b = arena.at<uint32_t>(o.flat_b);
+        l.flat_bond = arena.at<uint32_t>(o.flat_bond);
         l.ctl = ctl_dev;
         l.sphere = sphere_dev;
         for (int k = 0; k < 3; ++k) l.origin[k] = origin[k];
@@ -390,6 +416,9 @@ public:
         slot_palette = src.slot_palette;
         edge_palette = src.edge_palette;
         tensile_only = src.tensile_only;
+        contact = src.contact;
+        nc_table_mask = src.nc_table_mask;
+        if (contact) nc_temp.alloc(src.nc_temp.bytes);
         mass_host = src.mass_host;
         inv_mass_host = src.inv_mass_host;
         node_j_host = src.node_j_host;
@@ -585,7 +614,36 @@ public:
         o.cand_cnt = arena.reserve<uint32_t>(sch.blocks);
         o.nbb = arena.reserve<uint32_t>(sch.nbb.size());
         o.first_fail = arena.reserve<uint32_t>(B);
+        // Node-node contact: the broad phase's cells, sort, cell table and pair lists.
+        contact = S.nc.mode != 0U;
+        const size_t nN = contact ? N : 0, nP = contact ? size_t(kPairsPerNode) * N : 0;
+        uint32_t table = 64;
+        while (contact && table < 2U * N) table <<= 1;
+        nc_table_mask = table - 1;
+        o.nc_build = arena.reserve<R4<R>>(nN);
+        o.nc_key = arena.reserve<uint64_t>(nN);
+        o.nc_key_sorted = arena.reserve<uint64_t>(nN);
+        o.nc_node = arena.reserve<uint32_t>(nN);
+        o.nc_node_sorted = arena.reserve<uint32_t>(nN);
+        o.nc_table_key = arena.reserve<uint64_t>(contact ? table : 0);
+        o.nc_table_val = arena.reserve<uint2>(contact ? table : 0);
+        o.pair_fill = arena.reserve<uint32_t>(nN);
+        o.pair_other = arena.reserve<uint32_t>(nP);
+        o.pair_bond = arena.reserve<uint32_t>(nP);
+        o.pair_offset = arena.reserve<uint32_t>(nN);
+        o.flat_a = arena.reserve<uint32_t>(nP);
+        o.flat_b = arena.reserve<uint32_t>(nP);
+        o.flat_bond = arena.reserve<uint32_t>(nP);
         arena.commit();
+        if (contact) {
+            size_t sort_bytes = 0, scan_bytes = 0;
+            HIP_OK(rocprim::radix_sort_pairs(nullptr, sort_bytes, static_cast<uint64_t *>(nullptr),
+                                             static_cast<uint64_t *>(nullptr), static_cast<uint32_t *>(nullptr),
+                                             static_cast<uint32_t *>(nullptr), N, 0, 64, stream));
+            HIP_OK(rocprim::exclusive_scan(nullptr, scan_bytes, static_cast<uint32_t *>(nullptr),
+                                           static_cast<uint32_t *>(nullptr), 0U, N, rocprim::plus<uint32_t>(), stream));
+            nc_temp.alloc(std::max(sort_bytes, scan_bytes));
+        }
 
         // Zero everything, then copy the data in.
         HIP_OK(hipMemsetAsync(arena.at<char>(0), 0, arena.used(), stream));
@@ -669,6 +727,7 @@ public:
         c.dirty_start[1] = 1;  // the half of parity 1: read by substep 0 and the upload's k_rinv
         c.dirty_count[1] = N;
         c.rank_slot = 1;  // booked by the first substep's k_finish
+        c.nc_rebuild = 1;  // the first substep with contact on builds the pair list
         c.rinv_stamp = 0xffffffffU;
         c.flat_fraction = 1.0e-3;
         c.calm_margin = 0.5;
@@ -930,7 +989,16 @@ public:
             if (maps_dirty_) buildMaps();
             const uint64_t n = std::min<uint64_t>(profilingBatch(batch_len), substeps - done);
             HIP_OK(hipEventRecord(ev0_, stream_));
-            for (uint64_t q = 0; q < n; ++q) enqueueSubstep();
+            for (uint64_t q = 0; q < n; ++q) {
+                enqueueSubstepA();
+                if (S_.nc.mode != 0U) {
+                    // The rebuild decision of each slot, read back once a substep.
+                    HIP_OK(hipStreamSynchronize(stream_));
+                    readAllCtls();
+                    rebuildContacts();
+                }
+                enqueueSubstepB();
+            }
             // side_ ends each batch joined: its last k_rinv is done when the batch is.
             HIP_OK(hipStreamWaitEvent(stream_, ev_rinv_, 0));
             HIP_OK(hipEventRecord(ev1_, stream_));
@@ -1041,6 +1109,15 @@ public:
         std::memcpy(&o.max_plastic_stretch, &c.pstretch_bits, 4);
         o.rank_deficient_nodes = c.rank_deficient;
         o.plastic_work_j = c.plastic_work;
+        o.node_contact.contacts = c.nc_contacts;
+        o.node_contact.pair_tests = c.nc_tests;
+        o.node_contact.rebuilds = c.nc_rebuilds;
+        o.node_contact.pairs_listed = c.nc_listed;
+        o.node_contact.pair_overflow = c.nc_overflow;
+        o.node_contact.dissipated_kinetic_energy_j = c.nc_dissipated;
+        o.node_contact.maximum_overlap_m = c.nc_overlap;
+        o.node_contact.maximum_position_correction_m = c.nc_correction;
+        for (int q = 0; q < 3; ++q) o.node_contact.momentum_residual[q] = c.nc_momentum[q];
         // A batch slot reports how its last advance ended; a spent budget is
         // the reference's max-steps reason.
         if (c.stop && c.exit_reason) o.exit_reason = c.exit_reason == kBudgetSpent ? 3U : c.exit_reason;
@@ -1365,12 +1442,12 @@ private:
     // reference phase id it belongs to (FastLattice.cpp:22-26).
     enum Kernel : uint32_t { KStrainStart, KBondStart, KPredict, KPass1, KSolve, KProject, KVelocity, KDamp,
                              KSupportVelocity, KPass2, KStrain, KBondEnd, KFinish, KRinv, KMark, KStrainLate,
-                             KBondEndLate, KCount };
+                             KBondEndLate, KContact, KCount };
     static const char *kernelName(uint32_t k) {
         static const char *names[KCount] = {"k_strain_start", "k_bond_start", "k_predict", "k_pass1", "k_solve",
                                             "k_project", "k_velocity", "k_damp", "k_support_velocity", "k_pass2",
                                             "k_strain", "k_bond_end", "k_finish", "k_rinv", "k_mark_affected",
-                                            "k_strain_late", "k_bond_end_late"};
+                                            "k_strain_late", "k_bond_end_late", "k_node_contact"};
         return k < KCount ? names[k] : "unknown";
     }
     struct ProfRecord { uint32_t kernel, phase; hipEvent_t a, b; };
@@ -1423,12 +1500,20 @@ private:
     static dim3 tiles(const MapRef &r) { return dim3(std::max(1U, xcdGrid(r.count))); }
     static dim3 entries(const MapRef &r) { return dim3(std::max(1U, r.count)); }
 
-    template <bool DIRECT> void enqueueT() {
+    // A substep in two parts. Part A runs through striker pass 2 (and, with
+    // node contact on, the staleness test of the pair list); part B runs the
+    // contact narrow phase, the end sample and the folds. With contact on the
+    // host reads the rebuild flags between them and rebuilds the pair lists
+    // that need it (CpuLatticeBackend.cpp:424-437); otherwise the parts are
+    // enqueued back to back.
+    template <bool DIRECT> void enqueueA() {
         const Step<R> S = S_;
         const bool sphere = S.sphere_enabled != 0;
         const bool plastic = shared_.plastic;
-        // With a striker, the end sample of what pass 2 cannot reach overlaps pass 2.
-        const bool split = sphere && !plastic;
+        const bool contact = S.nc.mode != 0U;
+        // With a striker, the end sample of what pass 2 cannot reach overlaps
+        // pass 2, unless node contact moves nodes between pass 2 and the sample.
+        const bool split = sphere && !plastic && !contact;
         const Batch<R> nodes = batch(node_tiles_map_), bonds = batch(bond_tiles_map_), list = batch(list_map_),
                        small = batch(small_map_), lats = batch(lattice_map_);
         const dim3 ng = tiles(node_tiles_map_), bg = tiles(bond_tiles_map_), lg = entries(list_map_),
@@ -1512,6 +1597,23 @@ private:
                        hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), 0, stream_, small, S));
         } else {
             if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
+            if (contact) BLX_LAUNCH(KContact, 14, hipLaunchKernelGGL((k_nc_stale<R>), ng, dim3(kWG), 0, stream_, nodes, S));
+        }
+        HIP_OK(hipGetLastError());
+    }
+
+    template <bool DIRECT> void enqueueB() {
+        const Step<R> S = S_;
+        const bool sphere = S.sphere_enabled != 0;
+        const bool plastic = shared_.plastic;
+        const bool contact = S.nc.mode != 0U;
+        const bool split = sphere && !plastic && !contact;
+        const Batch<R> nodes = batch(node_tiles_map_), bonds = batch(bond_tiles_map_), list = batch(list_map_),
+                       lats = batch(lattice_map_);
+        const dim3 ng = tiles(node_tiles_map_), bg = tiles(bond_tiles_map_), lg = entries(list_map_),
+                   fg = entries(lattice_map_);
+        if (!split) {
+            if (contact) BLX_LAUNCH(KContact, 15, hipLaunchKernelGGL((k_nc_narrow<R>), fg, dim3(64), 0, stream_, lats, S));
             BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAll>), ng, dim3(kWG), 0, stream_, nodes));
             profBegin(KBondEnd, 12);
             if (plastic) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true, kBondAll>), bg, dim3(kWG), 0, stream_, bonds, S);
@@ -1536,9 +1638,41 @@ private:
         HIP_OK(hipGetLastError());
     }
 
-    void enqueueSubstep() {
-        if (shared_.direct) enqueueT<true>();
-        else enqueueT<false>();
+    void enqueueSubstepA() {
+        if (shared_.direct) enqueueA<true>();
+        else enqueueA<false>();
+    }
+    void enqueueSubstepB() {
+        if (shared_.direct) enqueueB<true>();
+        else enqueueB<false>();
+    }
+
+    // The contact pair list of every running slot whose flag asks for it
+    // (LatticePhysics.hpp:1430-1561): cells, a stable sort by cell, the cell
+    // table, the per-node gather and the pair list in order.
+    void rebuildContacts() {
+        for (auto &sp : slots_) {
+            Slot<R> &sl = *sp;
+            if (!sl.armed || !sl.contact || sl.ctl_host.stop || !sl.ctl_host.nc_rebuild) continue;
+            if (sl.ctl_host.nc_error) throw Invalid("a node's contact cell is out of range (positions beyond 2^20 cells)");
+            const DevLattice<R> &L = sl.L;
+            const uint32_t g = gridFor(sl.N);
+            profBegin(KContact, 14);
+            hipLaunchKernelGGL((k_nc_cells<R>), dim3(g), dim3(kWG), 0, stream_, L, S_);
+            size_t bytes = sl.nc_temp.bytes;
+            HIP_OK(rocprim::radix_sort_pairs(sl.nc_temp.p, bytes, L.nc_key, L.nc_key_sorted, L.nc_node, L.nc_node_sorted,
+                                             sl.N, 0, 64, stream_));
+            hipLaunchKernelGGL(k_nc_table_clear, dim3(gridFor(sl.nc_table_mask + 1U)), dim3(kWG), 0, stream_,
+                               L.nc_table_key, sl.nc_table_mask + 1U);
+            hipLaunchKernelGGL((k_nc_table_insert<R>), dim3(g), dim3(kWG), 0, stream_, L);
+            hipLaunchKernelGGL((k_nc_gather<R>), dim3(g), dim3(kWG), 0, stream_, L, S_);
+            bytes = sl.nc_temp.bytes;
+            HIP_OK(rocprim::exclusive_scan(sl.nc_temp.p, bytes, L.pair_fill, L.pair_offset, 0U, sl.N,
+                                           rocprim::plus<uint32_t>(), stream_));
+            hipLaunchKernelGGL((k_nc_flatten<R>), dim3(g), dim3(kWG), 0, stream_, L);
+            profEnd();
+            HIP_OK(hipGetLastError());
+        }
     }
 };
 

@@ -75,6 +75,12 @@ struct DevCtl {
     // RunControl of the current run() call.
     u64 quiet, min_steps, no_failure, energy_flat, calm;
     double flat_fraction, calm_margin, removable;
+    // Node-node contact: the rebuild flag (upload and every failure round set
+    // it, the staleness test sets it, a rebuild clears it), the listed pair
+    // count, a cell-range error flag and the ledger (NodeContactAccumulators).
+    u32 nc_rebuild, nc_pairs, nc_error, pad1;
+    u64 nc_contacts, nc_tests, nc_rebuilds, nc_listed, nc_overflow;
+    double nc_dissipated, nc_overlap, nc_correction, nc_momentum[3];
 };
 
 // Workgroup w runs on XCD w % 8, and every XCD has its own L2. This maps the
@@ -149,6 +155,16 @@ template <class R> struct DevLattice {
     u32 *cand_lo, *cand_cnt;
     const u32 *nbb;   // node block begin
     u32 *first_fail;  // schedule indices broken in the first failure round
+    // Node-node contact, allocated when it is on: u at the last rebuild, the
+    // nodes sorted by cell, a cell table of runs in that order, each node's
+    // pairs (kPairsPerNode slots) and the pair list in order (flat_*).
+    R4<R> *nc_build;
+    u64 *nc_key, *nc_key_sorted;
+    u32 *nc_node, *nc_node_sorted;
+    u64 *nc_table_key;
+    uint2 *nc_table_val;
+    u32 nc_table_mask;
+    u32 *pair_fill, *pair_other, *pair_bond, *pair_offset, *flat_a, *flat_b, *flat_bond;
     DevCtl *ctl;
     Sphere<R> *sphere;
     double origin[3];
@@ -1607,7 +1623,10 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     sphereSupport(S, sp, ev);
     ctl->ball_support_events = ev;
     *L.sphere = sp;
-    if (any) ctl->dirty_start[step & 1U] = 1;
+    if (any) {
+        ctl->dirty_start[step & 1U] = 1;
+        ctl->nc_rebuild = 1;  // a failure round forces the contact pair list's rebuild (CpuLatticeBackend.cpp:485)
+    }
     // Run loop bookkeeping (CpuLatticeBackend.cpp:279-298).
     const u64 total = step + 1;
     ctl->step = total;
@@ -1873,6 +1892,319 @@ __global__ void k_cc_rank(const u64 *sorted, u32 *rank, u32 c) {
 
 __global__ void k_cc_label(const u32 *root, const u32 *rank, u32 *label, u32 n) {
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) label[i] = rank[root[i]];
+}
+
+// ---------------------------------------------------------------------------
+// Node-node contact (LatticePhysics.hpp:1430-1667, CpuLatticeBackend.cpp:424-437).
+//
+// Broad phase. The reference hashes cells into buckets and scans, for node i,
+// the 27 cells around it in (dz, dy, dx) order, taking in each cell the nodes
+// j > i in ascending index order. Here the nodes are sorted by cell key with
+// a stable sort (so ascending index within a cell), a table maps each
+// occupied cell to its run in that order, and the per-node gather walks the
+// same 27 cells: the emitted pair order is the reference's. The host runs a
+// rebuild when the device's flag asks for one, which it reads once a substep.
+//
+// Narrow phase. The pairs are resolved in list order. One wave evaluates 64
+// pairs at once against the current state; the pairs before the first one
+// that moves a node only count a test and an overlap (order-free), and the
+// moving pair is applied alone, after which the wave evaluates again from the
+// next pair. So every pair sees exactly the state the serial pass gives it,
+// and the ledger's sums run in pair order.
+
+constexpr u32 kPairsPerNode = 24;  // kMaxPairsPerNode (LatticePhysics.hpp:179)
+constexpr int kCellBias = 1 << 20; // cell coordinates in [-2^20, 2^20) pack in 21 bits
+constexpr u64 kNoCell = ~0ULL;
+
+template <class R> __device__ __forceinline__ R ncCutoff(const NodeContactStep<R> &c) { return R(2) * c.radius + c.skin; }
+
+__device__ __forceinline__ u64 cellKey(int x, int y, int z) {
+    return (u64(u32(z + kCellBias)) << 42) | (u64(u32(y + kCellBias)) << 21) | u64(u32(x + kCellBias));
+}
+__device__ __forceinline__ int cellCoord(u64 key, int shift) { return int(u32(key >> shift) & 0x1fffffU) - kCellBias; }
+__device__ __forceinline__ u32 cellSlot(u64 key, u32 mask) {
+    key ^= key >> 33;
+    key *= 0xff51afd7ed558ccdULL;
+    key ^= key >> 33;
+    return u32(key) & mask;
+}
+
+// Staleness (LatticePhysics.hpp:1566-1574): a node moved more than half the
+// skin since the last rebuild.
+template <class R> __global__ __launch_bounds__(kWG) void k_nc_stale(Batch<R> B, Step<R> S) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
+    if (L.ctl->stop || L.ctl->nc_rebuild) return;
+    const u32 i = m->a * kWG + threadIdx.x;
+    if (i >= L.N) return;
+    const R half = R(0.5) * S.nc.skin, limit = half * half;
+    if (length2(xyz(L.u4[i]) - xyz(L.nc_build[i])) > limit) atomicOr(&L.ctl->nc_rebuild, 1U);
+}
+
+// Step 1 (LatticePhysics.hpp:1460-1468): each node's cell and the displacement
+// the list is built at; the sort key (massless nodes sort last, as the
+// reference leaves them out of the buckets).
+template <class R> __global__ void k_nc_cells(DevLattice<R> L, Step<R> S) {
+    const R inv = R(1) / ncCutoff(S.nc);
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < L.N; i += gridDim.x * blockDim.x) {
+        const R4<R> u = L.u4[i], x0 = L.x04[i];
+        const V3<R> p = xyz(x0) + xyz(u);
+        const R fx = floor(p.x * inv), fy = floor(p.y * inv), fz = floor(p.z * inv);
+        const R lo = R(-kCellBias), hi = R(kCellBias);
+        u64 key = kNoCell;
+        if (x0.w > R(0)) {
+            if (fx >= lo && fx < hi && fy >= lo && fy < hi && fz >= lo && fz < hi)
+                key = cellKey(int(fx), int(fy), int(fz));
+            else
+                atomicOr(&L.ctl->nc_error, 1U);
+        }
+        L.nc_build[i] = u;
+        L.nc_key[i] = key;
+        L.nc_node[i] = i;
+    }
+}
+
+__global__ void k_nc_table_clear(u64 *table_key, u32 size) {
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < size; t += gridDim.x * blockDim.x) table_key[t] = kNoCell;
+}
+
+// One table entry per occupied cell: its run (start, count) in the sorted
+// order. The table is at least twice the node count, so a probe always ends.
+template <class R> __global__ void k_nc_table_insert(DevLattice<R> L) {
+    for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < L.N; k += gridDim.x * blockDim.x) {
+        const u64 key = L.nc_key_sorted[k];
+        if (key == kNoCell || (k > 0 && L.nc_key_sorted[k - 1] == key)) continue;
+        u32 count = 1;
+        while (k + count < L.N && L.nc_key_sorted[k + count] == key) ++count;
+        u32 h = cellSlot(key, L.nc_table_mask);
+        while (atomicCAS(reinterpret_cast<unsigned long long *>(&L.nc_table_key[h]), kNoCell, key) != kNoCell)
+            h = (h + 1) & L.nc_table_mask;
+        L.nc_table_val[h] = make_uint2(k, count);
+    }
+}
+
+template <class R> __device__ __forceinline__ uint2 cellRun(const DevLattice<R> &L, u64 key) {
+    u32 h = cellSlot(key, L.nc_table_mask);
+    for (;;) {
+        const u64 k = L.nc_table_key[h];
+        if (k == key) return L.nc_table_val[h];
+        if (k == kNoCell) return make_uint2(0, 0);
+        h = (h + 1) & L.nc_table_mask;
+    }
+}
+
+// Step 3 (LatticePhysics.hpp:1494-1546): the pairs node i owns, skipping a
+// pair a live bond holds; the bond found in i's slots is kept with the pair.
+template <class R> __global__ void k_nc_gather(DevLattice<R> L, Step<R> S) {
+    const R cutoff = ncCutoff(S.nc), cutoff2 = cutoff * cutoff;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < L.N; i += gridDim.x * blockDim.x) {
+        const u64 own = L.nc_key[i];
+        u32 fill = 0, over = 0;
+        if (own != kNoCell) {
+            const V3<R> p = xyz(L.x04[i]) + xyz(L.u4[i]);
+            const int cx = cellCoord(own, 0), cy = cellCoord(own, 21), cz = cellCoord(own, 42);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int gx = cx + dx, gy = cy + dy, gz = cz + dz;
+                        if (gx < -kCellBias || gx >= kCellBias || gy < -kCellBias || gy >= kCellBias ||
+                            gz < -kCellBias || gz >= kCellBias)
+                            continue;  // no node has a cell out there
+                        const uint2 run = cellRun(L, cellKey(gx, gy, gz));
+                        for (u32 k = run.x; k < run.x + run.y; ++k) {
+                            const u32 j = L.nc_node_sorted[k];
+                            if (j <= i) continue;
+                            if (length2((xyz(L.x04[j]) + xyz(L.u4[j])) - p) > cutoff2) continue;
+                            u32 bond = kNone;
+                            bool held = false;
+                            for (u32 q = 0; q < L.D; ++q) {
+                                const u32 key = L.nbr_key[size_t(q) * L.N + i];
+                                if (key == kNone) break;
+                                if (u32(int(i) + L.sdelta[key & ~kDeadSlot]) != j) continue;
+                                bond = L.bnd[L.nbr_bond[size_t(q) * L.N + i]].j;
+                                held = !(key & kDeadSlot);
+                                break;
+                            }
+                            if (held) continue;
+                            if (fill >= kPairsPerNode) {
+                                ++over;
+                                continue;
+                            }
+                            L.pair_other[size_t(kPairsPerNode) * i + fill] = j;
+                            L.pair_bond[size_t(kPairsPerNode) * i + fill] = bond;
+                            ++fill;
+                        }
+                    }
+        }
+        L.pair_fill[i] = fill;
+        if (over) atomicAdd(reinterpret_cast<unsigned long long *>(&L.ctl->nc_overflow), (unsigned long long)over);
+    }
+}
+
+// Step 4 (LatticePhysics.hpp:1551-1561): the pairs in list order (owner
+// ascending, then slot), from the exclusive scan of the fills; the last node
+// books the rebuild.
+template <class R> __global__ void k_nc_flatten(DevLattice<R> L) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < L.N; i += gridDim.x * blockDim.x) {
+        const u32 off = L.pair_offset[i], fill = L.pair_fill[i];
+        for (u32 q = 0; q < fill; ++q) {
+            L.flat_a[off + q] = i;
+            L.flat_b[off + q] = L.pair_other[size_t(kPairsPerNode) * i + q];
+            L.flat_bond[off + q] = L.pair_bond[size_t(kPairsPerNode) * i + q];
+        }
+        if (i + 1 == L.N) {
+            DevCtl *ctl = L.ctl;
+            ctl->nc_pairs = off + fill;
+            ctl->nc_listed += off + fill;
+            ctl->nc_rebuilds += 1;
+            ctl->nc_rebuild = 0;
+        }
+    }
+}
+
+// One pair (LatticePhysics.hpp:1585-1648), evaluated on loaded values. Returns
+// whether the pair moves a node (an impulse or a correction), and fills in
+// what it would write.
+template <class R> struct PairEval {
+    bool valid, acts, impulse, correct;
+    R gap;
+    V3<R> va, vb, ua, ub;
+    R before_after, momentum[3], correction;
+};
+
+template <class R>
+__device__ __forceinline__ PairEval<R> evalPair(const Step<R> &S, const R4<R> &xa, const R4<R> &xb, R4<R> ua4, R4<R> ub4,
+                                                R4<R> va4, R4<R> vb4, bool bond_live) {
+    const NodeContactStep<R> &c = S.nc;
+    PairEval<R> e{};
+    const R ma = xa.w, mb = xb.w;
+    e.valid = !bond_live && ma > R(0) && mb > R(0);
+    if (!e.valid) return e;
+    const V3<R> r = (xyz(xb) + xyz(ub4)) - (xyz(xa) + xyz(ua4));
+    const R distance = length(r);
+    const V3<R> normal = normalized(r);
+    e.gap = distance - R(2) * c.radius;
+    if (c.mode != 2U) return e;
+    V3<R> va = xyz(va4), vb = xyz(vb4);
+    const R vn = dot(vb - va, normal);
+    if (e.gap > maxR(c.margin, -vn * S.dt)) return e;
+    const R inverse_a = R(1) / ma, inverse_b = R(1) / mb;
+    const R inverse_pair = inverse_a + inverse_b;
+    const R restitution = -vn > c.rest_thr ? c.restitution : R(0);
+    const R desired = e.gap > c.margin ? -e.gap / S.dt : -restitution * minR(vn, R(0));
+    const R normal_impulse = maxR(R(0), (desired - vn) / inverse_pair);
+    if (normal_impulse > R(0)) {
+        const V3<R> va0 = va, vb0 = vb;
+        const R before = R(0.5) * ma * length2(va) + R(0.5) * mb * length2(vb);
+        V3<R> imp = normal_impulse * normal;
+        vb = vb + inverse_b * imp;
+        va = va - inverse_a * imp;
+        const V3<R> relative = vb - va;
+        const V3<R> tangent = relative - dot(relative, normal) * normal;
+        const R tangent_speed = length(tangent);
+        if (tangent_speed > R(1.0e-12)) {
+            const V3<R> direction = tangent / tangent_speed;
+            const R sticking = tangent_speed / inverse_pair;
+            const R friction = sticking <= c.sf * normal_impulse ? sticking : minR(sticking, c.df * normal_impulse);
+            imp = -friction * direction;
+            vb = vb + inverse_b * imp;
+            va = va - inverse_a * imp;
+        }
+        e.impulse = true;
+        e.va = va;
+        e.vb = vb;
+        e.before_after = before - (R(0.5) * ma * length2(va) + R(0.5) * mb * length2(vb));
+        e.momentum[0] = ma * (va.x - va0.x) + mb * (vb.x - vb0.x);
+        e.momentum[1] = ma * (va.y - va0.y) + mb * (vb.y - vb0.y);
+        e.momentum[2] = ma * (va.z - va0.z) + mb * (vb.z - vb0.z);
+    }
+    if (e.gap < -c.margin) {
+        e.correction = minR(-e.gap - c.margin, R(0.2) * c.radius);
+        const V3<R> shift = (e.correction / inverse_pair) * normal;
+        e.ua = xyz(ua4) - inverse_a * shift;
+        e.ub = xyz(ub4) + inverse_b * shift;
+        e.correct = true;
+    }
+    e.acts = e.impulse || e.correct;
+    return e;
+}
+
+__device__ __forceinline__ double waveMaxD(double x) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double y = __shfl_xor(x, o);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// The narrow phase of one lattice per 64-lane workgroup (LatticePhysics.hpp:1655-1667).
+template <class R> __global__ __launch_bounds__(64) void k_nc_narrow(Batch<R> B, Step<R> S) {
+    const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
+    DevCtl *ctl = L.ctl;
+    if (ctl->stop) return;
+    const u32 P = ctl->nc_pairs, lane = threadIdx.x;
+    u64 tests = 0, contacts = 0;
+    double overlap = ctl->nc_overlap, corr = ctl->nc_correction, diss = ctl->nc_dissipated;
+    double mom[3] = {ctl->nc_momentum[0], ctl->nc_momentum[1], ctl->nc_momentum[2]};
+    for (u32 p0 = 0; p0 < P;) {
+        const u32 p = p0 + lane;
+        const bool in = p < P;
+        u32 a = 0, b = 0, bond = kNone;
+        if (in) {
+            a = L.flat_a[p];
+            b = L.flat_b[p];
+            bond = L.flat_bond[p];
+        }
+        PairEval<R> e{};
+        if (in) {
+            const bool live = bond != kNone && L.alive[bond];
+            e = evalPair(S, L.x04[a], L.x04[b], L.u4[a], L.u4[b], L.v4[a], L.v4[b], live);
+        }
+        const u64 actm = __ballot(in && e.valid && e.acts);
+        const u32 w = actm ? u32(__ffsll((long long)actm) - 1) : 64U;
+        // Pairs up to and including the first mover: a test each and the overlap.
+        const bool counted = in && e.valid && lane <= w;
+        tests += u64(__popcll(__ballot(counted)));
+        overlap = waveMaxD(counted && e.gap < R(0) && double(-e.gap) > overlap ? double(-e.gap) : overlap);
+        if (w == 64U) {
+            p0 += 64U;
+            continue;
+        }
+        if (lane == w) {
+            if (e.impulse) {
+                L.v4[a] = with(e.va, L.v4[a].w);
+                L.v4[b] = with(e.vb, L.v4[b].w);
+                diss += double(e.before_after);
+                mom[0] += double(e.momentum[0]);
+                mom[1] += double(e.momentum[1]);
+                mom[2] += double(e.momentum[2]);
+                contacts += 1;
+            }
+            if (e.correct) {
+                L.u4[a] = with(e.ua, L.u4[a].w);
+                L.u4[b] = with(e.ub, L.u4[b].w);
+                if (e.correction > R(corr)) corr = double(e.correction);
+            }
+        }
+        diss = laneValue(diss, w);
+        mom[0] = laneValue(mom[0], w);
+        mom[1] = laneValue(mom[1], w);
+        mom[2] = laneValue(mom[2], w);
+        corr = laneValue(corr, w);
+        contacts = u64(__builtin_amdgcn_readlane(int(contacts), int(w)));
+        // The mover's writes are seen by the next evaluation of this wave.
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        p0 += w + 1U;
+    }
+    if (lane == 0) {
+        ctl->nc_tests += tests;
+        ctl->nc_contacts += contacts;
+        ctl->nc_overlap = overlap;
+        ctl->nc_correction = corr;
+        ctl->nc_dissipated = diss;
+        for (int k = 0; k < 3; ++k) ctl->nc_momentum[k] = mom[k];
+    }
 }
 
 // A plain streaming copy, 16 bytes per lane per iteration: the known-size

@@ -139,10 +139,11 @@ def test_refusals(gpu):
         setattr(s, field, value)
         with pytest.raises(ValueError):
             be.upload(state, s, sc.sphere)
-    s = scenes.settings(sc.settings.dt)
-    s.node_contact.mode = 2
-    with pytest.raises(ValueError):
-        be.upload(state, s, sc.sphere)
+    for mode, mask in ((3, 63), (2, 5)):  # an unknown contact mode; a bucket mask that is no power of two minus one
+        s = scenes.settings(sc.settings.dt)
+        s.node_contact.mode, s.node_contact.bucket_mask = mode, mask
+        with pytest.raises(ValueError):
+            be.upload(state, s, sc.sphere)
     with pytest.raises(AssertionError):
         be.run(control(1))  # run before a successful upload: std::logic_error
     be.upload(state, sc.settings, sc.sphere)
