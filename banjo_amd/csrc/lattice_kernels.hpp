@@ -2138,7 +2138,15 @@ __device__ __forceinline__ double waveMaxD(double x) {
     return x;
 }
 
+template <class R> __device__ __forceinline__ R4<R> laneR4(const R4<R> &v, u32 lane) {
+    return {laneValue(v.x, lane), laneValue(v.y, lane), laneValue(v.z, lane), laneValue(v.w, lane)};
+}
+
 // The narrow phase of one lattice per 64-lane workgroup (LatticePhysics.hpp:1655-1667).
+// A window of 64 pairs loads its node states once. The movers of the window
+// are applied in order; each broadcasts its nodes' new states, and the later
+// pairs of the window that share a node take them and evaluate again, so
+// every pair is evaluated on exactly the state the serial pass gives it.
 template <class R> __global__ __launch_bounds__(64) void k_nc_narrow(Batch<R> B, Step<R> S) {
     const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
     DevCtl *ctl = L.ctl;
@@ -2147,55 +2155,100 @@ template <class R> __global__ __launch_bounds__(64) void k_nc_narrow(Batch<R> B,
     u64 tests = 0, contacts = 0;
     double overlap = ctl->nc_overlap, corr = ctl->nc_correction, diss = ctl->nc_dissipated;
     double mom[3] = {ctl->nc_momentum[0], ctl->nc_momentum[1], ctl->nc_momentum[2]};
-    for (u32 p0 = 0; p0 < P;) {
+    // A window's pair indices, bond flags and rest positions do not change in
+    // the narrow phase: the next window's are loaded while this one runs.
+    u32 na = kNone, nb = kNone;
+    bool nlive = false;
+    R4<R> nxa{}, nxb{};
+    auto loadStatic = [&](u32 q) {
+        if (q < P) {
+            na = L.flat_a[q];
+            nb = L.flat_b[q];
+            const u32 bond = L.flat_bond[q];
+            nlive = bond != kNone && L.alive[bond];
+            nxa = L.x04[na];
+            nxb = L.x04[nb];
+        }
+    };
+    loadStatic(lane);
+    for (u32 p0 = 0; p0 < P; p0 += 64U) {
         const u32 p = p0 + lane;
         const bool in = p < P;
-        u32 a = 0, b = 0, bond = kNone;
+        const u32 a = na, b = nb;
+        const bool live = nlive;
+        const R4<R> xa = nxa, xb = nxb;
+        R4<R> ua{}, ub{}, va{}, vb{};
         if (in) {
-            a = L.flat_a[p];
-            b = L.flat_b[p];
-            bond = L.flat_bond[p];
+            ua = L.u4[a];
+            ub = L.u4[b];
+            va = L.v4[a];
+            vb = L.v4[b];
         }
+        loadStatic(p + 64U);
         PairEval<R> e{};
-        if (in) {
-            const bool live = bond != kNone && L.alive[bond];
-            e = evalPair(S, L.x04[a], L.x04[b], L.u4[a], L.u4[b], L.v4[a], L.v4[b], live);
-        }
-        const u64 actm = __ballot(in && e.valid && e.acts);
-        const u32 w = actm ? u32(__ffsll((long long)actm) - 1) : 64U;
-        // Pairs up to and including the first mover: a test each and the overlap.
-        const bool counted = in && e.valid && lane <= w;
-        tests += u64(__popcll(__ballot(counted)));
-        overlap = waveMaxD(counted && e.gap < R(0) && double(-e.gap) > overlap ? double(-e.gap) : overlap);
-        if (w == 64U) {
-            p0 += 64U;
-            continue;
-        }
-        if (lane == w) {
+        if (in) e = evalPair(S, xa, xb, ua, ub, va, vb, live);
+        u32 done = 0;  // lanes below are final
+        bool fin_valid = false;  // this lane's final evaluation: a test, and its gap
+        R fin_gap = R(0);
+        for (;;) {
+            const u64 actm = __ballot(in && e.valid && e.acts && lane >= done);
+            const u32 m = actm ? u32(__ffsll((long long)actm) - 1) : 64U;
+            // Pairs done..m are final.
+            if (in && lane >= done && lane <= m) {
+                fin_valid = e.valid;
+                fin_gap = e.gap;
+            }
+            if (m == 64U) break;
+            // The mover's new node states, then its writes and its ledger terms.
+            R4<R> na_u = ua, na_v = va, nb_u = ub, nb_v = vb;
             if (e.impulse) {
-                L.v4[a] = with(e.va, L.v4[a].w);
-                L.v4[b] = with(e.vb, L.v4[b].w);
-                diss += double(e.before_after);
-                mom[0] += double(e.momentum[0]);
-                mom[1] += double(e.momentum[1]);
-                mom[2] += double(e.momentum[2]);
-                contacts += 1;
+                na_v = with(e.va, va.w);
+                nb_v = with(e.vb, vb.w);
             }
             if (e.correct) {
-                L.u4[a] = with(e.ua, L.u4[a].w);
-                L.u4[b] = with(e.ub, L.u4[b].w);
-                if (e.correction > R(corr)) corr = double(e.correction);
+                na_u = with(e.ua, ua.w);
+                nb_u = with(e.ub, ub.w);
             }
+            if (lane == m) {
+                if (e.impulse) {
+                    L.v4[a] = na_v;
+                    L.v4[b] = nb_v;
+                    diss += double(e.before_after);
+                    mom[0] += double(e.momentum[0]);
+                    mom[1] += double(e.momentum[1]);
+                    mom[2] += double(e.momentum[2]);
+                    contacts += 1;
+                }
+                if (e.correct) {
+                    L.u4[a] = na_u;
+                    L.u4[b] = nb_u;
+                    if (e.correction > R(corr)) corr = double(e.correction);
+                }
+            }
+            diss = laneValue(diss, m);
+            mom[0] = laneValue(mom[0], m);
+            mom[1] = laneValue(mom[1], m);
+            mom[2] = laneValue(mom[2], m);
+            corr = laneValue(corr, m);
+            contacts = u64(__builtin_amdgcn_readlane(int(contacts), int(m)));
+            const u32 am = u32(__builtin_amdgcn_readlane(int(a), int(m)));
+            const u32 bm = u32(__builtin_amdgcn_readlane(int(b), int(m)));
+            const R4<R> mau = laneR4(na_u, m), mav = laneR4(na_v, m), mbu = laneR4(nb_u, m), mbv = laneR4(nb_v, m);
+            bool changed = false;
+            if (in && lane > m) {
+                if (a == am) { ua = mau; va = mav; changed = true; }
+                else if (a == bm) { ua = mbu; va = mbv; changed = true; }
+                if (b == am) { ub = mau; vb = mav; changed = true; }
+                else if (b == bm) { ub = mbu; vb = mbv; changed = true; }
+            }
+            if (changed) e = evalPair(S, xa, xb, ua, ub, va, vb, live);
+            done = m + 1U;
         }
-        diss = laneValue(diss, w);
-        mom[0] = laneValue(mom[0], w);
-        mom[1] = laneValue(mom[1], w);
-        mom[2] = laneValue(mom[2], w);
-        corr = laneValue(corr, w);
-        contacts = u64(__builtin_amdgcn_readlane(int(contacts), int(w)));
-        // The mover's writes are seen by the next evaluation of this wave.
+        // The window's tests and worst overlap, each pair on its final state.
+        tests += u64(__popcll(__ballot(fin_valid)));
+        overlap = waveMaxD(fin_valid && fin_gap < R(0) && double(-fin_gap) > overlap ? double(-fin_gap) : overlap);
+        // The movers' writes are seen by the next window's loads.
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        p0 += w + 1U;
     }
     if (lane == 0) {
         ctl->nc_tests += tests;

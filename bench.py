@@ -28,6 +28,7 @@ Printed JSON carries, besides the rate:
                 batch (trials/s, p50/p99 latency);
   cut           a 256 x 32 x 32 plastic plank cut by a kinematic blade, with
                 fragment labels after every 12-substep step;
+  contact       the cut with node-node contact on, against off, over a window;
   c1            the 16^3 cantilever of config 1, run whole.
 The CPU baseline also carries, under "more", the reference's threaded and
 serial CPU lanes on config 1, config 2 and the plank.
@@ -332,6 +333,44 @@ def copy_probe(nbytes, repeats):
             "gbps_read_plus_write": 2 * nbytes / secs.value / 1e9}
 
 
+def contact_leg(args):
+    """Node-node contact on the cut plank: the same plank and blade with
+    contact on (radius half a cell, skin a quarter), so the crack faces and
+    the blade's two halves do not pass through each other; the same window
+    without contact beside it."""
+    from banjo_amd import abi, scenes
+    from banjo_amd.lattice import control, product
+    from banjo_amd.runner import prepare
+
+    hip = product()
+    steps, per = args.contact_steps, 12
+    out = {}
+    for label, mode in (("without", 0), ("with", 2)):
+        sc = scenes.config5(substeps=steps * per)
+        if mode:
+            scenes.with_node_contact(sc.settings, sc.cell, sc.node_count, mode)
+        sched, state = prepare(hip, sc)
+        be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+        be.upload(state, sc.settings, sc.sphere)
+        be.set_blade(sc.info["blade"])
+        t0 = time.perf_counter()
+        st = be.run(control(steps * per))
+        secs = time.perf_counter() - t0
+        be.close()
+        nc = st.node_contact
+        out[label] = {"seconds": secs, "bond_solves_per_s": sc.bond_count * st.total_steps / secs,
+                      "broken_bonds": st.broken_bonds}
+        if mode:
+            out[label].update({"rebuilds": nc.rebuilds, "pairs_listed": nc.pairs_listed, "pair_tests": nc.pair_tests,
+                               "contacts": nc.contacts, "pair_overflow": nc.pair_overflow,
+                               "maximum_overlap_m": nc.maximum_overlap_m,
+                               "maximum_position_correction_m": nc.maximum_position_correction_m})
+    out["substeps"] = steps * per
+    out["workload"] = ("the 256x32x32 plastic plank cut by the blade (as the cut form), first "
+                       f"{steps * per} substeps, node contact on against off; no labels")
+    return out
+
+
 def c1_leg(args):
     """Config 1 on the GPU: the 16^3 cantilever, 1,920 substeps, run whole."""
     from banjo_amd import abi, scenes
@@ -422,11 +461,12 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-quick", action="store_true", help="only the config-2 window of the CPU baseline")
-    ap.add_argument("--forms", default="plain,batched,served,cut,c1",
+    ap.add_argument("--forms", default="plain,batched,served,cut,contact,c1",
                     help="comma list of plain (the headline impact), batched (many uneven lattices), "
                          "served (a stream of trials), cut (a plank cut by a blade, labels every step), "
-                         "c1 (the 16^3 cantilever in full)")
+                         "contact (the cut with node-node contact on), c1 (the 16^3 cantilever in full)")
     ap.add_argument("--cut-steps", type=int, default=600)
+    ap.add_argument("--contact-steps", type=int, default=120)
     ap.add_argument("--batch-trials", type=int, default=512)
     ap.add_argument("--batch-substeps", type=int, default=8 * 240)
     ap.add_argument("--batch-steps-per-launch", type=int, default=0)
@@ -457,6 +497,8 @@ def main():
             out["served"] = served_leg(args)
         if "cut" in forms:
             out["cut"] = cut_leg(args)
+        if "contact" in forms:
+            out["contact"] = contact_leg(args)
         if "c1" in forms:
             out["c1"] = c1_leg(args)
         print(json.dumps(out))
@@ -511,6 +553,8 @@ def main():
         out["served"] = served_leg(args)
     if "cut" in forms:
         out["cut"] = cut_leg(args)
+    if "contact" in forms:
+        out["contact"] = contact_leg(args)
     if "c1" in forms:
         out["c1"] = c1_leg(args)
     print(json.dumps(out))
