@@ -285,6 +285,7 @@ public:
         size_t nbb, first_fail;
         size_t nc_build, nc_key, nc_key_sorted, nc_node, nc_node_sorted, nc_table_key, nc_table_val;
         size_t pair_fill, pair_other, pair_bond, pair_offset, flat_a, flat_b, flat_bond;
+        size_t nc_partners, nc_done, nc_rank_b, nc_sort_key, nc_sort_in, nc_sort_out, nc_terms, nc_mover;
     } off{};
     size_t used = 0;  // arena bytes the arrays occupy
     Arena arena;
@@ -393,6 +394,14 @@ public:
         l.flat_a = arena.at<uint32_t>(o.flat_a);
         l.flat_b = arena.at<uint32_t>(o.flat_b);
         l.flat_bond = arena.at<uint32_t>(o.flat_bond);
+        l.nc_partners = arena.at<uint32_t>(o.nc_partners);
+        l.nc_done = arena.at<uint32_t>(o.nc_done);
+        l.nc_rank_b = arena.at<uint32_t>(o.nc_rank_b);
+        l.nc_sort_key = arena.at<uint32_t>(o.nc_sort_key);
+        l.nc_sort_in = arena.at<uint32_t>(o.nc_sort_in);
+        l.nc_sort_out = arena.at<uint32_t>(o.nc_sort_out);
+        l.nc_terms = arena.at<R4<R>>(o.nc_terms);
+        l.nc_mover = arena.at<uint8_t>(o.nc_mover);
         l.ctl = ctl_dev;
         l.sphere = sphere_dev;
         for (int k = 0; k < 3; ++k) l.origin[k] = origin[k];
@@ -634,6 +643,14 @@ public:
         o.flat_a = arena.reserve<uint32_t>(nP);
         o.flat_b = arena.reserve<uint32_t>(nP);
         o.flat_bond = arena.reserve<uint32_t>(nP);
+        o.nc_partners = arena.reserve<uint32_t>(nN);
+        o.nc_done = arena.reserve<uint32_t>(nN);
+        o.nc_rank_b = arena.reserve<uint32_t>(nP);
+        o.nc_sort_key = arena.reserve<uint32_t>(nP);
+        o.nc_sort_in = arena.reserve<uint32_t>(nP);
+        o.nc_sort_out = arena.reserve<uint32_t>(nP);
+        o.nc_terms = arena.reserve<R4<R>>(nP);
+        o.nc_mover = arena.reserve<uint8_t>(nP);
         arena.commit();
         if (contact) {
             size_t sort_bytes = 0, scan_bytes = 0;
@@ -642,7 +659,11 @@ public:
                                              static_cast<uint32_t *>(nullptr), N, 0, 64, stream));
             HIP_OK(rocprim::exclusive_scan(nullptr, scan_bytes, static_cast<uint32_t *>(nullptr),
                                            static_cast<uint32_t *>(nullptr), 0U, N, rocprim::plus<uint32_t>(), stream));
-            nc_temp.alloc(std::max(sort_bytes, scan_bytes));
+            size_t pair_bytes = 0;
+            HIP_OK(rocprim::radix_sort_pairs(nullptr, pair_bytes, static_cast<uint32_t *>(nullptr),
+                                             static_cast<uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
+                                             static_cast<uint32_t *>(nullptr), size_t(kPairsPerNode) * N, 0, 32, stream));
+            nc_temp.alloc(std::max({sort_bytes, scan_bytes, pair_bytes}));
         }
 
         // Zero everything, then copy the data in.
@@ -1613,7 +1634,8 @@ private:
         const dim3 ng = tiles(node_tiles_map_), bg = tiles(bond_tiles_map_), lg = entries(list_map_),
                    fg = entries(lattice_map_);
         if (!split) {
-            if (contact) BLX_LAUNCH(KContact, 15, hipLaunchKernelGGL((k_nc_narrow<R>), fg, dim3(64), 0, stream_, lats, S));
+            if (contact)
+                BLX_LAUNCH(KContact, 15, hipLaunchKernelGGL((k_nc_narrow<R>), fg, dim3(kNarrowThreads), 0, stream_, lats, S));
             BLX_LAUNCH(KStrain, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAll>), ng, dim3(kWG), 0, stream_, nodes));
             profBegin(KBondEnd, 12);
             if (plastic) hipLaunchKernelGGL((k_bond_end<R, DIRECT, true, kBondAll>), bg, dim3(kWG), 0, stream_, bonds, S);
@@ -1651,8 +1673,8 @@ private:
     // (LatticePhysics.hpp:1430-1561): cells, a stable sort by cell, the cell
     // table, the per-node gather and the pair list in order.
     void rebuildContacts() {
-        for (auto &sp : slots_) {
-            Slot<R> &sl = *sp;
+        for (uint32_t k = 0; k < slots_.size(); ++k) {
+            Slot<R> &sl = *slots_[k];
             if (!sl.armed || !sl.contact || sl.ctl_host.stop || !sl.ctl_host.nc_rebuild) continue;
             if (sl.ctl_host.nc_error) throw Invalid("a node's contact cell is out of range (positions beyond 2^20 cells)");
             const DevLattice<R> &L = sl.L;
@@ -1670,6 +1692,18 @@ private:
             HIP_OK(rocprim::exclusive_scan(sl.nc_temp.p, bytes, L.pair_fill, L.pair_offset, 0U, sl.N,
                                            rocprim::plus<uint32_t>(), stream_));
             hipLaunchKernelGGL((k_nc_flatten<R>), dim3(g), dim3(kWG), 0, stream_, L);
+            // The narrow phase's schedule: pair ranks at their partners, and each
+            // node's count of pairs naming it as partner.
+            HIP_OK(hipStreamSynchronize(stream_));
+            const uint32_t P = readCtl(k).nc_pairs;
+            hipLaunchKernelGGL(k_nc_clear, dim3(g), dim3(kWG), 0, stream_, L.nc_partners, sl.N);
+            if (P > 0) {
+                hipLaunchKernelGGL(k_nc_iota, dim3(gridFor(P)), dim3(kWG), 0, stream_, L.nc_sort_in, P);
+                bytes = sl.nc_temp.bytes;
+                HIP_OK(rocprim::radix_sort_pairs(sl.nc_temp.p, bytes, L.flat_b, L.nc_sort_key, L.nc_sort_in,
+                                                 L.nc_sort_out, P, 0, 32, stream_));
+                hipLaunchKernelGGL((k_nc_ranks<R>), dim3(gridFor(P)), dim3(kWG), 0, stream_, L, P);
+            }
             profEnd();
             HIP_OK(hipGetLastError());
         }

@@ -165,6 +165,12 @@ template <class R> struct DevLattice {
     uint2 *nc_table_val;
     u32 nc_table_mask;
     u32 *pair_fill, *pair_other, *pair_bond, *pair_offset, *flat_a, *flat_b, *flat_bond;
+    // The narrow phase's schedule: per node, how many pairs name it as partner
+    // and how many of its pairs are done; per pair, its rank among the pairs
+    // with the same partner; the sort by partner; and each pair's ledger terms.
+    u32 *nc_partners, *nc_done, *nc_rank_b, *nc_sort_key, *nc_sort_in, *nc_sort_out;
+    R4<R> *nc_terms;
+    u8 *nc_mover;
     DevCtl *ctl;
     Sphere<R> *sphere;
     double origin[3];
@@ -2138,123 +2144,165 @@ __device__ __forceinline__ double waveMaxD(double x) {
     return x;
 }
 
-template <class R> __device__ __forceinline__ R4<R> laneR4(const R4<R> &v, u32 lane) {
-    return {laneValue(v.x, lane), laneValue(v.y, lane), laneValue(v.z, lane), laneValue(v.w, lane)};
+// The narrow phase's schedule. A pair must see the state every earlier pair
+// sharing one of its nodes leaves, and nothing else orders it. A node's pairs
+// in list order are first those that name it as partner (their owners come
+// earlier), then its own block; so a pair's rank at its owner a is
+// partners(a) + its slot in a's block, and its rank at its partner b is its
+// position among the pairs naming b, from a stable sort by partner.
+__global__ void k_nc_iota(u32 *out, u32 n) {
+    for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = k;
 }
 
-// The narrow phase of one lattice per 64-lane workgroup (LatticePhysics.hpp:1655-1667).
-// A window of 64 pairs loads its node states once. The movers of the window
-// are applied in order; each broadcasts its nodes' new states, and the later
-// pairs of the window that share a node take them and evaluate again, so
-// every pair is evaluated on exactly the state the serial pass gives it.
-template <class R> __global__ __launch_bounds__(64) void k_nc_narrow(Batch<R> B, Step<R> S) {
+template <class R> __global__ void k_nc_ranks(DevLattice<R> L, u32 P) {
+    for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
+        const u32 b = L.nc_sort_key[k];
+        u32 s0 = k;
+        while (s0 > 0 && L.nc_sort_key[s0 - 1] == b) --s0;
+        L.nc_rank_b[L.nc_sort_out[k]] = k - s0;
+        if (s0 == k) {
+            u32 e = k + 1;
+            while (e < P && L.nc_sort_key[e] == b) ++e;
+            L.nc_partners[b] = e - k;
+        }
+    }
+}
+
+__global__ void k_nc_clear(u32 *a, u32 n) {
+    for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) a[k] = 0;
+}
+
+constexpr u32 kNarrowThreads = 1024;
+constexpr u32 kNarrowSpinLimit = 1U << 22;  // polls of one pair before the phase gives up with an error
+
+// The narrow phase of one lattice per workgroup (LatticePhysics.hpp:1655-1667).
+// Every lane takes a pair and applies it once the pairs before it at both
+// its nodes are done (per-node counters, workgroup-scope acquire/release).
+// Pairs that run at the same time share no node, so each sees the serial
+// pass's state. The ledger's sums then run in pair order over the recorded
+// terms; the counts and maxima do not depend on the order.
+template <class R> __global__ __launch_bounds__(kNarrowThreads) void k_nc_narrow(Batch<R> B, Step<R> S) {
     const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
-    const u32 P = ctl->nc_pairs, lane = threadIdx.x;
+    __shared__ u32 s_abort, s_waves[kNarrowThreads / 64];
+    __shared__ R4<R> s_terms[kNarrowThreads];
+    __shared__ double s_max[2][kNarrowThreads / 64];
+    __shared__ unsigned long long s_count[2];
+    const u32 P = ctl->nc_pairs, tid = threadIdx.x;
+    if (tid == 0) {
+        s_abort = 0;
+        s_count[0] = s_count[1] = 0;
+    }
+    for (u32 p = tid; p < P; p += kNarrowThreads) {
+        L.nc_done[L.flat_a[p]] = 0;
+        L.nc_done[L.flat_b[p]] = 0;
+    }
+    __syncthreads();
     u64 tests = 0, contacts = 0;
-    double overlap = ctl->nc_overlap, corr = ctl->nc_correction, diss = ctl->nc_dissipated;
-    double mom[3] = {ctl->nc_momentum[0], ctl->nc_momentum[1], ctl->nc_momentum[2]};
-    // A window's pair indices, bond flags and rest positions do not change in
-    // the narrow phase: the next window's are loaded while this one runs.
-    u32 na = kNone, nb = kNone;
-    bool nlive = false;
-    R4<R> nxa{}, nxb{};
-    auto loadStatic = [&](u32 q) {
-        if (q < P) {
-            na = L.flat_a[q];
-            nb = L.flat_b[q];
-            const u32 bond = L.flat_bond[q];
-            nlive = bond != kNone && L.alive[bond];
-            nxa = L.x04[na];
-            nxb = L.x04[nb];
+    double overlap = 0.0, corr = 0.0;
+    for (u32 base = 0; base < P; base += kNarrowThreads) {
+        const u32 p = base + tid;
+        bool pending = p < P;
+        u32 a = 0, b = 0, ra = 0, rb = 0;
+        bool live = false;
+        R4<R> xa{}, xb{};
+        if (pending) {
+            a = L.flat_a[p];
+            b = L.flat_b[p];
+            const u32 bond = L.flat_bond[p];
+            live = bond != kNone && L.alive[bond];
+            ra = L.nc_partners[a] + (p - L.pair_offset[a]);
+            rb = L.nc_rank_b[p];
+            xa = L.x04[a];
+            xb = L.x04[b];
         }
-    };
-    loadStatic(lane);
-    for (u32 p0 = 0; p0 < P; p0 += 64U) {
-        const u32 p = p0 + lane;
-        const bool in = p < P;
-        const u32 a = na, b = nb;
-        const bool live = nlive;
-        const R4<R> xa = nxa, xb = nxb;
-        R4<R> ua{}, ub{}, va{}, vb{};
-        if (in) {
-            ua = L.u4[a];
-            ub = L.u4[b];
-            va = L.v4[a];
-            vb = L.v4[b];
-        }
-        loadStatic(p + 64U);
-        PairEval<R> e{};
-        if (in) e = evalPair(S, xa, xb, ua, ub, va, vb, live);
-        u32 done = 0;  // lanes below are final
-        bool fin_valid = false;  // this lane's final evaluation: a test, and its gap
-        R fin_gap = R(0);
-        for (;;) {
-            const u64 actm = __ballot(in && e.valid && e.acts && lane >= done);
-            const u32 m = actm ? u32(__ffsll((long long)actm) - 1) : 64U;
-            // Pairs done..m are final.
-            if (in && lane >= done && lane <= m) {
-                fin_valid = e.valid;
-                fin_gap = e.gap;
-            }
-            if (m == 64U) break;
-            // The mover's new node states, then its writes and its ledger terms.
-            R4<R> na_u = ua, na_v = va, nb_u = ub, nb_v = vb;
-            if (e.impulse) {
-                na_v = with(e.va, va.w);
-                nb_v = with(e.vb, vb.w);
-            }
-            if (e.correct) {
-                na_u = with(e.ua, ua.w);
-                nb_u = with(e.ub, ub.w);
-            }
-            if (lane == m) {
+        u32 spins = 0;
+        while (__ballot(pending)) {
+            if (!pending) continue;
+            const u32 da = __hip_atomic_load(&L.nc_done[a], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const u32 db = __hip_atomic_load(&L.nc_done[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (da == ra && db == rb) {
+                const R4<R> ua = L.u4[a], ub = L.u4[b], va = L.v4[a], vb = L.v4[b];
+                const PairEval<R> e = evalPair(S, xa, xb, ua, ub, va, vb, live);
+                u8 mover = 0;
+                if (e.valid) {
+                    ++tests;
+                    if (e.gap < R(0) && double(-e.gap) > overlap) overlap = double(-e.gap);
+                }
                 if (e.impulse) {
-                    L.v4[a] = na_v;
-                    L.v4[b] = nb_v;
-                    diss += double(e.before_after);
-                    mom[0] += double(e.momentum[0]);
-                    mom[1] += double(e.momentum[1]);
-                    mom[2] += double(e.momentum[2]);
-                    contacts += 1;
+                    L.v4[a] = with(e.va, va.w);
+                    L.v4[b] = with(e.vb, vb.w);
+                    L.nc_terms[p] = R4<R>{e.before_after, e.momentum[0], e.momentum[1], e.momentum[2]};
+                    mover = 1;
+                    ++contacts;
                 }
                 if (e.correct) {
-                    L.u4[a] = na_u;
-                    L.u4[b] = nb_u;
-                    if (e.correction > R(corr)) corr = double(e.correction);
+                    L.u4[a] = with(e.ua, ua.w);
+                    L.u4[b] = with(e.ub, ub.w);
+                    if (double(e.correction) > corr) corr = double(e.correction);
                 }
+                L.nc_mover[p] = mover;
+                __hip_atomic_fetch_add(&L.nc_done[a], 1U, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&L.nc_done[b], 1U, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                pending = false;
+            } else if (++spins > kNarrowSpinLimit ||
+                       __hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                // A pair that never became ready is a schedule error: stop every
+                // lane, and report it (the host raises it).
+                __hip_atomic_store(&s_abort, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                atomicOr(&ctl->nc_error, 2U);
+                pending = false;
             }
-            diss = laneValue(diss, m);
-            mom[0] = laneValue(mom[0], m);
-            mom[1] = laneValue(mom[1], m);
-            mom[2] = laneValue(mom[2], m);
-            corr = laneValue(corr, m);
-            contacts = u64(__builtin_amdgcn_readlane(int(contacts), int(m)));
-            const u32 am = u32(__builtin_amdgcn_readlane(int(a), int(m)));
-            const u32 bm = u32(__builtin_amdgcn_readlane(int(b), int(m)));
-            const R4<R> mau = laneR4(na_u, m), mav = laneR4(na_v, m), mbu = laneR4(nb_u, m), mbv = laneR4(nb_v, m);
-            bool changed = false;
-            if (in && lane > m) {
-                if (a == am) { ua = mau; va = mav; changed = true; }
-                else if (a == bm) { ua = mbu; va = mbv; changed = true; }
-                if (b == am) { ub = mau; vb = mav; changed = true; }
-                else if (b == bm) { ub = mbu; vb = mbv; changed = true; }
-            }
-            if (changed) e = evalPair(S, xa, xb, ua, ub, va, vb, live);
-            done = m + 1U;
         }
-        // The window's tests and worst overlap, each pair on its final state.
-        tests += u64(__popcll(__ballot(fin_valid)));
-        overlap = waveMaxD(fin_valid && fin_gap < R(0) && double(-fin_gap) > overlap ? double(-fin_gap) : overlap);
-        // The movers' writes are seen by the next window's loads.
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+    __syncthreads();
+    // Counts and maxima, in any order.
+    const u32 lane = tid & 63U, wave = tid >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        tests += __shfl_xor(tests, o);
+        contacts += __shfl_xor(contacts, o);
+        const double ov = __shfl_xor(overlap, o), co = __shfl_xor(corr, o);
+        overlap = ov > overlap ? ov : overlap;
+        corr = co > corr ? co : corr;
     }
     if (lane == 0) {
-        ctl->nc_tests += tests;
-        ctl->nc_contacts += contacts;
-        ctl->nc_overlap = overlap;
-        ctl->nc_correction = corr;
+        atomicAdd(&s_count[0], (unsigned long long)tests);
+        atomicAdd(&s_count[1], (unsigned long long)contacts);
+        s_max[0][wave] = overlap;
+        s_max[1][wave] = corr;
+    }
+    // The ledger's sums in pair order, a chunk of pairs at a time: the movers'
+    // terms are compacted in order in LDS and one lane adds them.
+    double diss = ctl->nc_dissipated, mom[3] = {ctl->nc_momentum[0], ctl->nc_momentum[1], ctl->nc_momentum[2]};
+    for (u32 base = 0; base < P; base += kNarrowThreads) {
+        const u32 p = base + tid;
+        const u32 flag = p < P && L.nc_mover[p] ? 1U : 0U;
+        u32 total = 0;
+        const u32 at = blockScan1024(flag, s_waves, &total);
+        if (flag) s_terms[at] = L.nc_terms[p];
+        __syncthreads();
+        if (tid == 0)
+            for (u32 k = 0; k < total; ++k) {
+                const R4<R> t = s_terms[k];
+                diss += double(t.x);
+                mom[0] += double(t.y);
+                mom[1] += double(t.z);
+                mom[2] += double(t.w);
+            }
+        __syncthreads();
+    }
+    __syncthreads();  // every wave's maxima and counts are in LDS, also when no pair is listed
+    if (tid == 0) {
+        double ov = ctl->nc_overlap, co = ctl->nc_correction;
+        for (u32 w = 0; w < kNarrowThreads / 64; ++w) {
+            ov = s_max[0][w] > ov ? s_max[0][w] : ov;
+            co = s_max[1][w] > co ? s_max[1][w] : co;
+        }
+        ctl->nc_tests += s_count[0];
+        ctl->nc_contacts += s_count[1];
+        ctl->nc_overlap = ov;
+        ctl->nc_correction = co;
         ctl->nc_dissipated = diss;
         for (int k = 0; k < 3; ++k) ctl->nc_momentum[k] = mom[k];
     }
