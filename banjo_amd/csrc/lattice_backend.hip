@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <bit>
 
 #include <algorithm>
 #include <array>
@@ -283,7 +284,7 @@ public:
         size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
         size_t affected, affected_list, tile_plastic, load_part, fext, freq, cand_all, cand_tile, cand_lo, cand_cnt;
         size_t nbb, first_fail;
-        size_t nc_build, nc_key, nc_key_sorted, nc_node, nc_node_sorted, nc_table_key, nc_table_val;
+        size_t nc_build, nc_key, nc_bucket, nc_bucket_sorted, nc_node, nc_node_sorted, nc_table_val;
         size_t pair_fill, pair_other, pair_bond, pair_offset, flat_a, flat_b, flat_bond;
         size_t nc_partners, nc_done, nc_rank_b, nc_sort_key, nc_sort_in, nc_sort_out, nc_terms, nc_mover;
     } off{};
@@ -381,10 +382,10 @@ public:
         l.first_fail = arena.at<uint32_t>(o.first_fail);
         l.nc_build = arena.at<R4<R>>(o.nc_build);
         l.nc_key = arena.at<uint64_t>(o.nc_key);
-        l.nc_key_sorted = arena.at<uint64_t>(o.nc_key_sorted);
+        l.nc_bucket = arena.at<uint32_t>(o.nc_bucket);
+        l.nc_bucket_sorted = arena.at<uint32_t>(o.nc_bucket_sorted);
         l.nc_node = arena.at<uint32_t>(o.nc_node);
         l.nc_node_sorted = arena.at<uint32_t>(o.nc_node_sorted);
-        l.nc_table_key = arena.at<uint64_t>(o.nc_table_key);
         l.nc_table_val = arena.at<uint2>(o.nc_table_val);
         l.nc_table_mask = nc_table_mask;
         l.pair_fill = arena.at<uint32_t>(o.pair_fill);
@@ -631,10 +632,10 @@ public:
         nc_table_mask = table - 1;
         o.nc_build = arena.reserve<R4<R>>(nN);
         o.nc_key = arena.reserve<uint64_t>(nN);
-        o.nc_key_sorted = arena.reserve<uint64_t>(nN);
+        o.nc_bucket = arena.reserve<uint32_t>(nN);
+        o.nc_bucket_sorted = arena.reserve<uint32_t>(nN);
         o.nc_node = arena.reserve<uint32_t>(nN);
         o.nc_node_sorted = arena.reserve<uint32_t>(nN);
-        o.nc_table_key = arena.reserve<uint64_t>(contact ? table : 0);
         o.nc_table_val = arena.reserve<uint2>(contact ? table : 0);
         o.pair_fill = arena.reserve<uint32_t>(nN);
         o.pair_other = arena.reserve<uint32_t>(nP);
@@ -654,9 +655,9 @@ public:
         arena.commit();
         if (contact) {
             size_t sort_bytes = 0, scan_bytes = 0;
-            HIP_OK(rocprim::radix_sort_pairs(nullptr, sort_bytes, static_cast<uint64_t *>(nullptr),
-                                             static_cast<uint64_t *>(nullptr), static_cast<uint32_t *>(nullptr),
-                                             static_cast<uint32_t *>(nullptr), N, 0, 64, stream));
+            HIP_OK(rocprim::radix_sort_pairs(nullptr, sort_bytes, static_cast<uint32_t *>(nullptr),
+                                             static_cast<uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
+                                             static_cast<uint32_t *>(nullptr), N, 0, 32, stream));
             HIP_OK(rocprim::exclusive_scan(nullptr, scan_bytes, static_cast<uint32_t *>(nullptr),
                                            static_cast<uint32_t *>(nullptr), 0U, N, rocprim::plus<uint32_t>(), stream));
             size_t pair_bytes = 0;
@@ -1682,10 +1683,13 @@ private:
             profBegin(KContact, 14);
             hipLaunchKernelGGL((k_nc_cells<R>), dim3(g), dim3(kWG), 0, stream_, L, S_);
             size_t bytes = sl.nc_temp.bytes;
-            HIP_OK(rocprim::radix_sort_pairs(sl.nc_temp.p, bytes, L.nc_key, L.nc_key_sorted, L.nc_node, L.nc_node_sorted,
-                                             sl.N, 0, 64, stream_));
+            // Stable sort by bucket: ascending node index within a bucket. The
+            // buckets take log2(table) bits, plus one for the massless sentinel.
+            const int bucket_bits = std::countr_zero(sl.nc_table_mask + 1U) + 1;
+            HIP_OK(rocprim::radix_sort_pairs(sl.nc_temp.p, bytes, L.nc_bucket, L.nc_bucket_sorted, L.nc_node,
+                                             L.nc_node_sorted, sl.N, 0, bucket_bits, stream_));
             hipLaunchKernelGGL(k_nc_table_clear, dim3(gridFor(sl.nc_table_mask + 1U)), dim3(kWG), 0, stream_,
-                               L.nc_table_key, sl.nc_table_mask + 1U);
+                               L.nc_table_val, sl.nc_table_mask + 1U);
             hipLaunchKernelGGL((k_nc_table_insert<R>), dim3(g), dim3(kWG), 0, stream_, L);
             hipLaunchKernelGGL((k_nc_gather<R>), dim3(g), dim3(kWG), 0, stream_, L, S_);
             bytes = sl.nc_temp.bytes;

@@ -159,9 +159,8 @@ template <class R> struct DevLattice {
     // nodes sorted by cell, a cell table of runs in that order, each node's
     // pairs (kPairsPerNode slots) and the pair list in order (flat_*).
     R4<R> *nc_build;
-    u64 *nc_key, *nc_key_sorted;
-    u32 *nc_node, *nc_node_sorted;
-    u64 *nc_table_key;
+    u64 *nc_key;
+    u32 *nc_bucket, *nc_bucket_sorted, *nc_node, *nc_node_sorted;
     uint2 *nc_table_val;
     u32 nc_table_mask;
     u32 *pair_fill, *pair_other, *pair_bond, *pair_offset, *flat_a, *flat_b, *flat_bond;
@@ -1949,8 +1948,10 @@ template <class R> __global__ __launch_bounds__(kWG) void k_nc_stale(Batch<R> B,
 }
 
 // Step 1 (LatticePhysics.hpp:1460-1468): each node's cell and the displacement
-// the list is built at; the sort key (massless nodes sort last, as the
-// reference leaves them out of the buckets).
+// the list is built at, and its bucket, the sort key: a hash of the cell
+// masked to the table (massless nodes sort last, as the reference leaves them
+// out of the buckets). A bucket can hold several cells; the gather takes only
+// the nodes of the cell it visits, by the exact cell key.
 template <class R> __global__ void k_nc_cells(DevLattice<R> L, Step<R> S) {
     const R inv = R(1) / ncCutoff(S.nc);
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < L.N; i += gridDim.x * blockDim.x) {
@@ -1959,49 +1960,43 @@ template <class R> __global__ void k_nc_cells(DevLattice<R> L, Step<R> S) {
         const R fx = floor(p.x * inv), fy = floor(p.y * inv), fz = floor(p.z * inv);
         const R lo = R(-kCellBias), hi = R(kCellBias);
         u64 key = kNoCell;
+        u32 bucket = L.nc_table_mask + 1U;  // past every bucket: sorts last
         if (x0.w > R(0)) {
-            if (fx >= lo && fx < hi && fy >= lo && fy < hi && fz >= lo && fz < hi)
+            if (fx >= lo && fx < hi && fy >= lo && fy < hi && fz >= lo && fz < hi) {
                 key = cellKey(int(fx), int(fy), int(fz));
-            else
+                bucket = cellSlot(key, L.nc_table_mask);
+            } else {
                 atomicOr(&L.ctl->nc_error, 1U);
+            }
         }
         L.nc_build[i] = u;
         L.nc_key[i] = key;
+        L.nc_bucket[i] = bucket;
         L.nc_node[i] = i;
     }
 }
 
-__global__ void k_nc_table_clear(u64 *table_key, u32 size) {
-    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < size; t += gridDim.x * blockDim.x) table_key[t] = kNoCell;
+__global__ void k_nc_table_clear(uint2 *table, u32 size) {
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < size; t += gridDim.x * blockDim.x) table[t] = make_uint2(0, 0);
 }
 
-// One table entry per occupied cell: its run (start, count) in the sorted
-// order. The table is at least twice the node count, so a probe always ends.
+// Each bucket's run (start, count) in the sorted order.
 template <class R> __global__ void k_nc_table_insert(DevLattice<R> L) {
     for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < L.N; k += gridDim.x * blockDim.x) {
-        const u64 key = L.nc_key_sorted[k];
-        if (key == kNoCell || (k > 0 && L.nc_key_sorted[k - 1] == key)) continue;
+        const u32 bucket = L.nc_bucket_sorted[k];
+        if (bucket > L.nc_table_mask || (k > 0 && L.nc_bucket_sorted[k - 1] == bucket)) continue;
         u32 count = 1;
-        while (k + count < L.N && L.nc_key_sorted[k + count] == key) ++count;
-        u32 h = cellSlot(key, L.nc_table_mask);
-        while (atomicCAS(reinterpret_cast<unsigned long long *>(&L.nc_table_key[h]), kNoCell, key) != kNoCell)
-            h = (h + 1) & L.nc_table_mask;
-        L.nc_table_val[h] = make_uint2(k, count);
+        while (k + count < L.N && L.nc_bucket_sorted[k + count] == bucket) ++count;
+        L.nc_table_val[bucket] = make_uint2(k, count);
     }
 }
 
-template <class R> __device__ __forceinline__ uint2 cellRun(const DevLattice<R> &L, u64 key) {
-    u32 h = cellSlot(key, L.nc_table_mask);
-    for (;;) {
-        const u64 k = L.nc_table_key[h];
-        if (k == key) return L.nc_table_val[h];
-        if (k == kNoCell) return make_uint2(0, 0);
-        h = (h + 1) & L.nc_table_mask;
-    }
-}
+constexpr u32 kSlotRegs = 32;  // a node's neighbour slots the gather keeps in registers
 
 // Step 3 (LatticePhysics.hpp:1494-1546): the pairs node i owns, skipping a
 // pair a live bond holds; the bond found in i's slots is kept with the pair.
+// The node's slots are read once into registers: the first slot naming j
+// decides, as in the reference's scan.
 template <class R> __global__ void k_nc_gather(DevLattice<R> L, Step<R> S) {
     const R cutoff = ncCutoff(S.nc), cutoff2 = cutoff * cutoff;
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < L.N; i += gridDim.x * blockDim.x) {
@@ -2010,6 +2005,19 @@ template <class R> __global__ void k_nc_gather(DevLattice<R> L, Step<R> S) {
         if (own != kNoCell) {
             const V3<R> p = xyz(L.x04[i]) + xyz(L.u4[i]);
             const int cx = cellCoord(own, 0), cy = cellCoord(own, 21), cz = cellCoord(own, 42);
+            // Neighbour slots: the other end, with bit 31 set where the bond is
+            // dead; kNone for no slot.
+            u32 other[kSlotRegs];
+            const bool regs = L.D <= kSlotRegs;
+#pragma unroll
+            for (u32 q = 0; q < kSlotRegs; ++q) {
+                u32 o = kNone;
+                if (regs && q < L.D) {
+                    const u32 key = L.nbr_key[size_t(q) * L.N + i];
+                    if (key != kNone) o = u32(int(i) + L.sdelta[key & ~kDeadSlot]) | (key & kDeadSlot);
+                }
+                other[q] = o;
+            }
             for (int dz = -1; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
@@ -2017,20 +2025,35 @@ template <class R> __global__ void k_nc_gather(DevLattice<R> L, Step<R> S) {
                         if (gx < -kCellBias || gx >= kCellBias || gy < -kCellBias || gy >= kCellBias ||
                             gz < -kCellBias || gz >= kCellBias)
                             continue;  // no node has a cell out there
-                        const uint2 run = cellRun(L, cellKey(gx, gy, gz));
+                        const u64 cell = cellKey(gx, gy, gz);
+                        const uint2 run = L.nc_table_val[cellSlot(cell, L.nc_table_mask)];
                         for (u32 k = run.x; k < run.x + run.y; ++k) {
                             const u32 j = L.nc_node_sorted[k];
-                            if (j <= i) continue;
+                            if (j <= i || L.nc_key[j] != cell) continue;
                             if (length2((xyz(L.x04[j]) + xyz(L.u4[j])) - p) > cutoff2) continue;
                             u32 bond = kNone;
                             bool held = false;
-                            for (u32 q = 0; q < L.D; ++q) {
-                                const u32 key = L.nbr_key[size_t(q) * L.N + i];
-                                if (key == kNone) break;
-                                if (u32(int(i) + L.sdelta[key & ~kDeadSlot]) != j) continue;
-                                bond = L.bnd[L.nbr_bond[size_t(q) * L.N + i]].j;
-                                held = !(key & kDeadSlot);
-                                break;
+                            if (regs) {
+                                u32 found = kSlotRegs, fo = kNone;
+#pragma unroll
+                                for (u32 q = 0; q < kSlotRegs; ++q)
+                                    if (found == kSlotRegs && other[q] != kNone && (other[q] & ~kDeadSlot) == j) {
+                                        found = q;
+                                        fo = other[q];
+                                    }
+                                if (found < kSlotRegs) {
+                                    held = !(fo & kDeadSlot);
+                                    if (!held) bond = L.bnd[L.nbr_bond[size_t(found) * L.N + i]].j;
+                                }
+                            } else {
+                                for (u32 q = 0; q < L.D; ++q) {
+                                    const u32 key = L.nbr_key[size_t(q) * L.N + i];
+                                    if (key == kNone) break;
+                                    if (u32(int(i) + L.sdelta[key & ~kDeadSlot]) != j) continue;
+                                    bond = L.bnd[L.nbr_bond[size_t(q) * L.N + i]].j;
+                                    held = !(key & kDeadSlot);
+                                    break;
+                                }
                             }
                             if (held) continue;
                             if (fill >= kPairsPerNode) {
