@@ -6,6 +6,8 @@ cites the reference type it mirrors (paths in the reference tree).
 """
 import ctypes as C
 
+import numpy as np
+
 BLX_OK = 0
 BLX_ERR_RUNTIME = -1
 BLX_ERR_INVALID = -2
@@ -103,6 +105,25 @@ class NodeContactAccumulators(C.Structure):
     _fields_ = [("contacts", u64), ("pair_tests", u64), ("rebuilds", u64), ("pairs_listed", u64),
                 ("pair_overflow", u64), ("dissipated_kinetic_energy_j", f64), ("maximum_overlap_m", f64),
                 ("maximum_position_correction_m", f64), ("momentum_residual", f64 * 3)]
+
+
+class FragmentMass(C.Structure):
+    """FragmentMassProperties, src/fracture/FragmentMassProperties.hpp:11-21,
+    with the fragment's node count and smallest node."""
+    _fields_ = [("mass_kg", f64), ("center_of_mass_world_m", f64 * 3), ("linear_velocity_m_s", f64 * 3),
+                ("inertia_world_kg_m2", f64 * 9), ("angular_momentum_kg_m2_s", f64 * 3),
+                ("angular_velocity_rad_s", f64 * 3), ("source_kinetic_energy_j", f64),
+                ("rigid_kinetic_energy_j", f64), ("coarsening_kinetic_loss_j", f64),
+                ("node_count", u32), ("first_node", u32)]
+
+
+# The same record as a numpy structured dtype (one row per fragment).
+FRAGMENT_MASS_DTYPE = np.dtype([
+    ("mass_kg", "<f8"), ("center_of_mass_world_m", "<f8", (3,)), ("linear_velocity_m_s", "<f8", (3,)),
+    ("inertia_world_kg_m2", "<f8", (3, 3)), ("angular_momentum_kg_m2_s", "<f8", (3,)),
+    ("angular_velocity_rad_s", "<f8", (3,)), ("source_kinetic_energy_j", "<f8"),
+    ("rigid_kinetic_energy_j", "<f8"), ("coarsening_kinetic_loss_j", "<f8"),
+    ("node_count", "<u4"), ("first_node", "<u4")])
 
 
 class RunStatus(C.Structure):

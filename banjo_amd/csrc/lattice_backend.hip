@@ -291,6 +291,12 @@ public:
     size_t used = 0;  // arena bytes the arrays occupy
     Arena arena;
     DevBuf cc_parent, cc_root, cc_size, cc_rank, cc_label, cc_keys, cc_sorted, cc_count, cc_temp;
+    // Reference positions and masses as uploaded, in double (x, y, z, mass per
+    // node), for fragment mass properties; copied to mp_x0m on first use.
+    std::shared_ptr<const std::vector<double>> x0m_host;
+    bool mp_ready = false;
+    DevBuf mp_x0m, mp_spin, mp_key, mp_order, mp_iota, mp_bounds, mp_sum, mp_first, mp_last, mp_cross, mp_frag, mp_bad,
+        mp_temp;
 
     void setSchedule(Schedule s) {
         sch = std::move(s);
@@ -432,6 +438,8 @@ public:
         mass_host = src.mass_host;
         inv_mass_host = src.inv_mass_host;
         node_j_host = src.node_j_host;
+        x0m_host = src.x0m_host;
+        mp_ready = false;
         off = src.off;
         used = src.used;
         arena.reset();
@@ -482,6 +490,15 @@ public:
         for (uint32_t i = 0; i < N; ++i) {
             mass_host[i] = R(st.mass[i]);
             inv_mass_host[i] = R(st.inv_mass[i]);
+        }
+        {
+            auto x0m = std::make_shared<std::vector<double>>(size_t(N) * 4);
+            for (uint32_t i = 0; i < N; ++i) {
+                for (int c = 0; c < 3; ++c) (*x0m)[4 * size_t(i) + c] = st.x0[3 * size_t(i) + c];
+                (*x0m)[4 * size_t(i) + 3] = st.mass[i];
+            }
+            x0m_host = std::move(x0m);
+            mp_ready = false;
         }
         // Bonds in node-major order for the end sample: a counting sort by the
         // first end, stable in schedule index.
@@ -1263,6 +1280,86 @@ public:
         return c;
     }
 
+    // calculateFragmentMassProperties (FragmentMassProperties.cpp:7-80) of
+    // every fragment of slot k, on the device: canonical labels, a stable
+    // sort of the nodes by label, two fixed-order segmented sums and the
+    // per-fragment closing algebra. Returns the fragment count; out[f] for
+    // f < cap.
+    uint32_t massProperties(uint32_t k, double cell, const double *spin, blx_fragment_mass *out, uint32_t cap) {
+        const uint32_t c = labels(k, nullptr);
+        Slot<R> &sl = slot(k);
+        const uint32_t N = sl.N;
+        if (c == 0) return 0;
+        if (!sl.x0m_host || sl.x0m_host->size() != size_t(N) * 4) throw Logic("fragment mass before upload");
+        if (!sl.mp_ready) {
+            sl.mp_x0m.alloc(size_t(N) * 32);
+            HIP_OK(hipMemcpy(sl.mp_x0m.p, sl.x0m_host->data(), size_t(N) * 32, hipMemcpyHostToDevice));
+            sl.mp_ready = true;
+        }
+        if (spin) {
+            sl.mp_spin.alloc(size_t(N) * 24);
+            HIP_OK(hipMemcpyAsync(sl.mp_spin.p, spin, size_t(N) * 24, hipMemcpyHostToDevice, stream_));
+        }
+        const uint32_t tiles = (N + kMpTile - 1) / kMpTile;
+        const size_t n4 = size_t(N) * 4;
+        sl.mp_key.alloc(n4);
+        sl.mp_order.alloc(n4);
+        sl.mp_iota.alloc(n4);
+        sl.mp_bounds.alloc(size_t(c) * 8);
+        sl.mp_sum.alloc(size_t(c) * 10 * 8);
+        sl.mp_first.alloc(size_t(tiles) * 10 * 8);
+        sl.mp_last.alloc(size_t(tiles) * 10 * 8);
+        sl.mp_cross.alloc(size_t(tiles) * 4);
+        sl.mp_frag.alloc(size_t(c) * sizeof(blx_fragment_mass));
+        sl.mp_bad.alloc(4);
+        const unsigned bits = std::max(1U, unsigned(std::bit_width(c - 1)));
+        uint32_t *label = sl.cc_label.template as<uint32_t>(), *key = sl.mp_key.template as<uint32_t>(),
+                 *iota = sl.mp_iota.template as<uint32_t>(), *order = sl.mp_order.template as<uint32_t>();
+        size_t temp = 0;
+        HIP_OK(rocprim::radix_sort_pairs(nullptr, temp, label, key, iota, order, N, 0, bits, stream_));
+        sl.mp_temp.alloc(temp);
+        temp = sl.mp_temp.bytes;
+        hipLaunchKernelGGL(k_nc_iota, dim3(gridFor(N)), dim3(kWG), 0, stream_, iota, N);
+        HIP_OK(rocprim::radix_sort_pairs(sl.mp_temp.p, temp, label, key, iota, order, N, 0, bits, stream_));
+        const uint32_t bad0 = kNone;
+        HIP_OK(hipMemcpyAsync(sl.mp_bad.p, &bad0, 4, hipMemcpyHostToDevice, stream_));
+        MpArgs<R> a{};
+        a.u4 = sl.L.u4;
+        a.v4 = sl.L.v4;
+        a.x0m = sl.mp_x0m.template as<double4>();
+        a.spin = spin ? sl.mp_spin.template as<double>() : nullptr;
+        a.key = key;
+        a.order = order;
+        a.bounds = sl.mp_bounds.template as<uint2>();
+        a.sum = sl.mp_sum.template as<double>();
+        a.first = sl.mp_first.template as<double>();
+        a.last = sl.mp_last.template as<double>();
+        a.cross = sl.mp_cross.template as<uint32_t>();
+        a.frag = sl.mp_frag.template as<blx_fragment_mass>();
+        a.bad = sl.mp_bad.template as<uint32_t>();
+        for (int q = 0; q < 3; ++q) a.origin[q] = sl.L.origin[q];
+        a.cell = cell;
+        a.n = N;
+        a.k = c;
+        hipLaunchKernelGGL(k_mp_bounds, dim3(gridFor(N)), dim3(kWG), 0, stream_, key,
+                           sl.mp_bounds.template as<uint2>(), N);
+        hipLaunchKernelGGL((k_mp_tile<R, 1>), dim3(tiles), dim3(kMpTile), 0, stream_, a);
+        hipLaunchKernelGGL((k_mp_combine<R, 1>), dim3(tiles), dim3(kMpTile), 0, stream_, a);
+        hipLaunchKernelGGL((k_mp_final1<R>), dim3(gridFor(c)), dim3(kWG), 0, stream_, a);
+        hipLaunchKernelGGL((k_mp_tile<R, 2>), dim3(tiles), dim3(kMpTile), 0, stream_, a);
+        hipLaunchKernelGGL((k_mp_combine<R, 2>), dim3(tiles), dim3(kMpTile), 0, stream_, a);
+        hipLaunchKernelGGL((k_mp_final2<R>), dim3(gridFor(c)), dim3(kWG), 0, stream_, a);
+        HIP_OK(hipGetLastError());
+        uint32_t bad = kNone;
+        HIP_OK(hipMemcpyAsync(&bad, sl.mp_bad.p, 4, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        if (bad != kNone) throw Runtime(bad & 1U ? "fragment inertia is singular" : "fragment has no mass");
+        if (out && cap)
+            HIP_OK(hipMemcpy(out, sl.mp_frag.p, size_t(std::min(c, cap)) * sizeof(blx_fragment_mass),
+                             hipMemcpyDeviceToHost));
+        return c;
+    }
+
     // Device bytes one slot holds (its arena).
     size_t slotBytes(uint32_t k) const { return slot(k).arena.bytes(); }
 
@@ -1854,6 +1951,13 @@ int blx_backend_clear_frames(blx_backend *b) {
 int blx_backend_fragment_labels(blx_backend *b, uint32_t *label, uint32_t *count) {
     return guard([&] { *count = dispatch(H(b), [&](auto &e) { return e.labels(0, label); }); });
 }
+
+int blx_backend_fragment_mass_properties(blx_backend *b, double cell_size_m, const double *spin,
+                                         blx_fragment_mass *out, uint32_t capacity, uint32_t *count) {
+    return guard([&] {
+        *count = dispatch(H(b), [&](auto &e) { return e.massProperties(0, cell_size_m, spin, out, capacity); });
+    });
+}
 int blx_backend_set_profiling(blx_backend *b, int on) {
     return guard([&] { dispatch(H(b), [&](auto &e) { e.setProfiling(on != 0); }); });
 }
@@ -1936,6 +2040,13 @@ int blx_batch_first_failure_bonds(const blx_batch *b, uint32_t slot, uint32_t *o
 }
 int blx_batch_fragment_labels(blx_batch *b, uint32_t slot, uint32_t *label, uint32_t *count) {
     return guard([&] { *count = dispatch(H(b), [&](auto &e) { return e.labels(slot, label); }); });
+}
+
+int blx_batch_fragment_mass_properties(blx_batch *b, uint32_t slot, double cell_size_m, const double *spin,
+                                       blx_fragment_mass *out, uint32_t capacity, uint32_t *count) {
+    return guard([&] {
+        *count = dispatch(H(b), [&](auto &e) { return e.massProperties(slot, cell_size_m, spin, out, capacity); });
+    });
 }
 int blx_batch_set_blade(blx_batch *b, const blx_blade *blade) {
     return guard([&] {

@@ -28,6 +28,7 @@
 // tree, so every run is bit-reproducible.
 #pragma once
 
+#include "banjo_lattice.h"
 #include "lattice_device.hpp"
 
 namespace blx {
@@ -2331,8 +2332,238 @@ template <class R> __global__ __launch_bounds__(kNarrowThreads) void k_nc_narrow
     }
 }
 
-// A plain streaming copy, 16 bytes per lane per iteration: the known-size
-// workload rocprofv3's FETCH_SIZE and WRITE_SIZE counters are scaled against.
+// ---------------------------------------------------------------------------
+// Fragment mass properties (fracture/FragmentMassProperties.cpp:7-80).
+//
+// The nodes, stably sorted by canonical label (so each fragment's nodes are in
+// ascending order, as its node_indices), are cut into tiles of kMpTile. A tile
+// builds each node's terms, runs a segmented inclusive scan in LDS over the
+// fragment pieces it holds, and writes the sums of the fragments that lie
+// inside it. A fragment that crosses tiles leaves one piece sum per tile; the
+// tile it starts in owns it, and k_mp_combine adds its pieces in a fixed tree.
+// Pass 1 sums mass, m p and m v; pass 2, with the centre of mass and linear
+// velocity, the inertia, angular momentum and kinetic energy terms. Every sum
+// has a fixed association, so results repeat bit for bit; the reference adds
+// serially in node order, so the two agree to rounding. Each node's terms keep
+// the reference's expressions.
+constexpr u32 kMpTile = 256;
+
+template <class R> struct MpArgs {
+    const R4<R> *u4, *v4;
+    const double4 *x0m;      // reference position and mass, as uploaded in double
+    const double *spin;      // 3N, or null for no spin
+    const u32 *key, *order;  // label and node at each sorted position
+    const uint2 *bounds;     // [begin, end) of each fragment in the sorted order
+    double *sum;             // fragments x terms
+    double *first, *last;    // tiles x terms: piece sums of a tile's first and last fragment
+    u32 *cross;              // per tile: the fragment it starts and that continues past it, or kNone
+    blx_fragment_mass *frag;
+    u32 *bad;                // least 2 f + kind over failing fragments (0 no mass, 1 singular inertia)
+    double origin[3], cell;
+    u32 n, k;
+};
+
+template <int PASS> constexpr int kMpTerms = PASS == 1 ? 7 : 10;
+
+template <class R, int PASS> __device__ __forceinline__ void mpTerms(const MpArgs<R> &a, u32 i, u32 f, double *t) {
+    const double4 x = a.x0m[i];
+    const R4<R> u = a.u4[i], v = a.v4[i];
+    const double m = x.w;
+    // position_world_m = origin + x0 + u (FastLattice.cpp:179)
+    const double px = (a.origin[0] + x.x) + double(u.x), py = (a.origin[1] + x.y) + double(u.y),
+                 pz = (a.origin[2] + x.z) + double(u.z);
+    const double vx = double(v.x), vy = double(v.y), vz = double(v.z);
+    if (PASS == 1) {  // FragmentMassProperties.cpp:20-26
+        t[0] = m;
+        t[1] = px * m;
+        t[2] = py * m;
+        t[3] = pz * m;
+        t[4] = vx * m;
+        t[5] = vy * m;
+        t[6] = vz * m;
+        return;
+    }
+    // FragmentMassProperties.cpp:35-58; the off-diagonal terms are summed and subtracted from zero at the end.
+    const blx_fragment_mass &g = a.frag[f];
+    const double rx = px - g.center_of_mass_world_m[0], ry = py - g.center_of_mass_world_m[1],
+                 rz = pz - g.center_of_mass_world_m[2];
+    const double cdi = m * a.cell * a.cell / 6.0;
+    t[0] = m * (ry * ry + rz * rz) + cdi;
+    t[1] = m * (rx * rx + rz * rz) + cdi;
+    t[2] = m * (rx * rx + ry * ry) + cdi;
+    t[3] = m * rx * ry;
+    t[4] = m * rx * rz;
+    t[5] = m * ry * rz;
+    const double wx = (vx - g.linear_velocity_m_s[0]) * m, wy = (vy - g.linear_velocity_m_s[1]) * m,
+                 wz = (vz - g.linear_velocity_m_s[2]) * m;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (a.spin) {
+        sx = a.spin[3 * size_t(i)];
+        sy = a.spin[3 * size_t(i) + 1];
+        sz = a.spin[3 * size_t(i) + 2];
+    }
+    t[6] = (ry * wz - rz * wy) + sx * cdi;
+    t[7] = (rz * wx - rx * wz) + sy * cdi;
+    t[8] = (rx * wy - ry * wx) + sz * cdi;
+    t[9] = 0.5 * m * (vx * vx + vy * vy + vz * vz) + 0.5 * cdi * (sx * sx + sy * sy + sz * sz);
+}
+
+// Fragment bounds in the sorted order (every fragment has a node).
+__global__ void k_mp_bounds(const u32 *key, uint2 *bounds, u32 n) {
+    for (u32 q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const u32 f = key[q];
+        if (q == 0 || key[q - 1] != f) bounds[f].x = q;
+        if (q + 1 == n || key[q + 1] != f) bounds[f].y = q + 1;
+    }
+}
+
+template <class R, int PASS> __global__ __launch_bounds__(kMpTile) void k_mp_tile(MpArgs<R> a) {
+    constexpr int M = kMpTerms<PASS>;
+    __shared__ double s[M][kMpTile];
+    const u32 t = threadIdx.x, base = blockIdx.x * kMpTile, q = base + t;
+    const u32 top = min(base + kMpTile, a.n);
+    const bool in = q < a.n;
+    double v[M];
+    u32 f = kNone;
+    uint2 b = make_uint2(0U, 0U);
+    if (in) {
+        f = a.key[q];
+        b = a.bounds[f];
+        mpTerms<R, PASS>(a, a.order[q], f, v);
+    } else {
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) s[m][t] = v[m];
+    const u32 lo = max(b.x, base);  // first position of this node's piece in the tile
+    __syncthreads();
+    for (u32 d = 1; d < kMpTile; d <<= 1) {
+        const bool take = in && q >= lo + d;
+        double w[M];
+        if (take) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) w[m] = s[m][t - d];
+        }
+        __syncthreads();
+        if (take) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                v[m] = w[m] + v[m];
+                s[m][t] = v[m];
+            }
+        }
+        __syncthreads();
+    }
+    if (!in || q + 1 != min(b.y, top)) return;  // not the last node of its piece
+    const bool starts = b.x >= base, ends = b.y <= top;
+    if (starts && ends) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) a.sum[size_t(f) * M + m] = v[m];
+    } else {
+        if (!starts) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) a.first[size_t(blockIdx.x) * M + m] = v[m];
+        }
+        if (!ends) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) a.last[size_t(blockIdx.x) * M + m] = v[m];
+        }
+    }
+    if (q + 1 == top) a.cross[blockIdx.x] = starts && !ends ? f : kNone;
+}
+
+// One workgroup per tile; the tile a crossing fragment starts in adds its
+// pieces: each lane serially over a stride of tiles, then a tree in LDS.
+template <class R, int PASS> __global__ __launch_bounds__(kMpTile) void k_mp_combine(MpArgs<R> a) {
+    constexpr int M = kMpTerms<PASS>;
+    __shared__ double s[M][kMpTile];
+    const u32 t0 = blockIdx.x, f = a.cross[t0];
+    if (f == kNone) return;
+    const u32 t1 = (a.bounds[f].y - 1) / kMpTile, parts = t1 - t0 + 1, t = threadIdx.x;
+    double acc[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] = 0.0;
+    for (u32 j = t; j < parts; j += kMpTile) {
+        const double *p = j == 0 ? a.last + size_t(t0) * M : a.first + size_t(t0 + j) * M;
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[m] = acc[m] + p[m];
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) s[m][t] = acc[m];
+    __syncthreads();
+    for (u32 w = kMpTile / 2; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) s[m][t] = s[m][t] + s[m][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) a.sum[size_t(f) * M + m] = s[m][0];
+    }
+}
+
+// Mass, centre of mass and linear velocity (FragmentMassProperties.cpp:27-33).
+// Each sum starts from zero as the reference's accumulators do (0 + -0 = +0).
+template <class R> __global__ void k_mp_final1(MpArgs<R> a) {
+    for (u32 f = blockIdx.x * blockDim.x + threadIdx.x; f < a.k; f += gridDim.x * blockDim.x) {
+        const double *s = a.sum + size_t(f) * 7;
+        blx_fragment_mass g{};
+        g.mass_kg = 0.0 + s[0];
+        if (g.mass_kg <= 0.0) atomicMin(a.bad, 2 * f);
+        for (int c = 0; c < 3; ++c) {
+            g.center_of_mass_world_m[c] = (0.0 + s[1 + c]) / g.mass_kg;
+            g.linear_velocity_m_s[c] = (0.0 + s[4 + c]) / g.mass_kg;
+        }
+        const uint2 b = a.bounds[f];
+        g.node_count = b.y - b.x;
+        g.first_node = a.order[b.x];
+        a.frag[f] = g;
+    }
+}
+
+// Inertia, angular momentum and velocity, energies (FragmentMassProperties.cpp:35-78; Mat3::inverse, core/Math.hpp).
+template <class R> __global__ void k_mp_final2(MpArgs<R> a) {
+    for (u32 f = blockIdx.x * blockDim.x + threadIdx.x; f < a.k; f += gridDim.x * blockDim.x) {
+        const double *s = a.sum + size_t(f) * 10;
+        blx_fragment_mass g = a.frag[f];
+        double I[3][3];
+        I[0][0] = 0.0 + s[0];
+        I[1][1] = 0.0 + s[1];
+        I[2][2] = 0.0 + s[2];
+        I[0][1] = I[1][0] = 0.0 - s[3];
+        I[0][2] = I[2][0] = 0.0 - s[4];
+        I[1][2] = I[2][1] = 0.0 - s[5];
+        const double L[3] = {0.0 + s[6], 0.0 + s[7], 0.0 + s[8]};
+        g.source_kinetic_energy_j = 0.0 + s[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) g.inertia_world_kg_m2[3 * r + c] = I[r][c];
+        for (int c = 0; c < 3; ++c) g.angular_momentum_kg_m2_s[c] = L[c];
+        const double det = I[0][0] * (I[1][1] * I[2][2] - I[1][2] * I[2][1]) -
+                           I[0][1] * (I[1][0] * I[2][2] - I[1][2] * I[2][0]) +
+                           I[0][2] * (I[1][0] * I[2][1] - I[1][1] * I[2][0]);
+        if (fabs(det) <= 0.0) atomicMin(a.bad, 2 * f + 1);
+        double J[3][3];
+        J[0][0] = (I[1][1] * I[2][2] - I[1][2] * I[2][1]) / det;
+        J[0][1] = (I[0][2] * I[2][1] - I[0][1] * I[2][2]) / det;
+        J[0][2] = (I[0][1] * I[1][2] - I[0][2] * I[1][1]) / det;
+        J[1][0] = (I[1][2] * I[2][0] - I[1][0] * I[2][2]) / det;
+        J[1][1] = (I[0][0] * I[2][2] - I[0][2] * I[2][0]) / det;
+        J[1][2] = (I[0][2] * I[1][0] - I[0][0] * I[1][2]) / det;
+        J[2][0] = (I[1][0] * I[2][1] - I[1][1] * I[2][0]) / det;
+        J[2][1] = (I[0][1] * I[2][0] - I[0][0] * I[2][1]) / det;
+        J[2][2] = (I[0][0] * I[1][1] - I[0][1] * I[1][0]) / det;
+        for (int r = 0; r < 3; ++r) g.angular_velocity_rad_s[r] = J[r][0] * L[0] + J[r][1] * L[1] + J[r][2] * L[2];
+        const double *vl = g.linear_velocity_m_s, *w = g.angular_velocity_rad_s;
+        g.rigid_kinetic_energy_j = 0.5 * g.mass_kg * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) +
+                                   0.5 * (w[0] * L[0] + w[1] * L[1] + w[2] * L[2]);
+        g.coarsening_kinetic_loss_j = g.source_kinetic_energy_j - g.rigid_kinetic_energy_j;
+        a.frag[f] = g;
+    }
+}
+
 // Streaming copy, one 16-byte element per thread: the achievable-bandwidth
 // reference, and the known traffic that scales the HBM byte counters.
 __global__ __launch_bounds__(kWG) void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {

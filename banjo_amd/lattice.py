@@ -252,6 +252,25 @@ class LatticeLibrary:
                                  C.byref(count)), name)
         return label, count.value
 
+    def fragment_mass(self, position, velocity, mass, cell_size, offsets, nodes, spin=None):
+        """calculateFragmentMassProperties (src/fracture/FragmentMassProperties.cpp:7-80)
+        of fragments given as node lists, fragment f = nodes[offsets[f]:offsets[f+1]]
+        (oracle and reference libraries)."""
+        pos = np.ascontiguousarray(position, np.float64).reshape(-1)
+        vel = np.ascontiguousarray(velocity, np.float64).reshape(-1)
+        m = np.ascontiguousarray(mass, np.float64)
+        off = np.ascontiguousarray(offsets, np.uint32)
+        nd = np.ascontiguousarray(nodes, np.uint32)
+        sp = None if spin is None else np.ascontiguousarray(spin, np.float64).reshape(-1)
+        k = len(off) - 1
+        out = np.zeros(k, abi.FRAGMENT_MASS_DTYPE)
+        self.check(self.fn("fragment_mass")(
+            C.c_uint32(len(m)), _ptr(pos, C.c_double), _ptr(vel, C.c_double),
+            None if sp is None else _ptr(sp, C.c_double), _ptr(m, C.c_double), C.c_double(cell_size),
+            C.c_uint32(k), _ptr(off, C.c_uint32), _ptr(nd, C.c_uint32),
+            out.ctypes.data_as(C.POINTER(abi.FragmentMass))), "fragment_mass")
+        return out
+
     def state_metrics(self, state: "State"):
         """(substep limit s, elastic energy J, kinetic energy J) of a state:
         latticeStateSubstepLimit / ElasticEnergy / KineticEnergy
@@ -266,6 +285,18 @@ class LatticeLibrary:
 
     def backend(self, precision, schedule: Schedule, *extra):
         return Backend(self, precision, schedule, *extra)
+
+
+def _mass_properties(library, name, head, node_count, cell_size, spin):
+    sp = None if spin is None else np.ascontiguousarray(spin, np.float64).reshape(-1)
+    if sp is not None and len(sp) != 3 * node_count:
+        raise ValueError("spin holds three components per node")
+    out = np.empty(max(node_count, 1), abi.FRAGMENT_MASS_DTYPE)
+    count = C.c_uint32()
+    library.check(library.fn(name)(*head, C.c_double(cell_size), None if sp is None else _ptr(sp, C.c_double),
+                                   out.ctypes.data_as(C.POINTER(abi.FragmentMass)), C.c_uint32(len(out)),
+                                   C.byref(count)), name)
+    return out[:count.value].copy()
 
 
 class Backend:
@@ -366,6 +397,12 @@ class Backend:
                      "fragment_labels")
         return label, count.value
 
+    def fragment_mass_properties(self, cell_size, spin=None):
+        """Mass properties of every fragment of the current state, in canonical
+        label order (blx_backend_fragment_mass_properties)."""
+        return _mass_properties(self.L, "backend_fragment_mass_properties", (self.h,), self.node_count,
+                                cell_size, spin)
+
 
 class Batch:
     """A batch of independent lattices advanced together (blx_batch_* of
@@ -454,6 +491,10 @@ class Batch:
         self.L.check(self.L.fn("batch_fragment_labels")(self.h, C.c_uint32(slot), _ptr(label, C.c_uint32),
                                                         C.byref(count)), "batch_fragment_labels")
         return label, count.value
+
+    def fragment_mass_properties(self, slot, cell_size, spin=None):
+        return _mass_properties(self.L, "batch_fragment_mass_properties", (self.h, C.c_uint32(slot)),
+                                self.sizes[slot][0], cell_size, spin)
 
     def release(self, slot):
         self.L.check(self.L.fn("batch_release")(self.h, C.c_uint32(slot)), "batch_release")

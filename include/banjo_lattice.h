@@ -234,6 +234,21 @@ typedef struct blx_state_view {
     double *plastic_extension, *plastic_strain;                       /* B */
 } blx_state_view;
 
+/* FragmentMassProperties (fracture/FragmentMassProperties.hpp:11-21) of one
+ * fragment, with the fragment's node count and smallest node. */
+typedef struct blx_fragment_mass {
+    double mass_kg;
+    double center_of_mass_world_m[3];
+    double linear_velocity_m_s[3];
+    double inertia_world_kg_m2[9]; /* row-major, symmetric */
+    double angular_momentum_kg_m2_s[3];
+    double angular_velocity_rad_s[3];
+    double source_kinetic_energy_j;
+    double rigid_kinetic_energy_j;
+    double coarsening_kinetic_loss_j;
+    uint32_t node_count, first_node;
+} blx_fragment_mass;
+
 /* ---- errors and probes ------------------------------------------------- */
 
 /* Message of the last failed call on this thread ("" after success). */
@@ -341,6 +356,21 @@ int blx_backend_clear_frames(blx_backend *backend);
 int blx_backend_fragment_labels(blx_backend *backend, uint32_t *label,
                                 uint32_t *component_count);
 
+/* calculateFragmentMassProperties (fracture/FragmentMassProperties.cpp:7-80)
+ * of every fragment of the current state, on the device, as the fragment
+ * build calls it per component (FragmentGeometry.cpp:231-234): out[f] for
+ * canonical fragment f of blx_backend_fragment_labels, f < capacity;
+ * *component_count fragments. A node's world position is origin + x0 + u and
+ * its mass the state's, both as uploaded in double; its velocity v; its
+ * spin spin[3i..3i+2] (NULL: zero, as the lattice lane carries none).
+ * cell_size_m is the asset's voxel size (the finite-cell inertia m h^2 / 6).
+ * Sums run in a fixed order (deterministic run to run), not the reference's
+ * serial order, so they agree with it to rounding. A fragment of no mass or
+ * of singular inertia is BLX_ERR_RUNTIME, as the reference throws. */
+int blx_backend_fragment_mass_properties(blx_backend *backend, double cell_size_m,
+                                         const double *spin, blx_fragment_mass *out,
+                                         uint32_t capacity, uint32_t *component_count);
+
 /* ---- kinematic blade (no counterpart in the reference's lattice API) ------
  *
  * A thin blade moving at constant velocity through the lattice, for cutting
@@ -404,6 +434,10 @@ int blx_batch_status(const blx_batch *batch, uint32_t slot, blx_run_status *stat
 int blx_batch_download(blx_batch *batch, uint32_t slot, blx_state_view *state, blx_sphere *sphere);
 int blx_batch_fragment_labels(blx_batch *batch, uint32_t slot, uint32_t *label,
                               uint32_t *component_count);
+/* blx_backend_fragment_mass_properties for one slot. */
+int blx_batch_fragment_mass_properties(blx_batch *batch, uint32_t slot, double cell_size_m,
+                                       const double *spin, blx_fragment_mass *out,
+                                       uint32_t capacity, uint32_t *component_count);
 /* The slot's sphere without a state download. */
 int blx_batch_sphere(blx_batch *batch, uint32_t slot, blx_sphere *sphere);
 /* LatticeBackend::firstFailureBonds for one slot (see blx_backend_first_failure_bonds). */

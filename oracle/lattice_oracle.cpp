@@ -18,6 +18,8 @@
 //   run loop          CpuLatticeBackend.cpp:255-304
 //   external forces   ExternalLoads.hpp:28-50, 79-148
 //   fragments         fracture/ConnectedComponents.cpp:14-87
+//   fragment mass     fracture/FragmentMassProperties.cpp:7-80, Mat3::inverse
+//                     (core/Math.hpp:110-127)
 //   node contact      LatticePhysics.hpp:1430-1667, CpuLatticeBackend.cpp:424-437
 //   box stencil       matter/BoxLattice.cpp:49-171 (Moore stencil in place of
 //                     the Euclidean horizon)
@@ -1354,6 +1356,83 @@ inline Schedule fromView(const blx_schedule_view &v) {
     return s;
 }
 
+// calculateFragmentMassProperties (fracture/FragmentMassProperties.cpp:7-80)
+// of one fragment: serial sums in node order, the reference's accumulators
+// and expressions; Mat3::inverse with epsilon 0 (core/Math.hpp:110-127).
+inline void fragmentMass(const double *pos, const double *vel, const double *spin, const double *mass, double cell,
+                         const u32 *nodes, u32 count, u32 n_nodes, blx_fragment_mass &out) {
+    if (count == 0) throw invalid("fragment requires at least one material node");
+    out = blx_fragment_mass{};
+    double m_sum = 0.0, wp[3] = {0.0, 0.0, 0.0}, lm[3] = {0.0, 0.0, 0.0};
+    for (u32 k = 0; k < count; ++k) {
+        const u32 i = nodes[k];
+        if (i >= n_nodes) throw std::out_of_range("fragment node index is invalid");
+        const double m = mass[i];
+        m_sum += m;
+        for (int c = 0; c < 3; ++c) wp[c] += pos[3 * size_t(i) + c] * m;
+        for (int c = 0; c < 3; ++c) lm[c] += vel[3 * size_t(i) + c] * m;
+    }
+    if (m_sum <= 0.0) throw std::runtime_error("fragment has no mass");
+    double com[3], vl[3];
+    for (int c = 0; c < 3; ++c) {
+        com[c] = wp[c] / m_sum;
+        vl[c] = lm[c] / m_sum;
+    }
+    double I[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}}, L[3] = {0.0, 0.0, 0.0}, src = 0.0;
+    for (u32 k = 0; k < count; ++k) {
+        const u32 i = nodes[k];
+        const double m = mass[i];
+        const double *p = pos + 3 * size_t(i), *v = vel + 3 * size_t(i);
+        const double s0 = spin ? spin[3 * size_t(i)] : 0.0, s1 = spin ? spin[3 * size_t(i) + 1] : 0.0,
+                     s2 = spin ? spin[3 * size_t(i) + 2] : 0.0;
+        const double rx = p[0] - com[0], ry = p[1] - com[1], rz = p[2] - com[2];
+        const double cdi = m * cell * cell / 6.0;
+        I[0][0] += m * (ry * ry + rz * rz) + cdi;
+        I[1][1] += m * (rx * rx + rz * rz) + cdi;
+        I[2][2] += m * (rx * rx + ry * ry) + cdi;
+        I[0][1] -= m * rx * ry;
+        I[1][0] = I[0][1];
+        I[0][2] -= m * rx * rz;
+        I[2][0] = I[0][2];
+        I[1][2] -= m * ry * rz;
+        I[2][1] = I[1][2];
+        const double wx = (v[0] - vl[0]) * m, wy = (v[1] - vl[1]) * m, wz = (v[2] - vl[2]) * m;
+        L[0] += (ry * wz - rz * wy) + s0 * cdi;
+        L[1] += (rz * wx - rx * wz) + s1 * cdi;
+        L[2] += (rx * wy - ry * wx) + s2 * cdi;
+        src += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + 0.5 * cdi * (s0 * s0 + s1 * s1 + s2 * s2);
+    }
+    const double det = I[0][0] * (I[1][1] * I[2][2] - I[1][2] * I[2][1]) -
+                       I[0][1] * (I[1][0] * I[2][2] - I[1][2] * I[2][0]) +
+                       I[0][2] * (I[1][0] * I[2][1] - I[1][1] * I[2][0]);
+    if (std::abs(det) <= 0.0) throw std::runtime_error("fragment inertia is singular");
+    double J[3][3];
+    J[0][0] = (I[1][1] * I[2][2] - I[1][2] * I[2][1]) / det;
+    J[0][1] = (I[0][2] * I[2][1] - I[0][1] * I[2][2]) / det;
+    J[0][2] = (I[0][1] * I[1][2] - I[0][2] * I[1][1]) / det;
+    J[1][0] = (I[1][2] * I[2][0] - I[1][0] * I[2][2]) / det;
+    J[1][1] = (I[0][0] * I[2][2] - I[0][2] * I[2][0]) / det;
+    J[1][2] = (I[0][2] * I[1][0] - I[0][0] * I[1][2]) / det;
+    J[2][0] = (I[1][0] * I[2][1] - I[1][1] * I[2][0]) / det;
+    J[2][1] = (I[0][1] * I[2][0] - I[0][0] * I[2][1]) / det;
+    J[2][2] = (I[0][0] * I[1][1] - I[0][1] * I[1][0]) / det;
+    out.mass_kg = m_sum;
+    for (int c = 0; c < 3; ++c) {
+        out.center_of_mass_world_m[c] = com[c];
+        out.linear_velocity_m_s[c] = vl[c];
+        out.angular_momentum_kg_m2_s[c] = L[c];
+        out.angular_velocity_rad_s[c] = J[c][0] * L[0] + J[c][1] * L[1] + J[c][2] * L[2];
+        for (int r = 0; r < 3; ++r) out.inertia_world_kg_m2[3 * c + r] = I[c][r];
+    }
+    const double *w = out.angular_velocity_rad_s;
+    out.source_kinetic_energy_j = src;
+    out.rigid_kinetic_energy_j = 0.5 * m_sum * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) +
+                                 0.5 * (w[0] * L[0] + w[1] * L[1] + w[2] * L[2]);
+    out.coarsening_kinetic_loss_j = out.source_kinetic_energy_j - out.rigid_kinetic_energy_j;
+    out.node_count = count;
+    out.first_node = nodes[0];
+}
+
 } // namespace orc
 
 using namespace orc;
@@ -1423,6 +1502,16 @@ int orc_box26_bonds(uint32_t nx, uint32_t ny, uint32_t nz, double cell, double *
 int orc_components(uint32_t N, uint32_t B, const uint32_t *ba, const uint32_t *bb, const uint8_t *alive, uint32_t *label,
                    uint32_t *count) {
     return guard([&] { *count = components(N, B, ba, bb, alive, label); });
+}
+
+// Mass properties of fragments given as node lists: fragment f is
+// nodes[offsets[f] .. offsets[f+1]); pos, vel, spin (NULL: zero) 3N, mass N.
+int orc_fragment_mass(uint32_t N, const double *pos, const double *vel, const double *spin, const double *mass,
+                      double cell, uint32_t K, const uint32_t *offsets, const uint32_t *nodes, blx_fragment_mass *out) {
+    return guard([&] {
+        for (u32 f = 0; f < K; ++f)
+            fragmentMass(pos, vel, spin, mass, cell, nodes + offsets[f], offsets[f + 1] - offsets[f], N, out[f]);
+    });
 }
 
 int orc_backend_create(int precision, const blx_schedule_view *sv, void **out) {

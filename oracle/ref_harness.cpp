@@ -10,6 +10,8 @@
 //   ref_backend_*        -> makeCpuLatticeBackend / makeParallelCpuLatticeBackend
 //                           (FastLattice.hpp:371-381) and LatticeBackend methods
 //   ref_components       -> findConnectedComponents (ConnectedComponents.cpp:51-87)
+//   ref_fragment_mass    -> calculateFragmentMassProperties
+//                           (FragmentMassProperties.cpp:7-80)
 //   ref_box_slabs        -> boxLatticeSlabs         (BoxLattice.cpp:175-195)
 //   ref_state_substep_limit, ref_state_elastic_energy, ref_state_kinetic_energy
 //                        -> latticeStateSubstepLimit / ElasticEnergy / KineticEnergy
@@ -20,6 +22,7 @@
 #include "fastlattice/FastLattice.hpp"
 #include "fastlattice/LatticeSchedule.hpp"
 #include "fracture/ConnectedComponents.hpp"
+#include "fracture/FragmentMassProperties.hpp"
 #include "matter/BoxLattice.hpp"
 
 #include <atomic>
@@ -304,6 +307,48 @@ int ref_components(uint32_t N, uint32_t B, const uint32_t *ba, const uint32_t *b
         const auto comps = findConnectedComponents(m);
         for (const auto &c : comps) for (std::uint32_t n : c.node_indices) label[n] = c.id;
         *count = std::uint32_t(comps.size());
+    });
+}
+
+int ref_fragment_mass(uint32_t N, const double *pos, const double *vel, const double *spin, const double *mass,
+                      double cell, uint32_t K, const uint32_t *offsets, const uint32_t *nodes, blx_fragment_mass *out) {
+    return guard([&] {
+        LatticeAsset asset;
+        asset.recipe.voxel_size_m = cell;
+        asset.nodes.resize(N);
+        ActiveMatter m;
+        m.asset = &asset;
+        m.nodes.resize(N);
+        for (std::uint32_t i = 0; i < N; ++i) {
+            ActiveNodeState &n = m.nodes[i];
+            n.position_world_m = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+            n.velocity_m_s = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+            n.mass_kg = mass[i];
+            if (spin) n.spin_angular_velocity_rad_s = {spin[3 * i], spin[3 * i + 1], spin[3 * i + 2]};
+        }
+        for (std::uint32_t f = 0; f < K; ++f) {
+            const std::span<const std::uint32_t> idx(nodes + offsets[f], offsets[f + 1] - offsets[f]);
+            const FragmentMassProperties p = calculateFragmentMassProperties(m, idx);
+            blx_fragment_mass &o = out[f];
+            o = blx_fragment_mass{};
+            o.mass_kg = p.mass_kg;
+            const Vec3 *vs[5] = {&p.center_of_mass_world_m, &p.linear_velocity_m_s, &p.angular_momentum_kg_m2_s,
+                                 &p.angular_velocity_rad_s, nullptr};
+            double *dst[4] = {o.center_of_mass_world_m, o.linear_velocity_m_s, o.angular_momentum_kg_m2_s,
+                              o.angular_velocity_rad_s};
+            for (int q = 0; q < 4; ++q) {
+                dst[q][0] = vs[q]->x;
+                dst[q][1] = vs[q]->y;
+                dst[q][2] = vs[q]->z;
+            }
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) o.inertia_world_kg_m2[3 * r + c] = p.inertia_world_kg_m2.m[r][c];
+            o.source_kinetic_energy_j = p.source_kinetic_energy_j;
+            o.rigid_kinetic_energy_j = p.rigid_kinetic_energy_j;
+            o.coarsening_kinetic_loss_j = p.coarsening_kinetic_loss_j;
+            o.node_count = std::uint32_t(idx.size());
+            o.first_node = idx.empty() ? 0U : idx[0];
+        }
     });
 }
 
