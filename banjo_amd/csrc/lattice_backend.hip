@@ -280,7 +280,7 @@ public:
     struct Off {
         size_t x04, u4, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
         size_t nbr_key, spal, sdelta, nbr_bond, epal, nodir, rec, alive, lambda, pext;
-        size_t bnd, alive_n, thr, thr2, pvd, mode, pstr, slots;
+        size_t bnd, alive_n, thr, thr2, pv, pyz, mode, pstr, slots;
         size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
         size_t affected, affected_list, tile_plastic, load_part, fext, freq, cand_all, cand_tile, cand_lo, cand_cnt;
         size_t nbb, first_fail;
@@ -360,7 +360,8 @@ public:
         l.alive_n = arena.at<uint8_t>(o.alive_n);
         l.thr = arena.at<Thresholds<R>>(o.thr);
         l.thr2 = arena.at<Tensile<R>>(o.thr2);
-        l.pvd = arena.at<R4<R>>(o.pvd);
+        l.pv = arena.at<R2<R>>(o.pv);
+        l.pyz = arena.at<R2<R>>(o.pyz);
         l.mode = arena.at<uint8_t>(o.mode);
         l.pstr = arena.at<R>(o.pstr);
         l.slots = arena.at<uint2>(o.slots);
@@ -615,7 +616,8 @@ public:
         o.alive_n = arena.reserve<uint8_t>(B);
         o.thr = arena.reserve<Thresholds<R>>(tensile ? 0 : B);
         o.thr2 = arena.reserve<Tensile<R>>(tensile ? B : 0);
-        o.pvd = arena.reserve<R4<R>>(B);
+        o.pv = arena.reserve<R2<R>>(B);
+        o.pyz = arena.reserve<R2<R>>(B);
         o.mode = arena.reserve<uint8_t>(B);
         o.pstr = arena.reserve<R>(B);
         o.slots = arena.reserve<uint2>(B);
@@ -716,7 +718,7 @@ public:
         {
             std::vector<BondRec<R>> rec(B);
             std::vector<BondNode> bnd(B);
-            std::vector<R4<R>> pvd(B);
+            std::vector<R2<R>> pv(B), pyz(B);
             std::vector<R> pext(B), pstr(B);
             std::vector<uint2> slots(B);
             std::vector<uint8_t> alive_n(B), mode(B);
@@ -737,7 +739,8 @@ public:
                 const double *t = st.threshold + 6 * size_t(k);
                 if (tensile) thr2[q] = {R(t[0]), R(t[1])};
                 else thr[q] = {R(t[0]), R(t[1]), R(t[2]), R(t[3]), R(t[4]), R(t[5])};
-                pvd[q] = {R(st.prev_tensile[k]), R(st.prev_compressive[k]), R(st.prev_shear[k]), R(st.damage[k])};
+                pv[q] = {R(st.prev_tensile[k]), R(st.damage[k])};
+                pyz[q] = {R(st.prev_compressive[k]), R(st.prev_shear[k])};
                 pstr[q] = R(st.plastic_strain[k]);
                 slots[q] = make_uint2(st.bond_slot_a[k], st.bond_slot_b[k]);
                 alive_n[q] = st.alive[k];
@@ -747,7 +750,8 @@ public:
             putv(o.bnd, bnd);
             putv(o.thr, thr);
             putv(o.thr2, thr2);
-            putv(o.pvd, pvd);
+            putv(o.pv, pv);
+            putv(o.pyz, pyz);
             putv(o.pext, pext);
             putv(o.pstr, pstr);
             putv(o.slots, slots);
@@ -1168,13 +1172,25 @@ public:
         HIP_OK(hipSetDevice(device_));
         HIP_OK(hipStreamSynchronize(stream_));
         const uint32_t N = sl.N, B = sl.B;
-        std::vector<R4<R>> u4(N), up4(N), v4(N), pvd(B);
+        if (sl.tensile_only && B > 0 && readCtl(k).step > 0) {
+            // The live bonds' compressive and shear samples are not kept on a
+            // tensile-only lattice: rebuild what the last end sample stored.
+            if (shared_.direct)
+                hipLaunchKernelGGL((k_prev_sample<R, true>), dim3(gridFor(B)), dim3(kWG), 0, stream_, sl.L);
+            else
+                hipLaunchKernelGGL((k_prev_sample<R, false>), dim3(gridFor(B)), dim3(kWG), 0, stream_, sl.L);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipStreamSynchronize(stream_));
+        }
+        std::vector<R4<R>> u4(N), up4(N), v4(N);
+        std::vector<R2<R>> pv(B), pyz(B);
         std::vector<R> pext(B), pstr(B);
         std::vector<uint8_t> mode(B);
         copyDown(u4, sl.L.u4);
         copyDown(up4, sl.L.up4);
         copyDown(v4, sl.L.v4);
-        copyDown(pvd, sl.L.pvd);
+        copyDown(pv, sl.L.pv);
+        copyDown(pyz, sl.L.pyz);
         copyDown(pext, sl.L.pext);
         copyDown(pstr, sl.L.pstr);
         copyDown(mode, sl.L.mode);
@@ -1195,10 +1211,10 @@ public:
         // The end-sample arrays are node-major on the device.
         for (uint32_t o = 0; o < B; ++o) {
             const uint32_t q = sl.node_j_host[o];
-            st.prev_tensile[q] = double(pvd[o].x);
-            st.prev_compressive[q] = double(pvd[o].y);
-            st.prev_shear[q] = double(pvd[o].z);
-            st.damage[q] = double(pvd[o].w);
+            st.prev_tensile[q] = double(pv[o].x);
+            st.prev_compressive[q] = double(pyz[o].x);
+            st.prev_shear[q] = double(pyz[o].y);
+            st.damage[q] = double(pv[o].y);
             st.plastic_strain[q] = double(pstr[o]);
             st.failure_mode[q] = mode[o];
         }
@@ -1540,9 +1556,10 @@ private:
         Slot<R> &sl = slot(k);
         HostFrame<R> f;
         f.step = step;
-        std::vector<R4<R>> u4(sl.N), pvd(sl.B);
+        std::vector<R4<R>> u4(sl.N);
+        std::vector<R2<R>> pv(sl.B);
         copyDown(u4, sl.L.u4);
-        copyDown(pvd, sl.L.pvd);
+        copyDown(pv, sl.L.pv);
         f.u.resize(3 * size_t(sl.N));
         for (uint32_t i = 0; i < sl.N; ++i) {
             f.u[3 * size_t(i)] = float(u4[i].x);
@@ -1552,7 +1569,7 @@ private:
         f.alive.resize(sl.B);
         HIP_OK(hipMemcpy(f.alive.data(), sl.L.alive, sl.B, hipMemcpyDeviceToHost));
         f.damage.resize(sl.B);
-        for (uint32_t o = 0; o < sl.B; ++o) f.damage[sl.node_j_host[o]] = float(pvd[o].w);
+        for (uint32_t o = 0; o < sl.B; ++o) f.damage[sl.node_j_host[o]] = float(pv[o].y);
         f.sphere = readSphere(k);
         sl.frames.push_back(std::move(f));
     }

@@ -126,7 +126,12 @@ template <class R> struct DevLattice {
     u8 *alive_n;      // alive again, in node-major order (written together with alive)
     const Thresholds<R> *thr;
     const Tensile<R> *thr2;
-    R4<R> *pvd;       // prev tensile, compressive, shear; damage
+    R2<R> *pv;        // prev tensile sample; damage
+    // Prev compressive and shear samples. With compression and shear ramps
+    // on, read and written every substep. On a tensile-only lattice they feed
+    // no ramp, so they are written only at upload and at a break and rebuilt
+    // for the live bonds from the end strains at download.
+    R2<R> *pyz;
     u8 *mode;
     R *pstr;
     const uint2 *slots;
@@ -243,12 +248,10 @@ template <class R> __device__ void rinvNode(const DevLattice<R> &L, u32 i, u32 *
         // Counted once per node and break round, however often the node is recomputed.
         if (rank < 3 && atomicExch(&L.rank_stamp[i], stamp) != stamp) atomicAdd(rank_counter, 1U);
     }
+    // The strain pass clears an invalid node's sample flag itself (sb.z), so
+    // the strains of the last end sample stay as it read them until the next
+    // strain pass.
     L.valid[i] = valid;
-    if (!valid) {
-        R4<R> sb = L.sb[i];
-        sb.z = R(0);
-        L.sb[i] = sb;
-    }
 }
 
 // The nodal strain from the cached inverse (LatticePhysics.hpp:833-906).
@@ -271,7 +274,10 @@ template <class R, bool DIRECT, bool PREV> __device__ void strainNode(const DevL
     const R4<R> ui = ldg(&pos[i]);
     u32 next[kChunk];
     for (u32 q = 0; q < kChunk; ++q) next[q] = q < D ? ldStream(&L.nbr_key[size_t(q) * N + i]) : kNone;
-    if (!valid) return;
+    if (!valid) {  // no strain sample at this node (LatticePhysics.hpp:833-836)
+        stg(&L.sb[i], R4<R>{R(0), R(0), R(0), R(0)});
+        return;
+    }
     R acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const V3<R> xi = nodePos<R, DIRECT>(L, i, ui);
     for (u32 k0 = 0; k0 < D; k0 += kChunk) {
@@ -416,7 +422,7 @@ __device__ __forceinline__ V3<R> bondVec(const DevLattice<R> &L, u32 a, u32 b, c
 template <class R, bool DIRECT>
 __device__ __forceinline__ void sampleBondWith(const DevLattice<R> &L, u32 a, u32 b, u32 p, const EdgePal<R> &g,
                                                const R4<R> &ua, const R4<R> &ub, const R4<R> (&st)[4], R &t, R &c,
-                                               R &s) {
+                                               R &s, R shear_floor2 = R(0)) {
     const V3<R> delta = bondVec<R, DIRECT>(L, a, b, g.e, ua, ub);
     const R len = length(delta);
     const R rest = g.e.w;
@@ -437,7 +443,7 @@ __device__ __forceinline__ void sampleBondWith(const DevLattice<R> &L, u32 a, u3
             // st holds (sa, sb) of end a, then of end b; sb.z is the valid flag.
             if (st[2 * q + 1].z == R(0)) continue;
             R nn, sh;
-            resolve(st[2 * q], st[2 * q + 1], dir, nn, sh);
+            resolve(st[2 * q], st[2 * q + 1], dir, nn, sh, shear_floor2);
             t = maxR(t, nn);
             c = maxR(c, -nn);
             s = maxR(s, sh);
@@ -478,15 +484,25 @@ template <class R> __global__ void k_edge_palette(EdgePal<R> *pal, u32 *no_dir, 
     }
 }
 
+__device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, float ms, float mp,
+                                              float (*s_max)[kWG / 64]);
+
 // The start sample of the live bonds at dirty nodes (LatticePhysics.hpp:1118-1126).
 // A bond with no dirty end samples exactly what its end sample stored, so
 // only the bonds in dirty nodes' slots are resampled; a bond with both ends
-// dirty is written twice with the same bits.
+// dirty is written twice with the same bits. On a tensile-only lattice the
+// compressive and shear samples are not stored; their only other use, the
+// running maxima of the end sample (which take the larger of a bond's
+// previous and current sample), gets them here instead. A bond that was not
+// resampled holds the last end sample's values, which that sample already
+// counted; upload marks every node dirty, so the first substep resamples all.
 template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_start(Batch<R> B) {
     const WgMap m = B.map[blockIdx.x];
     const DevLattice<R> L = B.L[m.lat];
     const u32 half = u32(L.ctl->step & 1U) ^ 1U;  // the previous substep's half
     if (L.ctl->stop || !L.ctl->dirty_start[half]) return;
+    __shared__ float s_max[4][kWG / 64];
+    float mc = 0.f, ms = 0.f;
     const u64 pairs = u64(L.ctl->dirty_count[half]) * L.D;
     const u32 *list = L.dirty_list + size_t(half) * L.N;
     for (u64 q = u64(m.a) * kWG + threadIdx.x; q < pairs; q += u64(m.b) * kWG) {
@@ -497,11 +513,29 @@ template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_st
         R t, c, s;
         // Positions at substep start (up4): this runs after k_predict.
         sampleBond<R, DIRECT>(L, r.a, r.b, r.p, L.epal[r.p & ~kNoDir], L.up4[r.a], L.up4[r.b], t, c, s);
-        R4<R> pv = L.pvd[o];
-        pv.x = maxR(R(0), t);
-        pv.y = maxR(R(0), c);
-        pv.z = maxR(R(0), s);
-        L.pvd[o] = pv;
+        L.pv[o].x = maxR(R(0), t);
+        const R pc = maxR(R(0), c), ps = maxR(R(0), s);
+        if (L.tensile_only) {
+            mc = fmaxf(mc, float(pc));
+            ms = fmaxf(ms, float(ps));
+        } else {
+            L.pyz[o] = R2<R>{pc, ps};
+        }
+    }
+    if (L.tensile_only) publishMaxima(L.ctl, 0.f, mc, ms, 0.f, s_max);
+}
+
+// The previous compressive and shear samples of the live bonds of a
+// tensile-only lattice, rebuilt from the last end sample's inputs (the end
+// positions and end strains, which stay in place until the next substep), for
+// a download: the values that end sample would have stored.
+template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_prev_sample(DevLattice<R> L) {
+    for (u32 o = blockIdx.x * kWG + threadIdx.x; o < L.B; o += gridDim.x * kWG) {
+        if (!L.alive_n[o]) continue;
+        const BondNode r = L.bnd[o];
+        R t, c, s;
+        sampleBond<R, DIRECT>(L, r.a, r.b, r.p, L.epal[r.p & ~kNoDir], L.u4[r.a], L.u4[r.b], t, c, s);
+        L.pyz[o] = R2<R>{maxR(R(0), c), maxR(R(0), s)};
     }
 }
 
@@ -1124,11 +1158,11 @@ enum BondCover : int { kBondAll = 0, kBondUnaffected = 2, kBondAffected = 3 };
 // before it starts on the first, so the span's first memory round trips overlap.
 template <class R> struct EndLoad {
     BondNode r;
-    R4<R> pv;
+    R2<R> pv;
     u32 live;
 };
 template <class R> __device__ __forceinline__ EndLoad<R> loadEnd(const DevLattice<R> &L, u32 o) {
-    return {ldStream(&L.bnd[o]), ldStream(&L.pvd[o]), u32(ldStream(&L.alive_n[o]))};
+    return {ldStream(&L.bnd[o]), ldStream(&L.pv[o]), u32(ldStream(&L.alive_n[o]))};
 }
 
 // One bond's end sample, damage and break. Returns through the references the
@@ -1136,21 +1170,23 @@ template <class R> __device__ __forceinline__ EndLoad<R> loadEnd(const DevLattic
 // it claims for the dirty list.
 template <class R, bool DIRECT, bool PLASTIC, int COVER>
 __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step<R> &S, u32 o, const EndLoad<R> &in,
-                                              double m0, u32 stamp, float &mt, float &mc, float &ms, float &mp,
-                                              double &pw, u32 &claim_a, u32 &claim_b) {
+                                              double m0, u32 stamp, R shear_floor2, float &mt, float &mc, float &ms,
+                                              float &mp, double &pw, u32 &claim_a, u32 &claim_b) {
     const BondNode r = in.r;
-    R4<R> pv = in.pv;
+    R2<R> pv = in.pv;
     if (!in.live) return;
     const u32 j = r.j;
     // Every gather the bond needs goes out in one batch, before any test on
     // what it returns: one memory round trip after the record, not three.
     Thresholds<R> th{};
+    R2<R> pyz{R(0), R(0)};
     if (L.tensile_only) {
         const Tensile<R> t2 = ldStream(&L.thr2[o]);
         th.ds = t2.ds;
         th.de = t2.de;
     } else {
         th = L.thr[o];
+        pyz = ldStream(&L.pyz[o]);
     }
     u32 aff_a = 0, aff_b = 0;
     if (COVER == kBondUnaffected) {
@@ -1189,14 +1225,19 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
         mp = fmaxf(mp, float(absR(plastic) / restL));
     }
     R t, c, s;
-    sampleBondWith<R, DIRECT>(L, r.a, r.b, r.p, g, ua, ub, st, t, c, s);
-    const R pt = maxR(R(0), maxR(pv.x, t)), pc = maxR(R(0), maxR(pv.y, c)), pss = maxR(R(0), maxR(pv.z, s));
+    // On a tensile-only lattice the shear only feeds the running maximum (and
+    // the stored sample at a break, recomputed there), so a shear that cannot
+    // raise the maximum skips its square root.
+    sampleBondWith<R, DIRECT>(L, r.a, r.b, r.p, g, ua, ub, st, t, c, s, L.tensile_only ? shear_floor2 : R(0));
+    // The previous compressive and shear samples of a tensile-only lattice
+    // were counted in the maxima when they were taken (k_bond_start).
+    const R pt = maxR(R(0), maxR(pv.x, t)), pc = maxR(R(0), maxR(pyz.x, c)), pss = maxR(R(0), maxR(pyz.y, s));
     mt = fmaxf(mt, float(pt));
     mc = fmaxf(mc, float(pc));
     ms = fmaxf(ms, float(pss));
     pv.x = maxR(R(0), t);
-    pv.y = maxR(R(0), c);
-    pv.z = maxR(R(0), s);
+    pyz.x = maxR(R(0), c);
+    pyz.y = maxR(R(0), s);
     // With every compression and shear start non-finite their ramps are an
     // exact 0 (damageProgress), which never wins below.
     const R td = progress(pt, th.ds, th.de);
@@ -1206,16 +1247,22 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
     u8 md = 1;
     if (cd > dmg) { dmg = cd; md = 2; }
     if (sd > dmg) { dmg = sd; md = 3; }
-    if (dmg > pv.w) {
-        pv.w = dmg;
+    if (dmg > pv.y) {
+        pv.y = dmg;
         L.mode[o] = md;
     }
-    stStream(&L.pvd[o], pv);
-    const float event_damage = float(pv.w);
+    stStream(&L.pv[o], pv);
+    if (!L.tensile_only) stStream(&L.pyz[o], pyz);
+    const float event_damage = float(pv.y);
     bool event = double(event_damage) > m0 + 1.0e-6;
     R event_energy = R(0);
-    const bool broke = pv.w >= R(1);
+    const bool broke = pv.y >= R(1);
     if (broke) {
+        if (L.tensile_only) {  // the samples a dead bond keeps, with the exact shear
+            R t2, c2, s2;
+            sampleBondWith<R, DIRECT>(L, r.a, r.b, r.p, g, ua, ub, st, t2, c2, s2);
+            L.pyz[o] = R2<R>{maxR(R(0), c2), maxR(R(0), s2)};
+        }
         const R ext = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
         const R cc = L.rec[j].c;
         event_energy = cc <= R(0) ? R(0) : R(0.5) * ext * ext / cc;
@@ -1309,6 +1356,10 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
     const double m0 = ctl->max_damage;
     const u32 stamp = u32(ctl->step) + 1U;
     const u32 half = u32(ctl->step & 1U);
+    // A shear below the running maximum read here (which only grows) cannot
+    // raise it; the margin keeps the rounded square root at or below it.
+    const R shear_max = R(__uint_as_float(ctl->peak_bits[2]));
+    const R shear_floor2 = shear_max * shear_max * R(1.0 - 1.0 / 1048576.0);
     if (COVER != kBondAffected && m->a == 0 && threadIdx.x == 0) {
         ctl->rank_slot = u32(ctl->step & 1U);
         ctl->rinv_stamp = stamp;
@@ -1328,8 +1379,8 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
                 if (o != kNone) {
                     const u32 a = L.bnd[o].a;
                     if (a == i || L.affected[a] != stamp)
-                        endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, loadEnd(L, o), m0, stamp, mt, mc, ms, mp, pw,
-                                                                 claim_a, claim_b);
+                        endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, loadEnd(L, o), m0, stamp, shear_floor2, mt,
+                                                                 mc, ms, mp, pw, claim_a, claim_b);
                 }
             }
             appendDirty(L.dirty_list + size_t(half) * L.N, &ctl->dirty_count[half], claim_a, claim_b);
@@ -1344,7 +1395,9 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
         const u32 tile = m->a + q, o = tile * kWG + threadIdx.x;
         u32 claim_a = kNone, claim_b = kNone;
         double pw = 0.0;
-        if (o < L.B) endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, in[q], m0, stamp, mt, mc, ms, mp, pw, claim_a, claim_b);
+        if (o < L.B)
+            endSampleBond<R, DIRECT, PLASTIC, COVER>(L, S, o, in[q], m0, stamp, shear_floor2, mt, mc, ms, mp, pw,
+                                                     claim_a, claim_b);
         appendDirty(L.dirty_list + size_t(half) * L.N, &ctl->dirty_count[half], claim_a, claim_b);
         if (PLASTIC) {
             s_pw[threadIdx.x] = pw;

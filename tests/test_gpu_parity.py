@@ -23,6 +23,14 @@ EXACT_CONTACT = ("impulse_contacts", "impulse_to_material", "dissipated_kinetic_
                  "maximum_position_correction_m", "maximum_center_shift_m", "candidate_overflow", "ball_support_events")
 
 
+def by_asset(res, name):
+    """A per-bond state array in asset bond order (the state holds schedule order)."""
+    arr = getattr(res["state"], name)
+    out = np.empty_like(arr)
+    out[res["schedule"].bond_order] = arr
+    return out
+
+
 def assert_same(a, b, ledger_rtol=1e-12):
     assert np.array_equal(a["u"], b["u"]), f"max |du| {np.abs(a['u'] - b['u']).max()}"
     assert np.array_equal(a["v"], b["v"]), f"max |dv| {np.abs(a['v'] - b['v']).max()}"
@@ -32,6 +40,9 @@ def assert_same(a, b, ledger_rtol=1e-12):
     assert np.array_equal(a["mode"], b["mode"])
     assert np.array_equal(a["labels"], b["labels"])
     assert np.array_equal(a["first_failure"], b["first_failure"])
+    if "state" in a and "state" in b:
+        for name in ("prev_tensile", "prev_compressive", "prev_shear", "plastic_extension", "plastic_strain"):
+            assert np.array_equal(by_asset(a, name), by_asset(b, name)), name
     sa, sb = a["status"], b["status"]
     for f in EXACT_STATUS:
         assert getattr(sa, f) == getattr(sb, f), f
