@@ -707,25 +707,20 @@ template <class R> __device__ __forceinline__ Spec<R> speculate(const Step<R> &S
     return o;
 }
 
-// Returns true when a position correction moved the sphere centre.
+// The contact of a candidate that passed the contact test at the current
+// sphere state (a node of positive mass whose gap is within the margin or
+// the closing distance). Its normal impulse ni = max(0, (dvn - vn) / (1/m +
+// 1/M + crn / I)) was evaluated at that same state by the caller, and inm,
+// ism, iin are 1/m, 1/M and 1/I: the same expressions on the same inputs as
+// the reference's in-line ones, taken off the serial chain. Returns true when
+// a position correction moved the sphere centre.
 template <class R>
-__device__ bool contactNode(const Step<R> &S, Sphere<R> &sp, const Spec<R> &q, V3<R> &u, V3<R> &v, R mass,
-                            bool correct, ContactAcc<R> &acc) {
-    if (mass <= R(0)) return false;
-    const R dt = S.dt;
-    const R ism = R(1) / sp.mass;
-    const R iin = R(1) / sp.inertia;
+__device__ bool contactNode(const Step<R> &S, Sphere<R> &sp, const Spec<R> &q, V3<R> &u, V3<R> &v, R mass, R inm,
+                            R ism, R iin, R ni, bool correct, ContactAcc<R> &acc) {
     const V3<R> r = q.r, n = q.n;
     const R gap = q.gap;
     V3<R> nv = v;
-    const V3<R> rel = nv - sp.velocity - cross(sp.omega, r);
-    const R vn = dot(rel, n);
-    if (gap > maxR(S.c_margin, -vn * dt)) return false;
     if (-gap > R(acc.max_pen)) acc.max_pen = double(-gap);
-    const R inm = R(1) / mass;
-    const R rest = -vn > S.c_rest_thr ? S.c_rest : R(0);
-    const R dvn = gap > S.c_margin ? -gap / dt : -rest * minR(vn, R(0));
-    const R ni = maxR(R(0), (dvn - vn) / (inm + ism + iin * q.crn));
     if (ni > R(0)) {
         const R before = R(0.5) * mass * length2(nv);
         V3<R> imp = ni * n;
@@ -868,6 +863,8 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
     ContactAcc<R> acc;
     Sphere<R> sp = *L.sphere;
     if (PASS == 1) sp.velocity = sp.velocity + S.dt * S.gravity;
+    // The sphere's mass and inertia are constant through the pass.
+    const R ism = R(1) / sp.mass, iin = R(1) / sp.inertia;
     acc.contacts = ctl->impulse_contacts;
     for (int k = 0; k < 3; ++k) acc.imp[k] = ctl->impulse[k];
     acc.dissipated = ctl->dissipated;
@@ -902,8 +899,10 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
                 const Spec<R> q = speculate(S, sp, x0, mu);
                 const u32 span = min(64U, n - k0);
                 u32 next = k0 + span;
-                // The impulse denominator of contactNode, a per-candidate constant.
-                const R den = R(1) / mass + R(1) / sp.mass + R(1) / sp.inertia * q.crn;
+                // The impulse denominator (LatticePhysics.hpp:1344-1346), a per-candidate constant.
+                const R inm = R(1) / mass;
+                const R den = inm + ism + iin * q.crn;
+                const R closing = -q.gap / S.dt;  // the desired normal speed of a gap beyond the margin
                 // A candidate changes nothing but the maximum penetration unless
                 // it passes the contact test (LatticePhysics.hpp:1341) and then
                 // takes an impulse or, in pass 2, a position correction. Both
@@ -920,7 +919,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
                     const bool touch = lane >= pos && lane < span && !(mass <= R(0)) &&
                                        !(q.gap > maxR(S.c_margin, -vn * S.dt));
                     const R rest = -vn > S.c_rest_thr ? S.c_rest : R(0);
-                    const R dvn = q.gap > S.c_margin ? -q.gap / S.dt : -rest * minR(vn, R(0));
+                    const R dvn = q.gap > S.c_margin ? closing : -rest * minR(vn, R(0));
                     const R ni = maxR(R(0), (dvn - vn) / den);
                     const bool acts = touch && (ni > R(0) || (PASS == 2 && q.gap < -S.c_margin));
                     const u64 am = __ballot(acts);
@@ -938,7 +937,8 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
                     qw.gap = laneValue(q.gap, w);
                     qw.crn = laneValue(q.crn, w);
                     V3<R> u = laneValue(mu, w), v = laneValue(mv, w);
-                    const bool moved = contactNode<R>(S, sp, qw, u, v, laneValue(mass, w), PASS == 2, acc);
+                    const bool moved = contactNode<R>(S, sp, qw, u, v, laneValue(mass, w), laneValue(inm, w), ism,
+                                                      iin, laneValue(ni, w), PASS == 2, acc);
                     if (lane == w) {
                         mu = u;
                         mv = v;
