@@ -687,11 +687,6 @@ template <class R> struct ContactAcc {
     double imp[3], dissipated, max_pen, max_corr;
 };
 
-__device__ __forceinline__ bool bitsEqual(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
-__device__ __forceinline__ bool bitsEqual(double a, double b) {
-    return __double_as_longlong(a) == __double_as_longlong(b);
-}
-
 template <class R> struct Spec {
     V3<R> r, n;
     R gap, crn;  // crn = |r x n|^2
@@ -709,60 +704,65 @@ template <class R> __device__ __forceinline__ Spec<R> speculate(const Step<R> &S
 
 // The contact of a candidate that passed the contact test at the current
 // sphere state (a node of positive mass whose gap is within the margin or
-// the closing distance). Its normal impulse ni = max(0, (dvn - vn) / (1/m +
-// 1/M + crn / I)) was evaluated at that same state by the caller, and inm,
-// ism, iin are 1/m, 1/M and 1/I: the same expressions on the same inputs as
-// the reference's in-line ones, taken off the serial chain. Returns true when
-// a position correction moved the sphere centre.
+// the closing distance), in two parts that read nothing the other writes: the
+// impulses (LatticePhysics.hpp:1350-1376), which change the node's velocity
+// and the sphere's velocity and spin, and pass 2's position correction
+// (:1377-1386), which moves the node and the sphere centre by the gap alone.
+// The normal impulse ni = max(0, (dvn - vn) / (1/m + 1/M + crn / I)) was
+// evaluated at the same sphere state by the caller, and inm, ism, iin are 1/m,
+// 1/M and 1/I: the reference's in-line expressions on the same inputs, taken
+// off the serial chain. The impulse is written without branches (every value
+// is computed and the applied ones selected) so that, on the chain, it can
+// overlap the re-speculation a centre move calls for.
 template <class R>
-__device__ bool contactNode(const Step<R> &S, Sphere<R> &sp, const Spec<R> &q, V3<R> &u, V3<R> &v, R mass, R inm,
-                            R ism, R iin, R ni, bool correct, ContactAcc<R> &acc) {
+__device__ __forceinline__ void impulseNode(const Step<R> &S, Sphere<R> &sp, const Spec<R> &q, V3<R> &v, R mass,
+                                            R inm, R ism, R iin, R ni, ContactAcc<R> &acc) {
     const V3<R> r = q.r, n = q.n;
-    const R gap = q.gap;
-    V3<R> nv = v;
-    if (-gap > R(acc.max_pen)) acc.max_pen = double(-gap);
-    if (ni > R(0)) {
-        const R before = R(0.5) * mass * length2(nv);
-        V3<R> imp = ni * n;
-        nv = nv + inm * imp;
-        sp.velocity = sp.velocity - ism * imp;
-        sp.omega = sp.omega + iin * (-cross(r, imp));
-        acc.imp[0] += double(imp.x);
-        acc.imp[1] += double(imp.y);
-        acc.imp[2] += double(imp.z);
-        const V3<R> anr = nv - sp.velocity - cross(sp.omega, r);
-        const V3<R> tan = anr - dot(anr, n) * n;
-        const R ts = length(tan);
-        if (ts > R(1.0e-12)) {
-            const V3<R> dir = tan / ts;
-            const R stick = ts / (inm + ism + iin * length2(cross(r, dir)));
-            const R fi = stick <= S.c_sf * ni ? stick : minR(stick, S.c_df * ni);
-            imp = -fi * dir;
-            nv = nv + inm * imp;
-            sp.velocity = sp.velocity - ism * imp;
-            sp.omega = sp.omega + iin * (-cross(r, imp));
-            acc.imp[0] += double(imp.x);
-            acc.imp[1] += double(imp.y);
-            acc.imp[2] += double(imp.z);
-        }
+    const bool on = ni > R(0);
+    const R before = R(0.5) * mass * length2(v);
+    const V3<R> imp = ni * n;
+    const V3<R> nv1 = v + inm * imp;
+    const V3<R> vel1 = sp.velocity - ism * imp;
+    const V3<R> om1 = sp.omega + iin * (-cross(r, imp));
+    const double i1[3] = {acc.imp[0] + double(imp.x), acc.imp[1] + double(imp.y), acc.imp[2] + double(imp.z)};
+    const V3<R> anr = nv1 - vel1 - cross(om1, r);
+    const V3<R> tan = anr - dot(anr, n) * n;
+    const R ts = length(tan);
+    const bool fr = on && ts > R(1.0e-12);
+    const V3<R> dir = tan / ts;
+    const R stick = ts / (inm + ism + iin * length2(cross(r, dir)));
+    const R fi = stick <= S.c_sf * ni ? stick : minR(stick, S.c_df * ni);
+    const V3<R> imp2 = -fi * dir;
+    const V3<R> nv2 = nv1 + inm * imp2;
+    const V3<R> vel2 = vel1 - ism * imp2;
+    const V3<R> om2 = om1 + iin * (-cross(r, imp2));
+    const double i2[3] = {i1[0] + double(imp2.x), i1[1] + double(imp2.y), i1[2] + double(imp2.z)};
+    const V3<R> nv = fr ? nv2 : nv1;
+    const double gone = double(before - R(0.5) * mass * length2(nv));
+    if (on) {
+        sp.velocity = fr ? vel2 : vel1;
+        sp.omega = fr ? om2 : om1;
+        for (int k = 0; k < 3; ++k) acc.imp[k] = fr ? i2[k] : i1[k];
         v = nv;
-        acc.dissipated += double(before - R(0.5) * mass * length2(nv));
+        acc.dissipated += gone;
         ++acc.contacts;
     }
-    if (correct && gap < -S.c_margin) {
-        const R corr = minR(-gap - S.c_margin, R(0.2) * maxR(S.c_node_radius, sp.radius * R(0.01)));
-        const R sum = inm + ism;
-        const V3<R> ns = (corr * inm / sum) * n;
-        const V3<R> ss = -((corr * ism / sum) * n);
-        u = u + ns;
-        const V3<R> before_center = sp.center;
-        sp.center = sp.center + ss;
-        if (corr > R(acc.max_corr)) acc.max_corr = double(corr);
-        // Bitwise: a shift below one ulp leaves the speculated terms valid.
-        return !bitsEqual(sp.center.x, before_center.x) || !bitsEqual(sp.center.y, before_center.y) ||
-               !bitsEqual(sp.center.z, before_center.z);
-    }
-    return false;
+}
+
+// Pass 2's position correction of the node and the sphere centre.
+template <class R>
+__device__ __forceinline__ void correctNode(const Step<R> &S, Sphere<R> &sp, const Spec<R> &q, V3<R> &u, R inm, R ism,
+                                            ContactAcc<R> &acc) {
+    const R gap = q.gap;
+    const V3<R> n = q.n;
+    if (!(gap < -S.c_margin)) return;
+    const R corr = minR(-gap - S.c_margin, R(0.2) * maxR(S.c_node_radius, sp.radius * R(0.01)));
+    const R sum = inm + ism;
+    const V3<R> ns = (corr * inm / sum) * n;
+    const V3<R> ss = -((corr * ism / sum) * n);
+    u = u + ns;
+    sp.center = sp.center + ss;
+    if (corr > R(acc.max_corr)) acc.max_corr = double(corr);
 }
 
 // Broadcast of one lane's value to the wave (the lane index is uniform).
@@ -790,8 +790,8 @@ constexpr u32 kMaxNodeTiles = 16384;  // candidate tile prefix held in LDS: up t
 // the window in order with the walked candidate's values broadcast, so the
 // sphere state is uniform and nothing on the chain waits on memory. Candidates
 // whose contact test fails at the current sphere state are skipped by a ballot
-// over the window. A centre move (pass 2) re-speculates the window from the
-// next candidate.
+// over the window. A centre move (pass 2) re-speculates the window's
+// remaining candidates.
 template <class R, int PASS>
 __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) {
     const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
@@ -896,13 +896,12 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
                     mu = xyz(s_u[k]);
                     mv = xyz(s_v[k]);
                 }
-                const Spec<R> q = speculate(S, sp, x0, mu);
+                Spec<R> q = speculate(S, sp, x0, mu);
                 const u32 span = min(64U, n - k0);
-                u32 next = k0 + span;
-                // The impulse denominator (LatticePhysics.hpp:1344-1346), a per-candidate constant.
+                // The impulse denominator (LatticePhysics.hpp:1344-1346), constant while the centre is.
                 const R inm = R(1) / mass;
-                const R den = inm + ism + iin * q.crn;
-                const R closing = -q.gap / S.dt;  // the desired normal speed of a gap beyond the margin
+                R den = inm + ism + iin * q.crn;
+                R closing = -q.gap / S.dt;  // the desired normal speed of a gap beyond the margin
                 // A candidate changes nothing but the maximum penetration unless
                 // it passes the contact test (LatticePhysics.hpp:1341) and then
                 // takes an impulse or, in pass 2, a position correction. Both
@@ -911,7 +910,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
                 // chain jumps to the first lane that acts; the lanes before it
                 // fold their penetration into the running maximum in order. An
                 // impulse changes the sphere state, so the remaining lanes are
-                // evaluated again.
+                // evaluated again; a centre move (pass 2) re-speculates them.
                 u32 pos = 0;
                 while (pos < span) {
                     const V3<R> rel = mv - sp.velocity - cross(sp.omega, q.r);
@@ -937,23 +936,31 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
                     qw.gap = laneValue(q.gap, w);
                     qw.crn = laneValue(q.crn, w);
                     V3<R> u = laneValue(mu, w), v = laneValue(mv, w);
-                    const bool moved = contactNode<R>(S, sp, qw, u, v, laneValue(mass, w), laneValue(inm, w), ism,
-                                                      iin, laneValue(ni, w), PASS == 2, acc);
+                    const R mw = laneValue(mass, w), iw = laneValue(inm, w), nw = laneValue(ni, w);
+                    if (-qw.gap > R(acc.max_pen)) acc.max_pen = double(-qw.gap);
+                    if (PASS == 2) {
+                        // The correction reads only the gap and the inverse
+                        // masses, so it goes first and the window's
+                        // re-speculation at the new centre overlaps the
+                        // impulse; a centre that did not move gives the same
+                        // bits again.
+                        correctNode<R>(S, sp, qw, u, iw, ism, acc);
+                        q = speculate(S, sp, x0, mu);
+                        den = inm + ism + iin * q.crn;
+                        closing = -q.gap / S.dt;
+                    }
+                    impulseNode<R>(S, sp, qw, v, mw, iw, ism, iin, nw, acc);
                     if (lane == w) {
                         mu = u;
                         mv = v;
                     }
-                    if (moved) {
-                        next = k0 + w + 1;
-                        break;
-                    }
                     pos = w + 1;
                 }
-                if (have && k < next) {
+                if (have) {
                     s_u[k] = with(mu, s_u[k].w);
                     s_v[k] = with(mv, R(0));
                 }
-                k0 = next;
+                k0 += span;
             }
         }
         __syncthreads();
