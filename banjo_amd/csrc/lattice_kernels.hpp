@@ -439,14 +439,34 @@ __device__ __forceinline__ void sampleBondWith(const DevLattice<R> &L, u32 a, u3
     s = R(0);
     if (!(p & kNoDir)) {
         const V3<R> dir{g.d.y, g.d.z, g.d.w};
+        // st holds (sa, sb) of end a, then of end b; sb.z is the valid flag.
+        // resolve (LatticePhysics.hpp:911-919) with the shear's square root
+        // taken apart: the normals first, then the roots of the shears that
+        // are not below shear_floor2 (0: all of them).
+        R sq[2] = {R(0), R(0)};
+        bool on[2];
         for (int q = 0; q < 2; ++q) {
-            // st holds (sa, sb) of end a, then of end b; sb.z is the valid flag.
-            if (st[2 * q + 1].z == R(0)) continue;
-            R nn, sh;
-            resolve(st[2 * q], st[2 * q + 1], dir, nn, sh, shear_floor2);
+            on[q] = st[2 * q + 1].z != R(0);
+            if (!on[q]) continue;
+            const R4<R> &ea = st[2 * q], &eb = st[2 * q + 1];
+            const V3<R> tr{ea.x * dir.x + ea.w * dir.y + eb.x * dir.z, ea.w * dir.x + ea.y * dir.y + eb.y * dir.z,
+                           eb.x * dir.x + eb.y * dir.y + ea.z * dir.z};
+            const R nn = dot(dir, tr);
+            sq[q] = length2(tr - nn * dir);
             t = maxR(t, nn);
             c = maxR(c, -nn);
-            s = maxR(s, sh);
+        }
+        const bool root[2] = {on[0] && !(sq[0] < shear_floor2), on[1] && !(sq[1] < shear_floor2)};
+        if (root[0] || root[1]) {
+            // A real branch: a wave none of whose shears can raise the running
+            // maximum skips both square roots (an empty volatile asm keeps the
+            // compiler from evaluating them anyway and selecting).
+            asm volatile("");
+            for (int q = 0; q < 2; ++q)
+                if (on[q]) s = maxR(s, root[q] ? sqrtR(sq[q]) : R(0));
+        } else {
+            for (int q = 0; q < 2; ++q)
+                if (on[q]) s = maxR(s, R(0));
         }
     }
 }
