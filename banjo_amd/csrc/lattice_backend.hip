@@ -16,6 +16,7 @@
 #include "lattice_kernels.hpp"
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <bit>
@@ -1654,29 +1655,31 @@ private:
                        small = batch(small_map_), lats = batch(lattice_map_);
         const dim3 ng = tiles(node_tiles_map_), bg = tiles(bond_tiles_map_), lg = entries(list_map_),
                    sg = entries(small_map_), fg = entries(lattice_map_);
+        // An event another stream waits on rides on the kernel it follows
+        // (hipExtLaunchKernelGGL's stop event): a separate hipEventRecord puts a
+        // marker between two kernels of the stream, which cost about 6 us each.
         profBegin(KPredict, 3);
-        if (sphere && any_loads_) hipLaunchKernelGGL((k_predict<R, true, true, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
-        else if (sphere) hipLaunchKernelGGL((k_predict<R, true, false, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
-        else if (any_loads_) hipLaunchKernelGGL((k_predict<R, false, true, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
-        else hipLaunchKernelGGL((k_predict<R, false, false, DIRECT>), ng, dim3(kWG), 0, stream_, nodes, S);
+        if (sphere && any_loads_) hipExtLaunchKernelGGL((k_predict<R, true, true, DIRECT>), ng, dim3(kWG), 0, stream_, nullptr, ev_predict_, 0, nodes, S);
+        else if (sphere) hipExtLaunchKernelGGL((k_predict<R, true, false, DIRECT>), ng, dim3(kWG), 0, stream_, nullptr, ev_predict_, 0, nodes, S);
+        else if (any_loads_) hipExtLaunchKernelGGL((k_predict<R, false, true, DIRECT>), ng, dim3(kWG), 0, stream_, nullptr, ev_predict_, 0, nodes, S);
+        else hipExtLaunchKernelGGL((k_predict<R, false, false, DIRECT>), ng, dim3(kWG), 0, stream_, nullptr, ev_predict_, 0, nodes, S);
         profEnd();
         // The start sample of the nodes and bonds the last failure round made
         // dirty reads the positions at substep start from up4, so it runs on
         // side_ after k_predict, behind the previous substep's k_rinv on the
         // same stream, and overlaps the striker pass and the sweep; nothing
         // before the end sample reads what it writes.
-        HIP_OK(hipEventRecord(ev_predict_, stream_));
         HIP_OK(hipStreamWaitEvent(side_, ev_predict_, 0));
         profBegin(KStrainStart, 1, side_);
         hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainDirty>), lg, dim3(kWG), 0, side_, list);
         profEnd(side_);
         profBegin(KBondStart, 2, side_);
-        hipLaunchKernelGGL((k_bond_start<R, DIRECT>), lg, dim3(kWG), 0, side_, list);
+        hipExtLaunchKernelGGL((k_bond_start<R, DIRECT>), lg, dim3(kWG), 0, side_, nullptr, ev_start_, 0, list);
         profEnd(side_);
-        HIP_OK(hipEventRecord(ev_start_, side_));
-        if (sphere) BLX_LAUNCH(KPass1, 4, hipLaunchKernelGGL((k_pass<R, 1>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
+        if (sphere)
+            BLX_LAUNCH(KPass1, 4, hipExtLaunchKernelGGL((k_pass<R, 1>), fg, dim3(kFinishThreads), 0, stream_, nullptr,
+                                                        split ? ev_pass1_ : nullptr, 0, lats, S));
         if (split) {
-            HIP_OK(hipEventRecord(ev_pass1_, stream_));
             HIP_OK(hipStreamWaitEvent(early_, ev_pass1_, 0));
             profBegin(KMark, 4, early_);
             hipLaunchKernelGGL((k_mark_affected<R>), entries(mark_map_), dim3(kWG), 0, early_, batch(mark_map_));
@@ -1707,9 +1710,11 @@ private:
         }
         HIP_OK(hipStreamWaitEvent(stream_, ev_start_, 0));
         profBegin(KVelocity, 8);
+        // With no damping after it, the velocity update carries the event the early stream waits on.
+        hipEvent_t after_velocity = split && !(S.damping > R(0)) ? ev_velocity_ : nullptr;
         if (S.plane_count > 0 || S.blade.enabled)
-            hipLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nodes, S);
-        else hipLaunchKernelGGL((k_velocity<R, false>), ng, dim3(kWG), 0, stream_, nodes, S);
+            hipExtLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nullptr, after_velocity, 0, nodes, S);
+        else hipExtLaunchKernelGGL((k_velocity<R, false>), ng, dim3(kWG), 0, stream_, nullptr, after_velocity, 0, nodes, S);
         profEnd();
         if (S.damping > R(0)) {
             for (const MapRef &m : stage_maps_)
@@ -1717,20 +1722,21 @@ private:
             BLX_LAUNCH(KSupportVelocity, 9, hipLaunchKernelGGL((k_support_velocity<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
         if (split) {
-            HIP_OK(hipEventRecord(ev_velocity_, stream_));
+            if (!after_velocity) HIP_OK(hipEventRecord(ev_velocity_, stream_));
             HIP_OK(hipStreamWaitEvent(early_, ev_velocity_, 0));
             profBegin(KStrain, 11, early_);
             hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), ng, dim3(kWG), 0, early_, nodes);
             profEnd(early_);
             profBegin(KBondEnd, 12, early_);
-            hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), 0, early_, bonds, S);
+            hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), 0, early_, nullptr,
+                                  ev_early_, 0, bonds, S);
             profEnd(early_);
-            HIP_OK(hipEventRecord(ev_early_, early_));
             BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
             BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), sg, dim3(kWG), 0, stream_, small));
             HIP_OK(hipStreamWaitEvent(stream_, ev_early_, 0));
             BLX_LAUNCH(KBondEndLate, 12,
-                       hipLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), 0, stream_, small, S));
+                       hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), 0, stream_,
+                                             nullptr, ev_bond_end_, 0, small, S));
         } else {
             if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
             if (contact) BLX_LAUNCH(KContact, 14, hipLaunchKernelGGL((k_nc_stale<R>), ng, dim3(kWG), 0, stream_, nodes, S));
@@ -1760,12 +1766,11 @@ private:
         // The dirty nodes' inverse rest covariances overlap the ordered folds
         // and the next substep's k_predict; the next start sample follows them
         // on side_.
-        HIP_OK(hipEventRecord(ev_bond_end_, stream_));
+        if (!split) HIP_OK(hipEventRecord(ev_bond_end_, stream_));
         HIP_OK(hipStreamWaitEvent(side_, ev_bond_end_, 0));
         profBegin(KRinv, 1, side_);
-        hipLaunchKernelGGL((k_rinv<R>), lg, dim3(kWG), 0, side_, list);
+        hipExtLaunchKernelGGL((k_rinv<R>), lg, dim3(kWG), 0, side_, nullptr, ev_rinv_, 0, list);
         profEnd(side_);
-        HIP_OK(hipEventRecord(ev_rinv_, side_));
         profBegin(KFinish, 13);
         if (any_loads_ && plastic) hipLaunchKernelGGL((k_finish<R, true, true>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
         else if (any_loads_) hipLaunchKernelGGL((k_finish<R, true, false>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
