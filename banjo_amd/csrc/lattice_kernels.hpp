@@ -1538,6 +1538,21 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     __shared__ u32 s_take;
     __shared__ u32 s_flags[kFlagsInLds];
     const u32 tid = threadIdx.x, lane = tid & 63U;
+    // The control block and the sphere as the substep left them, copied to LDS
+    // while the events are folded: the serial tail then works on the copy
+    // instead of a chain of dependent loads. k_rinv, which runs beside this
+    // kernel, adds to the current substep's rank_pending slot and touches
+    // nothing else here, so that word alone is not written back.
+    __shared__ DevCtl s_c;
+    __shared__ Sphere<R> s_sp;
+    {
+        const u32 *src = reinterpret_cast<const u32 *>(ctl);
+        u32 *dst = reinterpret_cast<u32 *>(&s_c);
+        for (u32 w = tid; w < u32(sizeof(DevCtl) / 4); w += kFinishThreads) dst[w] = src[w];
+        const u32 *ssrc = reinterpret_cast<const u32 *>(L.sphere);
+        u32 *sdst = reinterpret_cast<u32 *>(&s_sp);
+        for (u32 w = tid; w < u32(sizeof(Sphere<R>) / 4); w += kFinishThreads) sdst[w] = ssrc[w];
+    }
     const u64 step = ctl->step;
     // Wave 0 carries the folds: its lanes all hold the same maximum damage and
     // counts; lane 0 holds the removed-energy sum and its gain flag.
@@ -1671,70 +1686,84 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
         for (int k = 0; k < 13; ++k) ledger[k] = fixedOrderSum<R>(L.load_part, L.node_tiles, 13, k, s_red);
     double plastic = 0.0;
     if (PLASTIC) plastic = fixedOrderSum<R>(L.tile_plastic, L.bond_tiles, 1, 0, s_red);
-    if (tid != 0) return;
-    {
-        const u32 slot = u32((step + 1U) & 1U);  // (step - 1) & 1: the slot the previous substep's k_rinv used
-        ctl->rank_deficient += ctl->rank_pending[slot];
-        ctl->rank_pending[slot] = 0;
-    }
-    if (damage_gain) {
-        ctl->max_damage = m;
-        ctl->last_damage_gain_step = step;
-    }
-    const bool any = broken_now > 0;
-    if (any) {
-        ctl->removed_energy_j = removed;
-        if (energy_gain) ctl->last_energy_gain_step = step;
-        ctl->broken_bonds += broken_now;
-        if (first_round) ctl->first_failure_count = ff;
-    }
-    if (PLASTIC) ctl->plastic_work += plastic;
-    if (LOADS && loads) {
-        ctl->ledger_steps += 1;
-        ctl->ledger_elapsed += double(S.dt);
-        for (int k = 0; k < 3; ++k) {
-            ctl->ledger_req[k] += ledger[k];
-            ctl->ledger_imp[k] += ledger[3 + k];
-            ctl->ledger_areq[k] += ledger[6 + k];
-            ctl->ledger_aimp[k] += ledger[9 + k];
+    // The serial tail, on the LDS copy; then every thread writes the copy back.
+    __syncthreads();
+    if (tid == 0) {
+        {
+            const u32 slot = u32((step + 1U) & 1U);  // (step - 1) & 1: the slot the previous substep's k_rinv used
+            s_c.rank_deficient += s_c.rank_pending[slot];
+            s_c.rank_pending[slot] = 0;
         }
-        ctl->ledger_work += ledger[12];
-        ctl->load_remaining -= 1;
+        if (damage_gain) {
+            s_c.max_damage = m;
+            s_c.last_damage_gain_step = step;
+        }
+        const bool any = broken_now > 0;
+        if (any) {
+            s_c.removed_energy_j = removed;
+            if (energy_gain) s_c.last_energy_gain_step = step;
+            s_c.broken_bonds += broken_now;
+            if (first_round) s_c.first_failure_count = ff;
+        }
+        if (PLASTIC) s_c.plastic_work += plastic;
+        if (LOADS && loads) {
+            s_c.ledger_steps += 1;
+            s_c.ledger_elapsed += double(S.dt);
+            for (int k = 0; k < 3; ++k) {
+                s_c.ledger_req[k] += ledger[k];
+                s_c.ledger_imp[k] += ledger[3 + k];
+                s_c.ledger_areq[k] += ledger[6 + k];
+                s_c.ledger_aimp[k] += ledger[9 + k];
+            }
+            s_c.ledger_work += ledger[12];
+            s_c.load_remaining -= 1;
+        }
+        // Sphere drift and sphere against the supports (CpuLatticeBackend.cpp:481-482).
+        Sphere<R> sp = s_sp;
+        if (!S.sphere_enabled) sp.velocity = sp.velocity + S.dt * S.gravity;
+        sp.center = sp.center + S.dt * sp.velocity;
+        u64 ev = s_c.ball_support_events;
+        sphereSupport(S, sp, ev);
+        s_c.ball_support_events = ev;
+        s_sp = sp;
+        if (any) {
+            s_c.dirty_start[step & 1U] = 1;
+            s_c.nc_rebuild = 1;  // a failure round forces the contact pair list's rebuild (CpuLatticeBackend.cpp:485)
+        }
+        // Run loop bookkeeping (CpuLatticeBackend.cpp:279-298).
+        const u64 total = step + 1;
+        s_c.step = total;
+        if (any) {
+            if (s_c.first_failure_step == ~0ULL) s_c.first_failure_step = step;
+            s_c.last_failure_step = step;
+            s_c.failure_rounds += 1;
+        }
+        const double available = s_c.removable + s_c.ledger_work;
+        const unsigned reason = s_c.removable > 0 && available <= 0
+            ? 6U
+            : exitReason(total, s_c.broken_bonds, s_c.last_failure_step, s_c.quiet, s_c.min_steps, s_c.no_failure,
+                         s_c.energy_flat, s_c.last_energy_gain_step, s_c.calm, s_c.last_damage_gain_step, s_c.max_damage,
+                         s_c.calm_margin, s_c.removed_energy_j, s_c.removable > 0 ? available : 0.0);
+        s_c.exit_reason = reason;
+        if (reason != 0 || s_c.error) s_c.stop = 1;
+        // The run's step budget (RunControl::max_steps, reason 3).
+        const u64 budget = s_c.budget > 0 ? s_c.budget - 1 : 0;
+        s_c.budget = budget;
+        if (reason == 0 && budget == 0) {
+            s_c.exit_reason = kBudgetSpent;
+            s_c.stop = 1;
+        }
     }
-    // Sphere drift and sphere against the supports (CpuLatticeBackend.cpp:481-482).
-    Sphere<R> sp = *L.sphere;
-    if (!S.sphere_enabled) sp.velocity = sp.velocity + S.dt * S.gravity;
-    sp.center = sp.center + S.dt * sp.velocity;
-    u64 ev = ctl->ball_support_events;
-    sphereSupport(S, sp, ev);
-    ctl->ball_support_events = ev;
-    *L.sphere = sp;
-    if (any) {
-        ctl->dirty_start[step & 1U] = 1;
-        ctl->nc_rebuild = 1;  // a failure round forces the contact pair list's rebuild (CpuLatticeBackend.cpp:485)
-    }
-    // Run loop bookkeeping (CpuLatticeBackend.cpp:279-298).
-    const u64 total = step + 1;
-    ctl->step = total;
-    if (any) {
-        if (ctl->first_failure_step == ~0ULL) ctl->first_failure_step = step;
-        ctl->last_failure_step = step;
-        ctl->failure_rounds += 1;
-    }
-    const double available = ctl->removable + ctl->ledger_work;
-    const unsigned reason = ctl->removable > 0 && available <= 0
-        ? 6U
-        : exitReason(total, ctl->broken_bonds, ctl->last_failure_step, ctl->quiet, ctl->min_steps, ctl->no_failure,
-                     ctl->energy_flat, ctl->last_energy_gain_step, ctl->calm, ctl->last_damage_gain_step, ctl->max_damage,
-                     ctl->calm_margin, ctl->removed_energy_j, ctl->removable > 0 ? available : 0.0);
-    ctl->exit_reason = reason;
-    if (reason != 0 || ctl->error) ctl->stop = 1;
-    // The run's step budget (RunControl::max_steps, reason 3).
-    const u64 budget = ctl->budget > 0 ? ctl->budget - 1 : 0;
-    ctl->budget = budget;
-    if (reason == 0 && budget == 0) {
-        ctl->exit_reason = kBudgetSpent;
-        ctl->stop = 1;
+    __syncthreads();
+    {
+        const u32 *src = reinterpret_cast<const u32 *>(&s_c);
+        u32 *dst = reinterpret_cast<u32 *>(ctl);
+        const u32 busy = u32(offsetof(DevCtl, rank_pending) / 4) + u32(step & 1U);
+        for (u32 w = tid; w < u32(sizeof(DevCtl) / 4); w += kFinishThreads)
+            if (w != busy) dst[w] = src[w];
+        const u32 *ssrc = reinterpret_cast<const u32 *>(&s_sp);
+        u32 *sdst = reinterpret_cast<u32 *>(L.sphere);
+        for (u32 w = tid; w < u32(sizeof(Sphere<R>) / 4); w += kFinishThreads) sdst[w] = ssrc[w];
     }
 }
 
