@@ -1734,14 +1734,14 @@ private:
             hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), 0, early_, nullptr,
                                   ev_early_, 0, bonds, S);
             profEnd(early_);
-            BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
+            BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kPass2Threads), 0, stream_, lats, S));
             BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), sg, dim3(kWG), 0, stream_, small));
             HIP_OK(hipStreamWaitEvent(stream_, ev_early_, 0));
             BLX_LAUNCH(KBondEndLate, 12,
                        hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), 0, stream_,
                                              nullptr, ev_bond_end_, 0, small, S));
         } else {
-            if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kFinishThreads), 0, stream_, lats, S));
+            if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kPass2Threads), 0, stream_, lats, S));
             if (contact) BLX_LAUNCH(KContact, 14, hipLaunchKernelGGL((k_nc_stale<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
         HIP_OK(hipGetLastError());

@@ -37,6 +37,11 @@ constexpr int kWG = 256;
 constexpr u32 kEndSpan = 1;  // bond tiles per end-sample workgroup (2 and 4 measured no faster)
 constexpr u32 kCap = 1024;        // striker candidates per block (LatticePhysics.hpp:105)
 constexpr int kFinishThreads = 1024;
+// Striker pass 2 stages its candidates with four waves and runs the chain in
+// one: launched as the end sample's wide launches become ready on the other
+// stream, it fits beside their workgroups on a partly busy CU, where sixteen
+// waves would wait for a CU to drain.
+constexpr int kPass2Threads = 256;
 
 // Device-resident run state. Everything that changes from one substep to the
 // next lives here, so kernel arguments are constant and a batch of substeps
@@ -813,7 +818,8 @@ constexpr u32 kMaxNodeTiles = 16384;  // candidate tile prefix held in LDS: up t
 // over the window. A centre move (pass 2) re-speculates the window's
 // remaining candidates.
 template <class R, int PASS>
-__global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) {
+__global__ __launch_bounds__(PASS == 1 ? kFinishThreads : kPass2Threads) void k_pass(Batch<R> B, Step<R> S) {
+    constexpr u32 kThreads = PASS == 1 ? kFinishThreads : kPass2Threads;
     const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
@@ -894,7 +900,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
     const R energy_before = R(0.5) * sp.mass * length2(sp.velocity) + R(0.5) * sp.inertia * length2(sp.omega);
     for (u32 b = 0; b < L.blocks; ++b) {
         const u32 n = L.cand_cnt[b];
-        for (u32 k = tid; k < n; k += kFinishThreads) {
+        for (u32 k = tid; k < n; k += kThreads) {
             const u32 i = L.cand_all[size_t(b) * kCap + k];
             s_ids[k] = i;
             s_u[k] = L.u4[i];
@@ -984,7 +990,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_pass(Batch<R> B, Step<R> S) 
             }
         }
         __syncthreads();
-        for (u32 k = tid; k < n; k += kFinishThreads) {
+        for (u32 k = tid; k < n; k += kThreads) {
             const u32 i = s_ids[k];
             V3<R> v = xyz(s_v[k]);
             R4<R> u = s_u[k];
