@@ -806,7 +806,8 @@ public:
         // Only other streams of this device wait on these events, so their
         // release is device-scope; the default system-scope fence makes every
         // signal also write the caches back for the host (about 1% of the step).
-        for (hipEvent_t *e : {&ev_bond_end_, &ev_rinv_, &ev_pass1_, &ev_velocity_, &ev_early_, &ev_predict_, &ev_start_})
+        for (hipEvent_t *e : {&ev_bond_end_, &ev_rinv_, &ev_pass1_, &ev_velocity_, &ev_early_, &ev_predict_, &ev_start_,
+                              &ev_sweep_})
             HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
         HIP_OK(hipEventCreate(&ev0_));
         HIP_OK(hipEventCreate(&ev1_));
@@ -825,7 +826,8 @@ public:
         (void)hipSetDevice(device_);
         for (hipStream_t s : {stream_, side_, early_})
             if (s) (void)hipStreamSynchronize(s);
-        for (hipEvent_t e : {ev_bond_end_, ev_rinv_, ev_pass1_, ev_velocity_, ev_early_, ev_predict_, ev_start_, ev0_, ev1_})
+        for (hipEvent_t e : {ev_bond_end_, ev_rinv_, ev_pass1_, ev_velocity_, ev_early_, ev_predict_, ev_start_, ev_sweep_,
+                             ev0_, ev1_})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
         for (hipStream_t s : {early_, side_, stream_})
@@ -1405,7 +1407,7 @@ private:
     // overlap the folds; early_ the end sample of the nodes and bonds striker
     // pass 2 cannot reach, which overlaps pass 2.
     hipStream_t stream_ = nullptr, side_ = nullptr, early_ = nullptr;
-    hipEvent_t ev_predict_ = nullptr, ev_start_ = nullptr;
+    hipEvent_t ev_predict_ = nullptr, ev_start_ = nullptr, ev_sweep_ = nullptr;
     hipEvent_t ev_bond_end_ = nullptr, ev_rinv_ = nullptr, ev_pass1_ = nullptr, ev_velocity_ = nullptr,
                ev_early_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -1654,6 +1656,9 @@ private:
         // With a striker, the end sample of what pass 2 cannot reach overlaps
         // pass 2, unless node contact moves nodes between pass 2 and the sample.
         const bool split = sphere && !plastic && !contact;
+        // Without damping between the velocity update and the end sample,
+        // striker pass 2 need not wait for the velocity update of the rest.
+        const bool own = split && !(S.damping > R(0));
         const Batch<R> nodes = batch(node_tiles_map_), bonds = batch(bond_tiles_map_), list = batch(list_map_),
                        small = batch(small_map_), lats = batch(lattice_map_);
         const dim3 ng = tiles(node_tiles_map_), bg = tiles(bond_tiles_map_), lg = entries(list_map_),
@@ -1700,7 +1705,10 @@ private:
                 profBegin(KSolve, phase == 0 ? 5 : 6);
                 for (size_t k = lo; k < hi; ++k) {
                     const MapRef &m = stage_maps_[k];
-                    if (plastic)
+                    if (own && it + 1 == iterations && k + 1 == stage_maps_.size())
+                        hipExtLaunchKernelGGL((k_solve<R, DIRECT, false>), tiles(m), dim3(kWG), 0, stream_, nullptr,
+                                              ev_sweep_, 0, batch(m), S, flags);
+                    else if (plastic)
                         hipLaunchKernelGGL((k_solve<R, DIRECT, true>), tiles(m), dim3(kWG), 0, stream_, batch(m), S, flags);
                     else
                         hipLaunchKernelGGL((k_solve<R, DIRECT, false>), tiles(m), dim3(kWG), 0, stream_, batch(m), S, flags);
@@ -1710,6 +1718,35 @@ private:
             }
             if ((S.plane_count > 0 || S.blade.enabled) && it + 1 < iterations)
                 BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), ng, dim3(kWG), 0, stream_, nodes, S));
+        }
+        if (own) {
+            // Striker pass 2 follows the sweep directly and takes its listed
+            // candidates' velocity update itself. The velocity update of the
+            // rest and the end sample of what pass 2 cannot reach run on the
+            // early stream; the strain of the affected nodes waits for both.
+            HIP_OK(hipStreamWaitEvent(early_, ev_start_, 0));
+            HIP_OK(hipStreamWaitEvent(early_, ev_sweep_, 0));
+            profBegin(KVelocity, 8, early_);
+            if (S.plane_count > 0 || S.blade.enabled)
+                hipExtLaunchKernelGGL((k_velocity<R, true, true>), ng, dim3(kWG), 0, early_, nullptr, ev_velocity_, 0, nodes, S);
+            else hipExtLaunchKernelGGL((k_velocity<R, false, true>), ng, dim3(kWG), 0, early_, nullptr, ev_velocity_, 0, nodes, S);
+            profEnd(early_);
+            profBegin(KStrain, 11, early_);
+            hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), ng, dim3(kWG), 0, early_, nodes);
+            profEnd(early_);
+            profBegin(KBondEnd, 12, early_);
+            hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), 0, early_, nullptr,
+                                  ev_early_, 0, bonds, S);
+            profEnd(early_);
+            BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2, true>), fg, dim3(kPass2Threads), 0, stream_, lats, S));
+            HIP_OK(hipStreamWaitEvent(stream_, ev_velocity_, 0));
+            BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), sg, dim3(kWG), 0, stream_, small));
+            HIP_OK(hipStreamWaitEvent(stream_, ev_early_, 0));
+            BLX_LAUNCH(KBondEndLate, 12,
+                       hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), 0, stream_,
+                                             nullptr, ev_bond_end_, 0, small, S));
+            HIP_OK(hipGetLastError());
+            return;
         }
         HIP_OK(hipStreamWaitEvent(stream_, ev_start_, 0));
         profBegin(KVelocity, 8);

@@ -817,7 +817,13 @@ constexpr u32 kMaxNodeTiles = 16384;  // candidate tile prefix held in LDS: up t
 // whose contact test fails at the current sphere state are skipped by a ballot
 // over the window. A centre move (pass 2) re-speculates the window's
 // remaining candidates.
-template <class R, int PASS>
+template <class R>
+__device__ __forceinline__ void projectNode(const DevLattice<R> &L, const Step<R> &S, u32 i, R4<R> &u, bool &moved);
+
+// OWN_VELOCITY (pass 2): the pass takes its listed candidates' last support
+// projection and velocity update itself, the expressions of k_velocity, which
+// then skips them; so the pass does not wait for k_velocity.
+template <class R, int PASS, bool OWN_VELOCITY = false>
 __global__ __launch_bounds__(PASS == 1 ? kFinishThreads : kPass2Threads) void k_pass(Batch<R> B, Step<R> S) {
     constexpr u32 kThreads = PASS == 1 ? kFinishThreads : kPass2Threads;
     const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
@@ -903,8 +909,16 @@ __global__ __launch_bounds__(PASS == 1 ? kFinishThreads : kPass2Threads) void k_
         for (u32 k = tid; k < n; k += kThreads) {
             const u32 i = L.cand_all[size_t(b) * kCap + k];
             s_ids[k] = i;
-            s_u[k] = L.u4[i];
-            s_v[k] = L.v4[i];
+            if (PASS == 2 && OWN_VELOCITY) {
+                R4<R> u = L.u4[i];
+                bool moved = false;
+                projectNode(L, S, i, u, moved);
+                s_u[k] = u;
+                s_v[k] = with((xyz(u) - xyz(L.up4[i])) / S.dt, R(0));
+            } else {
+                s_u[k] = L.u4[i];
+                s_v[k] = L.v4[i];
+            }
             s_x0[k] = L.x04[i];
         }
         __syncthreads();
@@ -998,6 +1012,7 @@ __global__ __launch_bounds__(PASS == 1 ? kFinishThreads : kPass2Threads) void k_
                 // nodeRecordApproach and the prediction (LatticePhysics.hpp:1408-1411).
                 for (u32 p = 0; p < S.plane_count; ++p) L.approach[size_t(p) * L.N + i] = dot(v, S.planes[p].normal);
                 u = with(xyz(u) + S.dt * v, u.w);
+                L.cand[i] = 2;  // listed: striker pass 2 contacts it
             } else {
                 // Candidates take their support velocity response here (LatticePhysics.hpp:1419-1425).
                 for (u32 p = 0; p < S.plane_count; ++p)
@@ -1129,8 +1144,9 @@ template <class R> __global__ __launch_bounds__(kWG) void k_project(Batch<R> B, 
 }
 
 // Last iteration's support projection fused with nodeVelocityUpdate
-// (LatticePhysics.hpp:1726-1730): both only touch node i.
-template <class R, bool PROJECT>
+// (LatticePhysics.hpp:1726-1730): both only touch node i. SKIP_LISTED leaves
+// the candidates on striker pass 2's lists to that pass.
+template <class R, bool PROJECT, bool SKIP_LISTED = false>
 __global__ __launch_bounds__(kWG) void k_velocity(Batch<R> B, Step<R> S) {
     const WgMap *m = tileEntry(B);
     if (!m) return;
@@ -1139,6 +1155,7 @@ __global__ __launch_bounds__(kWG) void k_velocity(Batch<R> B, Step<R> S) {
     {
         const u32 i = m->a * kWG + threadIdx.x;
         if (i >= L.N) return;
+        if (SKIP_LISTED && L.cand[i] == 2) return;
         R4<R> u = L.u4[i];
         if (PROJECT) {
             bool moved = false;
