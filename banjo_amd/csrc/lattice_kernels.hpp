@@ -265,13 +265,15 @@ template <class R> __device__ void rinvNode(const DevLattice<R> &L, u32 i, u32 *
 template <class R, bool DIRECT, bool PREV> __device__ void strainNode(const DevLattice<R> &L, u32 i) {
     const u32 N = L.N, D = L.D;
 #ifndef BLX_STRAIN_CHUNK
-#define BLX_STRAIN_CHUNK 8
+#define BLX_STRAIN_CHUNK 4
 #endif
 #ifndef BLX_STRAIN_PREFETCH
 #define BLX_STRAIN_PREFETCH 1
 #endif
     // Slots in chunks: every load of a chunk is issued before the sums, which
     // stay in slot order, and the next chunk's keys are in flight meanwhile.
+    // Four slots a chunk keep the kernel at 54 VGPRs, eight waves per SIMD;
+    // eight slots took 91 and five waves, and ran slower.
     constexpr u32 kChunk = BLX_STRAIN_CHUNK;
     // The node's flag, own displacement and first keys go out together.
     const u8 valid = ldg(&L.valid[i]);
