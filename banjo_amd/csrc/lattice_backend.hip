@@ -1686,8 +1686,11 @@ private:
         profEnd(side_);
         if (sphere)
             BLX_LAUNCH(KPass1, 4, hipExtLaunchKernelGGL((k_pass<R, 1>), fg, dim3(kFinishThreads), 0, stream_, nullptr,
-                                                        split ? ev_pass1_ : nullptr, 0, lats, S));
-        if (split) {
+                                                        split && !own ? ev_pass1_ : nullptr, 0, lats, S));
+        // Without damping the velocity update on the early stream marks the
+        // nodes pass 2 can reach; otherwise k_mark_affected does it there
+        // during the sweep, after pass 1.
+        if (split && !own) {
             HIP_OK(hipStreamWaitEvent(early_, ev_pass1_, 0));
             profBegin(KMark, 4, early_);
             hipLaunchKernelGGL((k_mark_affected<R>), entries(mark_map_), dim3(kWG), 0, early_, batch(mark_map_));
@@ -1727,9 +1730,10 @@ private:
             HIP_OK(hipStreamWaitEvent(early_, ev_start_, 0));
             HIP_OK(hipStreamWaitEvent(early_, ev_sweep_, 0));
             profBegin(KVelocity, 8, early_);
+            const Batch<R> mark = batch(mark_map_);
             if (S.plane_count > 0 || S.blade.enabled)
-                hipExtLaunchKernelGGL((k_velocity<R, true, true>), ng, dim3(kWG), 0, early_, nullptr, ev_velocity_, 0, nodes, S);
-            else hipExtLaunchKernelGGL((k_velocity<R, false, true>), ng, dim3(kWG), 0, early_, nullptr, ev_velocity_, 0, nodes, S);
+                hipExtLaunchKernelGGL((k_velocity<R, true, true>), ng, dim3(kWG), 0, early_, nullptr, ev_velocity_, 0, nodes, S, mark);
+            else hipExtLaunchKernelGGL((k_velocity<R, false, true>), ng, dim3(kWG), 0, early_, nullptr, ev_velocity_, 0, nodes, S, mark);
             profEnd(early_);
             profBegin(KStrain, 11, early_);
             hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), ng, dim3(kWG), 0, early_, nodes);
@@ -1753,8 +1757,10 @@ private:
         // With no damping after it, the velocity update carries the event the early stream waits on.
         hipEvent_t after_velocity = split && !(S.damping > R(0)) ? ev_velocity_ : nullptr;
         if (S.plane_count > 0 || S.blade.enabled)
-            hipExtLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nullptr, after_velocity, 0, nodes, S);
-        else hipExtLaunchKernelGGL((k_velocity<R, false>), ng, dim3(kWG), 0, stream_, nullptr, after_velocity, 0, nodes, S);
+            hipExtLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nullptr, after_velocity, 0, nodes, S,
+                                  Batch<R>{});
+        else hipExtLaunchKernelGGL((k_velocity<R, false>), ng, dim3(kWG), 0, stream_, nullptr, after_velocity, 0, nodes, S,
+                                   Batch<R>{});
         profEnd();
         if (S.damping > R(0)) {
             for (const MapRef &m : stage_maps_)

@@ -370,29 +370,46 @@ __global__ __launch_bounds__(kWG) void k_strain(Batch<R> B) {
 // The nodes striker pass 2 can reach this substep: its candidates (the
 // staged lists of pass 1) and their neighbours, each listed once. Runs on a
 // side stream while the colour sweep runs.
-template <class R> __global__ __launch_bounds__(kWG) void k_mark_affected(Batch<R> B) {
-    const WgMap m = B.map[blockIdx.x];
+// Map entry `entry` of B: (lattice, slice a of b slices of the marking work).
+// Every lane of the wave calls it; each wave appends its claims with one
+// atomic, since thousands of appends to one counter would queue at it.
+template <class R> __device__ __forceinline__ void markAffected(const Batch<R> &B, u32 entry) {
+    const WgMap m = B.map[entry];
     const DevLattice<R> L = B.L[m.lat];
     DevCtl *ctl = L.ctl;
     if (ctl->stop) return;
     const u32 stamp = u32(ctl->step) + 1U;
-    const u32 per = L.D + 1;
+    const u32 per = L.D + 1, lane = threadIdx.x & 63U;
     const u64 total = u64(L.blocks) * kCap * per;
-    for (u64 q = u64(m.a) * kWG + threadIdx.x; q < total; q += u64(m.b) * kWG) {
-        const u32 b = u32(q / (u64(kCap) * per));
-        const u32 rem = u32(q % (u64(kCap) * per));
-        const u32 e = rem / per, k = rem % per;
-        if (e >= L.cand_cnt[b]) continue;
-        const u32 c = L.cand_all[size_t(b) * kCap + e];
-        u32 n = c;
-        if (k > 0) {
-            const u32 key = L.nbr_key[size_t(k - 1) * L.N + c];
-            if (key == kNone) continue;
-            n = u32(int(c) + L.sdelta[key & ~kDeadSlot]);
+    for (u64 base = u64(m.a) * kWG; base < total; base += u64(m.b) * kWG) {
+        const u64 q = base + threadIdx.x;
+        u32 n = kNone;
+        if (q < total) {
+            const u32 b = u32(q / (u64(kCap) * per));
+            const u32 rem = u32(q % (u64(kCap) * per));
+            const u32 e = rem / per, k = rem % per;
+            if (e < L.cand_cnt[b]) {
+                const u32 c = L.cand_all[size_t(b) * kCap + e];
+                if (k == 0) {
+                    n = c;
+                } else {
+                    const u32 key = L.nbr_key[size_t(k - 1) * L.N + c];
+                    if (key != kNone) n = u32(int(c) + L.sdelta[key & ~kDeadSlot]);
+                }
+            }
         }
-        if (atomicExch(&L.affected[n], stamp) != stamp) L.affected_list[atomicAdd(&ctl->affected_count, 1U)] = n;
+        const bool claim = n != kNone && atomicExch(&L.affected[n], stamp) != stamp;
+        const u64 mask = __ballot(claim);
+        if (mask) {
+            const u32 leader = u32(__ffsll((long long)mask) - 1);
+            u32 at = 0;
+            if (lane == leader) at = atomicAdd(&ctl->affected_count, u32(__popcll(mask)));
+            at = u32(__builtin_amdgcn_readlane(int(at), int(leader)));
+            if (claim) L.affected_list[at + u32(__popcll(mask & ((1ULL << lane) - 1ULL)))] = n;
+        }
     }
 }
+template <class R> __global__ __launch_bounds__(kWG) void k_mark_affected(Batch<R> B) { markAffected(B, blockIdx.x); }
 
 // Inverse rest covariances of the nodes the last breaks made dirty. It needs
 // only the live set k_bond_end left, so it runs on a side stream while
@@ -1147,9 +1164,13 @@ template <class R> __global__ __launch_bounds__(kWG) void k_project(Batch<R> B, 
 
 // Last iteration's support projection fused with nodeVelocityUpdate
 // (LatticePhysics.hpp:1726-1730): both only touch node i. SKIP_LISTED leaves
-// the candidates on striker pass 2's lists to that pass.
+// the candidates on striker pass 2's lists to that pass, and the first
+// workgroups first mark the nodes pass 2 can reach (k_mark_affected's work over
+// the entries of M), which the end sample after this kernel reads.
 template <class R, bool PROJECT, bool SKIP_LISTED = false>
-__global__ __launch_bounds__(kWG) void k_velocity(Batch<R> B, Step<R> S) {
+__global__ __launch_bounds__(kWG) void k_velocity(Batch<R> B, Step<R> S, Batch<R> M) {
+    if (SKIP_LISTED)
+        for (u32 e = blockIdx.x; e < M.count; e += gridDim.x) markAffected(M, e);
     const WgMap *m = tileEntry(B);
     if (!m) return;
     const DevLattice<R> L = B.L[m->lat];
