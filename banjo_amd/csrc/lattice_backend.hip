@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -811,6 +812,7 @@ public:
             HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
         HIP_OK(hipEventCreate(&ev0_));
         HIP_OK(hipEventCreate(&ev1_));
+        if (const char *e = std::getenv("BLX_GRAPHS")) use_graphs_ = e[0] != '\0' && e[0] != '0';
         hipDeviceProp_t prop{};
         HIP_OK(hipGetDeviceProperties(&prop, device_));
         cus_ = uint32_t(prop.multiProcessorCount);
@@ -824,6 +826,7 @@ public:
     }
     ~Engine() {
         (void)hipSetDevice(device_);
+        dropGraph();
         for (hipStream_t s : {stream_, side_, early_})
             if (s) (void)hipStreamSynchronize(s);
         for (hipEvent_t e : {ev_bond_end_, ev_rinv_, ev_pass1_, ev_velocity_, ev_early_, ev_predict_, ev_start_, ev_sweep_,
@@ -1031,6 +1034,7 @@ public:
     // still running. Abandon is polled between device batches.
     uint32_t advance(uint64_t substeps, uint64_t batch_len, const volatile uint8_t *abandon, double *kernel) {
         HIP_OK(hipSetDevice(device_));
+        dropGraph();  // settings, maps or slots may have changed since the last call
         uint64_t done = 0;
         uint32_t running = countRunning();
         double ksec = 0.0;
@@ -1038,7 +1042,12 @@ public:
             if (maps_dirty_) buildMaps();
             const uint64_t n = std::min<uint64_t>(profilingBatch(batch_len), substeps - done);
             HIP_OK(hipEventRecord(ev0_, stream_));
-            for (uint64_t q = 0; q < n; ++q) {
+            uint64_t q = 0;
+            if (use_graphs_ && S_.nc.mode == 0U && !prof_ && n >= kGraphSubsteps) {
+                if (!graph_exec_) captureSubsteps();
+                for (; q + kGraphSubsteps <= n; q += kGraphSubsteps) HIP_OK(hipGraphLaunch(graph_exec_, stream_));
+            }
+            for (; q < n; ++q) {
                 enqueueSubstepA();
                 if (S_.nc.mode != 0U) {
                     // The rebuild decision of each slot, read back once a substep.
@@ -1632,6 +1641,67 @@ private:
     }
     uint64_t profilingBatch(uint64_t batch) const { return prof_ ? std::min<uint64_t>(batch, 4) : batch; }
 
+    // Substep graphs, on request (BLX_GRAPHS=1 in the environment). Every
+    // launch argument of a substep is constant (the run state lives on the
+    // device), so kGraphSubsteps substeps are captured once per advance call
+    // and replayed; the side streams fork from stream_ and join back into it
+    // through the substep's own events, which become graph edges. Nothing in a
+    // substep records a memset. Replays measured slower than enqueuing every
+    // substep on every workload here (the host stays ahead of the device), so
+    // enqueuing is the default.
+    static constexpr uint64_t kGraphSubsteps = 8;
+    bool use_graphs_ = false;
+    bool capturing_ = false;
+    hipGraphExec_t graph_exec_ = nullptr;
+
+    void dropGraph() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    void captureSubsteps() {
+        dropGraph();
+        HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+        capturing_ = true;
+        hipError_t err = hipSuccess;
+        try {
+            for (uint64_t q = 0; q < kGraphSubsteps; ++q) {
+                enqueueSubstepA();
+                enqueueSubstepB();
+            }
+            err = hipStreamWaitEvent(stream_, ev_rinv_, 0);
+        } catch (...) {
+            capturing_ = false;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        capturing_ = false;
+        hipGraph_t g = nullptr;
+        const hipError_t end = hipStreamEndCapture(stream_, &g);
+        HIP_OK(err);
+        HIP_OK(end);
+        const hipError_t inst = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (inst != hipSuccess) graph_exec_ = nullptr;
+        HIP_OK(inst);
+    }
+
+// A launch whose end another stream waits on (ev; nullptr for none). The
+// event rides on the launch as its stop event: a separate record puts a marker
+// between two kernels of the stream, which costs about 5 us. While a substep
+// graph is being captured the record is a graph edge instead.
+#define BLX_SIGNAL(kernel, grid, block, stream, ev, ...)                                         \
+    do {                                                                                         \
+        hipEvent_t ev_ = (ev);                                                                   \
+        if (capturing_) {                                                                        \
+            hipLaunchKernelGGL(kernel, grid, block, 0, stream, __VA_ARGS__);                     \
+            if (ev_) HIP_OK(hipEventRecord(ev_, stream));                                        \
+        } else {                                                                                 \
+            hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, nullptr, ev_, 0, __VA_ARGS__); \
+        }                                                                                        \
+    } while (0)
+
 #define BLX_LAUNCH(kernel, phase, ...) \
     do {                                \
         profBegin(kernel, phase);       \
@@ -1667,10 +1737,10 @@ private:
         // (hipExtLaunchKernelGGL's stop event): a separate hipEventRecord puts a
         // marker between two kernels of the stream, which cost about 6 us each.
         profBegin(KPredict, 3);
-        if (sphere && any_loads_) hipExtLaunchKernelGGL((k_predict<R, true, true, DIRECT>), ng, dim3(kWG), 0, stream_, nullptr, ev_predict_, 0, nodes, S);
-        else if (sphere) hipExtLaunchKernelGGL((k_predict<R, true, false, DIRECT>), ng, dim3(kWG), 0, stream_, nullptr, ev_predict_, 0, nodes, S);
-        else if (any_loads_) hipExtLaunchKernelGGL((k_predict<R, false, true, DIRECT>), ng, dim3(kWG), 0, stream_, nullptr, ev_predict_, 0, nodes, S);
-        else hipExtLaunchKernelGGL((k_predict<R, false, false, DIRECT>), ng, dim3(kWG), 0, stream_, nullptr, ev_predict_, 0, nodes, S);
+        if (sphere && any_loads_) BLX_SIGNAL((k_predict<R, true, true, DIRECT>), ng, dim3(kWG), stream_, ev_predict_, nodes, S);
+        else if (sphere) BLX_SIGNAL((k_predict<R, true, false, DIRECT>), ng, dim3(kWG), stream_, ev_predict_, nodes, S);
+        else if (any_loads_) BLX_SIGNAL((k_predict<R, false, true, DIRECT>), ng, dim3(kWG), stream_, ev_predict_, nodes, S);
+        else BLX_SIGNAL((k_predict<R, false, false, DIRECT>), ng, dim3(kWG), stream_, ev_predict_, nodes, S);
         profEnd();
         // The start sample of the nodes and bonds the last failure round made
         // dirty reads the positions at substep start from up4, so it runs on
@@ -1682,11 +1752,10 @@ private:
         hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainDirty>), lg, dim3(kWG), 0, side_, list);
         profEnd(side_);
         profBegin(KBondStart, 2, side_);
-        hipExtLaunchKernelGGL((k_bond_start<R, DIRECT>), lg, dim3(kWG), 0, side_, nullptr, ev_start_, 0, list);
+        BLX_SIGNAL((k_bond_start<R, DIRECT>), lg, dim3(kWG), side_, ev_start_, list);
         profEnd(side_);
         if (sphere)
-            BLX_LAUNCH(KPass1, 4, hipExtLaunchKernelGGL((k_pass<R, 1>), fg, dim3(kFinishThreads), 0, stream_, nullptr,
-                                                        split && !own ? ev_pass1_ : nullptr, 0, lats, S));
+            BLX_LAUNCH(KPass1, 4, BLX_SIGNAL((k_pass<R, 1>), fg, dim3(kFinishThreads), stream_, split && !own ? ev_pass1_ : nullptr, lats, S));
         // Without damping the velocity update on the early stream marks the
         // nodes pass 2 can reach; otherwise k_mark_affected does it there
         // during the sweep, after pass 1.
@@ -1709,8 +1778,7 @@ private:
                 for (size_t k = lo; k < hi; ++k) {
                     const MapRef &m = stage_maps_[k];
                     if (own && it + 1 == iterations && k + 1 == stage_maps_.size())
-                        hipExtLaunchKernelGGL((k_solve<R, DIRECT, false>), tiles(m), dim3(kWG), 0, stream_, nullptr,
-                                              ev_sweep_, 0, batch(m), S, flags);
+                        BLX_SIGNAL((k_solve<R, DIRECT, false>), tiles(m), dim3(kWG), stream_, ev_sweep_, batch(m), S, flags);
                     else if (plastic)
                         hipLaunchKernelGGL((k_solve<R, DIRECT, true>), tiles(m), dim3(kWG), 0, stream_, batch(m), S, flags);
                     else
@@ -1732,23 +1800,21 @@ private:
             profBegin(KVelocity, 8, early_);
             const Batch<R> mark = batch(mark_map_);
             if (S.plane_count > 0 || S.blade.enabled)
-                hipExtLaunchKernelGGL((k_velocity<R, true, true>), ng, dim3(kWG), 0, early_, nullptr, ev_velocity_, 0, nodes, S, mark);
-            else hipExtLaunchKernelGGL((k_velocity<R, false, true>), ng, dim3(kWG), 0, early_, nullptr, ev_velocity_, 0, nodes, S, mark);
+                BLX_SIGNAL((k_velocity<R, true, true>), ng, dim3(kWG), early_, ev_velocity_, nodes, S, mark);
+            else BLX_SIGNAL((k_velocity<R, false, true>), ng, dim3(kWG), early_, ev_velocity_, nodes, S, mark);
             profEnd(early_);
             profBegin(KStrain, 11, early_);
             hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), ng, dim3(kWG), 0, early_, nodes);
             profEnd(early_);
             profBegin(KBondEnd, 12, early_);
-            hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), 0, early_, nullptr,
-                                  ev_early_, 0, bonds, S);
+            BLX_SIGNAL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), early_, ev_early_, bonds, S);
             profEnd(early_);
             BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2, true>), fg, dim3(kPass2Threads), 0, stream_, lats, S));
             HIP_OK(hipStreamWaitEvent(stream_, ev_velocity_, 0));
             BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), sg, dim3(kWG), 0, stream_, small));
             HIP_OK(hipStreamWaitEvent(stream_, ev_early_, 0));
             BLX_LAUNCH(KBondEndLate, 12,
-                       hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), 0, stream_,
-                                             nullptr, ev_bond_end_, 0, small, S));
+                       BLX_SIGNAL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), stream_, ev_bond_end_, small, S));
             HIP_OK(hipGetLastError());
             return;
         }
@@ -1757,10 +1823,8 @@ private:
         // With no damping after it, the velocity update carries the event the early stream waits on.
         hipEvent_t after_velocity = split && !(S.damping > R(0)) ? ev_velocity_ : nullptr;
         if (S.plane_count > 0 || S.blade.enabled)
-            hipExtLaunchKernelGGL((k_velocity<R, true>), ng, dim3(kWG), 0, stream_, nullptr, after_velocity, 0, nodes, S,
-                                  Batch<R>{});
-        else hipExtLaunchKernelGGL((k_velocity<R, false>), ng, dim3(kWG), 0, stream_, nullptr, after_velocity, 0, nodes, S,
-                                   Batch<R>{});
+            BLX_SIGNAL((k_velocity<R, true>), ng, dim3(kWG), stream_, after_velocity, nodes, S, Batch<R>{});
+        else BLX_SIGNAL((k_velocity<R, false>), ng, dim3(kWG), stream_, after_velocity, nodes, S, Batch<R>{});
         profEnd();
         if (S.damping > R(0)) {
             for (const MapRef &m : stage_maps_)
@@ -1774,15 +1838,13 @@ private:
             hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), ng, dim3(kWG), 0, early_, nodes);
             profEnd(early_);
             profBegin(KBondEnd, 12, early_);
-            hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), 0, early_, nullptr,
-                                  ev_early_, 0, bonds, S);
+            BLX_SIGNAL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), early_, ev_early_, bonds, S);
             profEnd(early_);
             BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kPass2Threads), 0, stream_, lats, S));
             BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), sg, dim3(kWG), 0, stream_, small));
             HIP_OK(hipStreamWaitEvent(stream_, ev_early_, 0));
             BLX_LAUNCH(KBondEndLate, 12,
-                       hipExtLaunchKernelGGL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), 0, stream_,
-                                             nullptr, ev_bond_end_, 0, small, S));
+                       BLX_SIGNAL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), stream_, ev_bond_end_, small, S));
         } else {
             if (sphere) BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2>), fg, dim3(kPass2Threads), 0, stream_, lats, S));
             if (contact) BLX_LAUNCH(KContact, 14, hipLaunchKernelGGL((k_nc_stale<R>), ng, dim3(kWG), 0, stream_, nodes, S));
@@ -1815,7 +1877,7 @@ private:
         if (!split) HIP_OK(hipEventRecord(ev_bond_end_, stream_));
         HIP_OK(hipStreamWaitEvent(side_, ev_bond_end_, 0));
         profBegin(KRinv, 1, side_);
-        hipExtLaunchKernelGGL((k_rinv<R>), lg, dim3(kWG), 0, side_, nullptr, ev_rinv_, 0, list);
+        BLX_SIGNAL((k_rinv<R>), lg, dim3(kWG), side_, ev_rinv_, list);
         profEnd(side_);
         profBegin(KFinish, 13);
         if (any_loads_ && plastic) hipLaunchKernelGGL((k_finish<R, true, true>), fg, dim3(kFinishThreads), 0, stream_, lats, S);
