@@ -468,6 +468,8 @@ public:
         if (st.node_count != sch.N || st.bond_count != sch.B) throw Invalid("state does not match the schedule");
         if (size_t(st.max_degree) * st.node_count > 0xffffffffULL) throw Invalid("neighbour slots exceed 32-bit indices");
         if (st.bond_count >= 0x80000000U) throw Invalid("at most 2^31 - 1 bonds");
+        if (uint64_t(st.max_degree) * st.node_count >= 0x100000000ULL)
+            throw Invalid("the node slots (max_degree x nodes) must be fewer than 2^32");
         if (S.sphere_enabled && (st.node_count + kWG - 1) / kWG > kMaxNodeTiles)
             throw Invalid("the striker pass holds at most 4,194,304 nodes");
         N = st.node_count;

@@ -380,16 +380,16 @@ template <class R> __device__ __forceinline__ void markAffected(const Batch<R> &
     if (ctl->stop) return;
     const u32 stamp = u32(ctl->step) + 1U;
     const u32 per = L.D + 1, lane = threadIdx.x & 63U;
-    const u64 total = u64(L.blocks) * kCap * per;
-    for (u64 base = u64(m.a) * kWG; base < total; base += u64(m.b) * kWG) {
-        const u64 q = base + threadIdx.x;
-        u32 n = kNone;
-        if (q < total) {
-            const u32 b = u32(q / (u64(kCap) * per));
-            const u32 rem = u32(q % (u64(kCap) * per));
-            const u32 e = rem / per, k = rem % per;
-            if (e < L.cand_cnt[b]) {
-                const u32 c = L.cand_all[size_t(b) * kCap + e];
+    // Block by block: a block's work (its listed candidates, each with its
+    // slots) is below kCap * per entries, a 32-bit index.
+    for (u32 b = 0; b < L.blocks; ++b) {
+        const u32 total = L.cand_cnt[b] * per;
+        const u32 *cands = L.cand_all + size_t(b) * kCap;
+        for (u32 base = m.a * kWG; base < total; base += m.b * kWG) {
+            const u32 q = base + threadIdx.x;
+            u32 n = kNone;
+            if (q < total) {
+                const u32 c = cands[q / per], k = q % per;
                 if (k == 0) {
                     n = c;
                 } else {
@@ -397,15 +397,15 @@ template <class R> __device__ __forceinline__ void markAffected(const Batch<R> &
                     if (key != kNone) n = u32(int(c) + L.sdelta[key & ~kDeadSlot]);
                 }
             }
-        }
-        const bool claim = n != kNone && atomicExch(&L.affected[n], stamp) != stamp;
-        const u64 mask = __ballot(claim);
-        if (mask) {
-            const u32 leader = u32(__ffsll((long long)mask) - 1);
-            u32 at = 0;
-            if (lane == leader) at = atomicAdd(&ctl->affected_count, u32(__popcll(mask)));
-            at = u32(__builtin_amdgcn_readlane(int(at), int(leader)));
-            if (claim) L.affected_list[at + u32(__popcll(mask & ((1ULL << lane) - 1ULL)))] = n;
+            const bool claim = n != kNone && atomicExch(&L.affected[n], stamp) != stamp;
+            const u64 mask = __ballot(claim);
+            if (mask) {
+                const u32 leader = u32(__ffsll((long long)mask) - 1);
+                u32 at = 0;
+                if (lane == leader) at = atomicAdd(&ctl->affected_count, u32(__popcll(mask)));
+                at = u32(__builtin_amdgcn_readlane(int(at), int(leader)));
+                if (claim) L.affected_list[at + u32(__popcll(mask & ((1ULL << lane) - 1ULL)))] = n;
+            }
         }
     }
 }
@@ -547,10 +547,11 @@ template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_st
     if (L.ctl->stop || !L.ctl->dirty_start[half]) return;
     __shared__ float s_max[4][kWG / 64];
     float mc = 0.f, ms = 0.f;
-    const u64 pairs = u64(L.ctl->dirty_count[half]) * L.D;
+    // Node slots D * N are below 2^32 (refused at upload otherwise).
+    const u32 pairs = L.ctl->dirty_count[half] * L.D;
     const u32 *list = L.dirty_list + size_t(half) * L.N;
-    for (u64 q = u64(m.a) * kWG + threadIdx.x; q < pairs; q += u64(m.b) * kWG) {
-        const u32 i = list[q / L.D], k = u32(q % L.D);
+    for (u32 q = m.a * kWG + threadIdx.x; q < pairs; q += m.b * kWG) {
+        const u32 i = list[q / L.D], k = q % L.D;
         const u32 o = L.nbr_bond[size_t(k) * L.N + i];
         if (o == kNone || !L.alive_n[o]) continue;
         const BondNode r = L.bnd[o];
@@ -1441,13 +1442,14 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
     if (COVER == kBondAffected) {
         // Bonds with an affected end, each taken once: from its first end when
         // that end is affected, else from its second.
-        const u64 pairs = u64(ctl->affected_count) * L.D;
-        for (u64 base = u64(m->a) * kWG; base < pairs; base += u64(m->b) * kWG) {
-            const u64 q = base + threadIdx.x;
+        // Node slots D * N are below 2^32 (refused at upload otherwise).
+        const u32 pairs = ctl->affected_count * L.D;
+        for (u32 base = m->a * kWG; base < pairs; base += m->b * kWG) {
+            const u32 q = base + threadIdx.x;
             u32 claim_a = kNone, claim_b = kNone;
             double pw = 0.0;
             if (q < pairs) {
-                const u32 i = L.affected_list[q / L.D], k = u32(q % L.D);
+                const u32 i = L.affected_list[q / L.D], k = q % L.D;
                 const u32 o = L.nbr_bond[size_t(k) * L.N + i];
                 if (o != kNone) {
                     const u32 a = L.bnd[o].a;
