@@ -1681,11 +1681,12 @@ private:
         capturing_ = false;
         hipGraph_t g = nullptr;
         const hipError_t end = hipStreamEndCapture(stream_, &g);
+        hipError_t inst = hipSuccess;
+        if (err == hipSuccess && end == hipSuccess) inst = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (inst != hipSuccess) graph_exec_ = nullptr;
         HIP_OK(err);
         HIP_OK(end);
-        const hipError_t inst = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (inst != hipSuccess) graph_exec_ = nullptr;
         HIP_OK(inst);
     }
 
