@@ -1041,7 +1041,12 @@ public:
         uint32_t running = countRunning();
         double ksec = 0.0;
         while (running > 0 && done < substeps) {
-            if (maps_dirty_) buildMaps();
+            if (maps_dirty_) {
+                // The captured substeps hold the old maps' offsets, counts and
+                // grids as launch arguments: capture again after a rebuild.
+                buildMaps();
+                dropGraph();
+            }
             const uint64_t n = std::min<uint64_t>(profilingBatch(batch_len), substeps - done);
             HIP_OK(hipEventRecord(ev0_, stream_));
             uint64_t q = 0;
@@ -1681,12 +1686,22 @@ private:
         capturing_ = false;
         hipGraph_t g = nullptr;
         const hipError_t end = hipStreamEndCapture(stream_, &g);
+        // Every stream the capture forked into must have been joined back into
+        // stream_ by an event (hipStreamEndCapture reports an unjoined fork,
+        // and a stream left capturing would capture the next eager launches).
+        bool left = false;
+        for (hipStream_t s : {side_, early_}) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) left = true;
+        }
         hipError_t inst = hipSuccess;
-        if (err == hipSuccess && end == hipSuccess) inst = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+        if (err == hipSuccess && end == hipSuccess && !left)
+            inst = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
         if (g) (void)hipGraphDestroy(g);
-        if (inst != hipSuccess) graph_exec_ = nullptr;
+        if (inst != hipSuccess || left) graph_exec_ = nullptr;
         HIP_OK(err);
         HIP_OK(end);
+        if (left) throw Runtime("a side stream was left unjoined in the substep capture");
         HIP_OK(inst);
     }
 
@@ -1699,6 +1714,7 @@ private:
         hipEvent_t ev_ = (ev);                                                                   \
         if (capturing_) {                                                                        \
             hipLaunchKernelGGL(kernel, grid, block, 0, stream, __VA_ARGS__);                     \
+            HIP_OK(hipGetLastError());                                                           \
             if (ev_) HIP_OK(hipEventRecord(ev_, stream));                                        \
         } else {                                                                                 \
             hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, nullptr, ev_, 0, __VA_ARGS__); \
@@ -1770,7 +1786,11 @@ private:
         }
         const uint32_t iterations = shared_.iterations;
         for (uint32_t it = 0; it < iterations; ++it) {
-            const uint32_t flags = (it == 0 ? kSolveFirst : 0U) | (it + 1 < iterations ? kSolveKeepLambda : 0U);
+            // With more than one iteration every solve stores its multiplier, as
+            // the reference does (LatticePhysics.hpp:1163): a bond that skips
+            // iteration 0 (a zero-length bond) reads the previous substep's
+            // last value in iteration 1.
+            const uint32_t flags = (it == 0 ? kSolveFirst : 0U) | (iterations > 1 ? kSolveKeepLambda : 0U);
             // One timing bracket over the back-to-back stage launches of the
             // interior phase and one over the boundary phases.
             for (int phase = 0; phase < 2; ++phase) {

@@ -95,7 +95,7 @@ struct DevCtl {
 // Launch xcdGrid(tiles) workgroups; a result >= tiles is an idle workgroup.
 constexpr u32 kXcds = 8;
 constexpr u32 kSolveFirst = 1U;      // k_solve: first constraint iteration, the multiplier starts at 0
-constexpr u32 kSolveKeepLambda = 2U; // k_solve: a later iteration reads the multiplier
+constexpr u32 kSolveKeepLambda = 2U; // k_solve: store the multiplier (runs of more than one iteration)
 __host__ __device__ __forceinline__ u32 xcdGrid(u32 tiles) { return (tiles + kXcds - 1) / kXcds * kXcds; }
 __device__ __forceinline__ u32 xcdTile(u32 w, u32 tiles) {
     const u32 per = (tiles + kXcds - 1) / kXcds;
@@ -1096,7 +1096,8 @@ __global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags)
     const R wa = ua.w, wb = ub.w;
     const R alpha = r.c / (S.dt * S.dt);
     const R dl = (-C - alpha * lam) / (wa + wb + alpha);
-    // The multiplier is read only by a later iteration of the same substep.
+    // The multiplier is read only by a later iteration, so a one-iteration run
+    // never stores it.
     if (flags & kSolveKeepLambda) stg(&L.lambda[j], lam + dl);
     stStream(&L.u4[r.a], with(xyz(ua) - wa * dl * dir, ua.w));
     stStream(&L.u4[r.b], with(xyz(ub) + wb * dl * dir, ub.w));

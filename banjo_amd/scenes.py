@@ -314,6 +314,21 @@ def small_impact(nx=24, ny=12, nz=24, ball_radius=0.05, ball_mass=2.0, ball_spee
                   substeps=substeps, blocks=blocks, seed=seed, name=f"impact_{nx}x{ny}x{nz}")
 
 
+def overflow_impact(substeps=30, blocks=1, iterations=1):
+    """A wide, thin slab under the config-2 sphere (r = 0.2 m, 10 kg, 8 m/s).
+
+    The sphere's footprint puts thousands of nodes on the striker's candidate
+    test per substep, more than the 1,024 a block lists
+    (LatticePhysics.hpp:105): the rest are neither contacted nor predicted
+    that substep (CpuLatticeBackend.cpp:377-385), the regime the 128x64x128
+    impact runs in with one block.
+    """
+    sc = impact(48, 10, 48, ball_radius=0.2, ball_mass=10.0, substeps=substeps, blocks=blocks,
+                name=f"overflow_impact_48x10x48_{blocks}blocks")
+    sc.settings.constraint_iterations = iterations
+    return sc
+
+
 def ledge_plane(y, node_radius, footprints, restitution=0.1, static_friction=0.6, dynamic_friction=0.5):
     """A reach-capped support plane with finite footprints (TileImpactScene.cpp:322-332)."""
     p = ground_plane(y, node_radius, restitution, static_friction, dynamic_friction)

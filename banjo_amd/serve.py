@@ -101,6 +101,12 @@ class TrialSession:
             raise ValueError(f"at most {self.max_templates} templates")
         sched, state = prepare(self.lib, scene)
         self.batch.upload(k, sched, state, scene.settings, scene.sphere)
+        need = self.batch.slot_bytes(k)
+        if need > self.budget:
+            # A trial of it could never be admitted: refuse the template
+            # rather than queue trials that would wait for ever.
+            self.batch.release(k)
+            raise ValueError(f"template needs {need} bytes of HBM, above the {self.budget}-byte budget")
         self.templates.append(scene)
         self.prepared.append((sched, state))
         self.footprint.append(self.batch.slot_bytes(k))
@@ -109,6 +115,8 @@ class TrialSession:
     def submit(self, t: Trial):
         if not 0 <= t.template < len(self.templates):
             raise ValueError(f"no template {t.template}")
+        if self.footprint[t.template] > self.budget:
+            raise ValueError(f"template {t.template} does not fit the HBM budget")
         self.pending.append(t)
 
     @property

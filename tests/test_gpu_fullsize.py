@@ -3,8 +3,9 @@
 The config-2 impact (128x64x128: 1,048,576 nodes, 13,338,492 bonds) and the
 config-5 plank cut (256x32x32: 262,144 nodes, 3,253,116 bonds) are checked
 through what holds at any size:
-- the first substeps equal the oracle bit for bit (the striker's first
-  contacts and the start sample of every node included);
+- the first 14 substeps equal the oracle bit for bit (the striker's first
+  contacts, the start sample of every node and the first substeps whose
+  candidates overflow the block's list included);
 - a run repeats bit for bit;
 - the device's fragment labels equal the host union-find on the same bonds;
 - a batch slot equals the plain backend;
@@ -30,11 +31,14 @@ def c2_run(gpu):
 
 
 def test_config2_first_substeps_equal_the_oracle(gpu, oracle):
-    sc = scenes.config2(substeps=3)
-    print("config 2, 3 substeps, HIP against the oracle", flush=True)
+    # The single block's candidate list first overflows its 1,024 entries in
+    # substep 10; 14 substeps take the comparison four substeps into that regime.
+    sc = scenes.config2(substeps=14)
+    print("config 2, 14 substeps, HIP against the oracle", flush=True)
     h = run_scene(gpu, sc, F, labels="device")
     o = run_scene(oracle, sc, F)
     assert h["status"].contact.impulse_contacts > 0
+    assert h["status"].contact.candidate_overflow > 0
     assert_same(h, o)
 
 

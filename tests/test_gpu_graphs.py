@@ -66,3 +66,23 @@ def test_batch_graphs_equal_enqueued_substeps(gpu, monkeypatch):
         for k in ("u", "v", "alive", "damage", "labels"):
             assert np.array_equal(a[k], b[k]), k
         assert a["status"].broken_bonds == b["status"].broken_bonds
+
+
+def test_batch_graphs_with_uneven_budgets_equal_enqueued(gpu, monkeypatch):
+    # Trials stop at different substeps, so the launch maps are rebuilt between
+    # graph replays; a rebuild must drop the captured graph (its launch
+    # arguments hold the old maps) and capture again.
+    trials = [scenes.config3_trial(t, substeps=96) for t in range(4)]
+    steps = [40, 96, 17, 64]
+    kws = [{}, {"no_failure_steps": 30}, {}, {"calm_steps": 20}]
+    print(f"batch of {len(trials)} with budgets {steps}, graphs against enqueued", flush=True)
+    monkeypatch.delenv("BLX_GRAPHS", raising=False)
+    eager = run_batch(gpu, trials, F, substeps=steps, control_kw=kws, steps_per_launch=8)
+    monkeypatch.setenv("BLX_GRAPHS", "1")
+    graphs = run_batch(gpu, trials, F, substeps=steps, control_kw=kws, steps_per_launch=8)
+    for a, b in zip(graphs, eager):
+        for k in ("u", "v", "alive", "damage", "labels"):
+            assert np.array_equal(a[k], b[k]), k
+        assert a["status"].total_steps == b["status"].total_steps
+        assert a["status"].exit_reason == b["status"].exit_reason
+        assert a["status"].broken_bonds == b["status"].broken_bonds

@@ -60,10 +60,22 @@ def assert_same(a, b, ledger_rtol=1e-12):
         assert getattr(a["sphere"], "velocity")[k] == getattr(b["sphere"], "velocity")[k]
 
 
+def _iterations(sc, n):
+    sc.settings.constraint_iterations = n
+    return sc
+
+
 CASES = {
     "cantilever16_300": lambda: scenes.config1(substeps=300),
     "impact_1block": lambda: scenes.small_impact(substeps=150),
     "impact_4slabs": lambda: scenes.small_impact(substeps=150, blocks=4),
+    # More striker candidates than a block lists: the capped, ordered lists and
+    # the overflow count of CpuLatticeBackend.cpp:377-385 on every contact substep.
+    "overflow_1block": lambda: scenes.overflow_impact(),
+    "overflow_4slabs": lambda: scenes.overflow_impact(blocks=4),
+    # Two constraint iterations: the carried multiplier and the projection between them.
+    "impact_2iterations": lambda: scenes.overflow_impact(substeps=20, iterations=2),
+    "bridge_2iterations": lambda: _iterations(scenes.bridge_impact(substeps=80), 2),
 }
 
 
@@ -75,4 +87,6 @@ def test_hip_matches_oracle(gpu, oracle, case, precision):
     h = run_scene(gpu, sc, precision, labels="device")
     o = run_scene(oracle, sc, precision)
     assert h["status"].broken_bonds > 0
+    if case.startswith("overflow"):
+        assert h["status"].contact.candidate_overflow > 1000
     assert_same(h, o)

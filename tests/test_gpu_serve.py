@@ -73,3 +73,15 @@ def test_admission_respects_the_memory_budget(gpu):
         assert s["trials"] == len(PLAN) and s["p99_s"] >= s["p50_s"] > 0
     finally:
         server.close()
+
+
+def test_template_above_the_budget_is_refused(gpu):
+    from banjo_amd.serve import TrialSession
+    s = TrialSession(gpu, slots=2, max_templates=2, hbm_budget_bytes=1 << 20)
+    try:
+        with pytest.raises(ValueError):
+            s.add_template(scenes.served_lattice((16, 8, 16)))
+        s.budget = 1 << 30
+        assert s.add_template(scenes.served_lattice((8, 6, 8))) == 0
+    finally:
+        s.close()

@@ -66,7 +66,22 @@ RUNS = {
     "plastic_double_4slabs": (lambda: scenes.plastic_impact(blocks=4, substeps=60), D, False),
     "bridge_ledges_float": (lambda: scenes.bridge_impact(substeps=100), F, False),
     "bridge_ledges_3slabs_double": (lambda: scenes.bridge_impact(blocks=3, substeps=80), D, True),
+    # More striker candidates than a block lists (1,024): the overflow branch
+    # of CpuLatticeBackend.cpp:377-385 on every substep of the contact.
+    "overflow_1block_float": (lambda: scenes.overflow_impact(), F, False),
+    "overflow_1block_double": (lambda: scenes.overflow_impact(), D, False),
+    "overflow_4slabs_float": (lambda: scenes.overflow_impact(blocks=4), F, False),
+    "overflow_4slabs_double": (lambda: scenes.overflow_impact(blocks=4), D, False),
+    # Two constraint iterations: the multiplier carried between them
+    # (LatticePhysics.hpp:1160-1163) and the support projection after each.
+    "impact_2iterations_float": (lambda: scenes.overflow_impact(substeps=20, iterations=2), F, False),
+    "bridge_2iterations_double": (lambda: _iterations(scenes.bridge_impact(substeps=80), 2), D, False),
 }
+
+
+def _iterations(sc, n):
+    sc.settings.constraint_iterations = n
+    return sc
 
 
 @pytest.mark.parametrize("name", list(RUNS))
@@ -77,6 +92,8 @@ def test_runs_match_reference_bit_for_bit(oracle, ref, name):
     o = run_scene(oracle, sc, precision)
     r = run_scene(ref, sc, precision, backend_args=(0, 1))
     assert r["status"].broken_bonds > 0
+    if name.startswith("overflow"):
+        assert r["status"].contact.candidate_overflow > 1000
     same_run(o, r)
     assert np.array_equal(o["labels"], r["labels"])
 
