@@ -30,6 +30,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -130,6 +131,42 @@ private:
         slots_.swap(next);
     }
 };
+
+// Scratch of the allocation-free fragment labels
+// (blx_backend_fragment_labels_noalloc): every buffer of one label computation
+// of at most kNoAllocMaxNodes nodes, carved from one file-scope device array
+// that each GPU holds from module load to process end. In host code `scratch`
+// names a host-side stand-in, not device memory: its device address on each
+// device comes from hipGetSymbolAddress (scratchOn), and only that address is
+// passed to kernels and copies. Calls on one GPU share its array, so calls
+// that overlap on one GPU race.
+constexpr uint32_t kNoAllocMaxNodes = 1U << 20;
+constexpr size_t kNoAllocTempBytes = size_t(16) << 20;  // radix sort temporary storage
+constexpr size_t kNoAllocBytes = size_t(kNoAllocMaxNodes) * (4 * 4 + 2 * 8) + 256 + kNoAllocTempBytes;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current.
+void *scratchOn(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Invalid("device index out of range for the label scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int current = 0;
+        HIP_OK(hipGetDevice(&current));
+        HIP_OK(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(current);
+        if (e != hipSuccess || !p)
+            throw Runtime(std::string("hipGetSymbolAddress of the label scratch: ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
 
 int deviceCount() {
     int n = 0;
@@ -840,6 +877,7 @@ public:
     }
 
     uint32_t capacity() const { return uint32_t(slots_.size()); }
+    int device() const { return device_; }
     std::string name() const { return std::string("hip-gfx950-") + (sizeof(R) == 4 ? "float" : "double"); }
 
     Slot<R> &slot(uint32_t k) {
@@ -1274,7 +1312,49 @@ public:
         slot(k).status.frames_captured = 0;
     }
 
-    // Union-find, roots, canonical rank and labels of one slot, on the device.
+    // The device buffers of one label computation: N-node parent, root, size
+    // and rank arrays, N 64-bit sort keys twice, the component counter and the
+    // radix sort's temporary storage.
+    struct LabelScratch {
+        uint32_t *parent, *root, *size, *rank, *count;
+        uint64_t *keys, *sorted;
+        void *temp;
+        size_t temp_bytes;
+    };
+    // The radix sort's temporary bytes for n keys (a size query; allocates nothing).
+    size_t labelTempBytes(uint32_t n) {
+        size_t temp = 0;
+        HIP_OK(rocprim::radix_sort_keys(nullptr, temp, static_cast<uint64_t *>(nullptr),
+                                        static_cast<uint64_t *>(nullptr), n, 0, 64, stream_));
+        return temp;
+    }
+
+    // Union-find, roots, canonical rank and labels of one slot, on the device,
+    // in the buffers of s; the labels go to label_dev (device memory).
+    uint32_t labelsInto(uint32_t k, const LabelScratch &s, uint32_t *label_dev) {
+        Slot<R> &sl = slot(k);
+        const uint32_t N = sl.N;
+        const uint32_t g = gridFor(N);
+        hipLaunchKernelGGL(k_cc_zero, dim3(1), dim3(64), 0, stream_, s.count);
+        const uint32_t ranges = std::max(1U, (N + kCcRange - 1) / kCcRange);
+        hipLaunchKernelGGL(k_cc_local<R>, dim3(ranges), dim3(kCcRange), 0, stream_, sl.L, s.parent, s.size);
+        hipLaunchKernelGGL(k_cc_cross<R>, dim3(ranges), dim3(kCcRange), 0, stream_, sl.L, s.parent);
+        hipLaunchKernelGGL(k_cc_flatten<true>, dim3(gridFew(N)), dim3(kWG), 0, stream_, s.parent, s.root, s.size, N);
+        hipLaunchKernelGGL(k_cc_keys, dim3(g), dim3(kWG), 0, stream_, s.root, s.size, s.keys, s.count, N);
+        HIP_OK(hipGetLastError());
+        uint32_t c = 0;
+        HIP_OK(hipMemcpyAsync(&c, s.count, 4, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        size_t bytes = s.temp_bytes;
+        HIP_OK(rocprim::radix_sort_keys(s.temp, bytes, s.keys, s.sorted, c, 0, 64, stream_));
+        hipLaunchKernelGGL(k_cc_rank, dim3(gridFor(c)), dim3(kWG), 0, stream_, s.sorted, s.rank, c);
+        hipLaunchKernelGGL(k_cc_label, dim3(g), dim3(kWG), 0, stream_, s.root, s.rank, label_dev, N);
+        HIP_OK(hipGetLastError());
+        return c;
+    }
+
+    // Labels of one slot in the slot's own buffers (allocated on first use);
+    // label (host memory, or NULL) receives them.
     uint32_t labels(uint32_t k, uint32_t *label) {
         Slot<R> &sl = slot(k);
         if (!sl.uploaded) throw Logic("fragments before upload");
@@ -1290,30 +1370,50 @@ public:
         sl.cc_keys.alloc(size_t(N) * 8);
         sl.cc_sorted.alloc(size_t(N) * 8);
         sl.cc_count.alloc(4);
-        size_t temp = 0;
-        uint64_t *keys = sl.cc_keys.template as<uint64_t>(), *sorted = sl.cc_sorted.template as<uint64_t>();
-        HIP_OK(rocprim::radix_sort_keys(nullptr, temp, keys, sorted, N, 0, 64, stream_));
-        sl.cc_temp.alloc(temp);
-        const uint32_t g = gridFor(N);
-        uint32_t *parent = sl.cc_parent.template as<uint32_t>(), *root = sl.cc_root.template as<uint32_t>(),
-                 *size = sl.cc_size.template as<uint32_t>(), *count = sl.cc_count.template as<uint32_t>();
-        HIP_OK(hipMemsetAsync(count, 0, 4, stream_));
-        const uint32_t ranges = std::max(1U, (N + kCcRange - 1) / kCcRange);
-        hipLaunchKernelGGL(k_cc_local<R>, dim3(ranges), dim3(kCcRange), 0, stream_, sl.L, parent, size);
-        hipLaunchKernelGGL(k_cc_cross<R>, dim3(ranges), dim3(kCcRange), 0, stream_, sl.L, parent);
-        hipLaunchKernelGGL(k_cc_flatten<true>, dim3(gridFew(N)), dim3(kWG), 0, stream_, parent, root, size, N);
-        hipLaunchKernelGGL(k_cc_keys, dim3(g), dim3(kWG), 0, stream_, root, size, keys, count, N);
-        HIP_OK(hipGetLastError());
-        uint32_t c = 0;
-        HIP_OK(hipMemcpyAsync(&c, count, 4, hipMemcpyDeviceToHost, stream_));
-        HIP_OK(hipStreamSynchronize(stream_));
-        size_t bytes = sl.cc_temp.bytes;
-        HIP_OK(rocprim::radix_sort_keys(sl.cc_temp.p, bytes, keys, sorted, c, 0, 64, stream_));
-        hipLaunchKernelGGL(k_cc_rank, dim3(gridFor(c)), dim3(kWG), 0, stream_, sorted, sl.cc_rank.template as<uint32_t>(), c);
-        hipLaunchKernelGGL(k_cc_label, dim3(g), dim3(kWG), 0, stream_, root, sl.cc_rank.template as<uint32_t>(),
-                           sl.cc_label.template as<uint32_t>(), N);
-        HIP_OK(hipGetLastError());
+        sl.cc_temp.alloc(labelTempBytes(N));
+        const LabelScratch s{sl.cc_parent.template as<uint32_t>(), sl.cc_root.template as<uint32_t>(),
+                             sl.cc_size.template as<uint32_t>(), sl.cc_rank.template as<uint32_t>(),
+                             sl.cc_count.template as<uint32_t>(), sl.cc_keys.template as<uint64_t>(),
+                             sl.cc_sorted.template as<uint64_t>(), sl.cc_temp.p, sl.cc_temp.bytes};
+        const uint32_t c = labelsInto(k, s, sl.cc_label.template as<uint32_t>());
         if (label) HIP_OK(hipMemcpyAsync(label, sl.cc_label.p, n4, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        return c;
+    }
+
+    // Labels of one slot with every buffer carved from `scratch` (bytes long,
+    // device memory): allocates nothing. label_dev is the caller's device buffer.
+    uint32_t labelsNoAlloc(uint32_t k, void *scratch, size_t bytes, uint32_t max_nodes, uint32_t *label_dev) {
+        Slot<R> &sl = slot(k);
+        if (!sl.uploaded) throw Logic("fragments before upload");
+        if (!label_dev) throw Invalid("the labels need a device buffer of node_count entries");
+        const uint32_t N = sl.N;
+        if (N > max_nodes)
+            throw Invalid("the allocation-free labels hold at most " + std::to_string(max_nodes) + " nodes; this lattice has " +
+                          std::to_string(N));
+        HIP_OK(hipSetDevice(device_));
+        if (N == 0) return 0;
+        // Carve at the maximum size, so every call uses the same offsets.
+        auto at = [&, off = size_t(0)](size_t n) mutable {
+            void *p = static_cast<char *>(scratch) + off;
+            off += (n + 255) & ~size_t(255);
+            return p;
+        };
+        const size_t m4 = size_t(max_nodes) * 4, m8 = size_t(max_nodes) * 8;
+        LabelScratch s{};
+        s.parent = static_cast<uint32_t *>(at(m4));
+        s.root = static_cast<uint32_t *>(at(m4));
+        s.size = static_cast<uint32_t *>(at(m4));
+        s.rank = static_cast<uint32_t *>(at(m4));
+        s.keys = static_cast<uint64_t *>(at(m8));
+        s.sorted = static_cast<uint64_t *>(at(m8));
+        s.count = static_cast<uint32_t *>(at(4));
+        const size_t used = 4 * ((m4 + 255) & ~size_t(255)) + 2 * ((m8 + 255) & ~size_t(255)) + 256;
+        const size_t temp = labelTempBytes(N);
+        if (used + temp > bytes) throw Overflow("the radix sort needs more temporary storage than the scratch array holds");
+        s.temp = static_cast<char *>(scratch) + used;
+        s.temp_bytes = bytes - used;
+        const uint32_t c = labelsInto(k, s, label_dev);
         HIP_OK(hipStreamSynchronize(stream_));
         return c;
     }
@@ -1987,6 +2087,8 @@ template <class F> auto dispatch(const Handle *h, F &&f) { return dispatch(const
 Handle *makeHandle(int device, int precision, uint32_t capacity) {
     if (deviceCount() <= device || device < 0)
         throw Runtime("no HIP device " + std::to_string(device) + " for the lattice backend");
+    // Set-up of this device's label scratch address, once per device.
+    (void)scratchOn(device);
     auto h = std::make_unique<Handle>();
     h->precision = precision;
     if (precision == BLX_PRECISION_FLOAT) h->impl = std::make_unique<EngineHolder<float>>(device, capacity);
@@ -2103,6 +2205,14 @@ int blx_backend_clear_frames(blx_backend *b) {
 }
 int blx_backend_fragment_labels(blx_backend *b, uint32_t *label, uint32_t *count) {
     return guard([&] { *count = dispatch(H(b), [&](auto &e) { return e.labels(0, label); }); });
+}
+
+int blx_backend_fragment_labels_noalloc(blx_backend *b, uint32_t *label_device, uint32_t *count) {
+    return guard([&] {
+        *count = dispatch(H(b), [&](auto &e) {
+            return e.labelsNoAlloc(0, scratchOn(e.device()), kNoAllocBytes, kNoAllocMaxNodes, label_device);
+        });
+    });
 }
 
 int blx_backend_fragment_mass_properties(blx_backend *b, double cell_size_m, const double *spin,

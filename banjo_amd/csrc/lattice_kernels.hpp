@@ -2049,6 +2049,12 @@ __global__ void k_cc_keys(const u32 *root, const u32 *size, u64 *keys, u32 *coun
         if (root[i] == i) keys[atomicAdd(count, 1U)] = (u64(n - size[i]) << 32) | u64(i);
 }
 
+// Clears the component counter with a kernel rather than a memset, so the
+// label launches hold no memset node when a caller captures them in a graph.
+__global__ void k_cc_zero(u32 *count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0U;
+}
+
 __global__ void k_cc_rank(const u64 *sorted, u32 *rank, u32 c) {
     for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < c; k += gridDim.x * blockDim.x)
         rank[u32(sorted[k] & 0xffffffffULL)] = k;

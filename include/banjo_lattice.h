@@ -356,6 +356,19 @@ int blx_backend_clear_frames(blx_backend *backend);
 int blx_backend_fragment_labels(blx_backend *backend, uint32_t *label,
                                 uint32_t *component_count);
 
+/* blx_backend_fragment_labels without allocating. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * same kernels and the same labels, but every buffer they use comes from one
+ * fixed scratch array in device memory that each GPU holds until the process
+ * ends (48 MiB), so the lattice may have at most 1,048,576 (2^20) nodes; a
+ * larger one is refused with BLX_ERR_INVALID. label_device is the caller's device buffer of
+ * node_count entries, written on the backend's stream (complete on return).
+ * Calls on one GPU share that GPU's scratch array: calls that overlap on one
+ * GPU, from several threads or several backends, race. */
+int blx_backend_fragment_labels_noalloc(blx_backend *backend, uint32_t *label_device,
+                                        uint32_t *component_count);
+
 /* calculateFragmentMassProperties (fracture/FragmentMassProperties.cpp:7-80)
  * of every fragment of the current state, on the device, as the fragment
  * build calls it per component (FragmentGeometry.cpp:231-234): out[f] for
