@@ -1092,7 +1092,11 @@ public:
                 if (!graph_exec_) captureSubsteps();
                 for (; q + kGraphSubsteps <= n; q += kGraphSubsteps) HIP_OK(hipGraphLaunch(graph_exec_, stream_));
             }
+            const uint64_t q_sample = q;
             for (; q < n; ++q) {
+                // The batch's first enqueued substep is timed phase by phase
+                // (phase_seconds); the rest run without markers.
+                sample_ = !prof_ && q == q_sample;
                 enqueueSubstepA();
                 if (S_.nc.mode != 0U) {
                     // The rebuild decision of each slot, read back once a substep.
@@ -1101,6 +1105,7 @@ public:
                     rebuildContacts();
                 }
                 enqueueSubstepB();
+                sample_ = false;
             }
             // side_ ends each batch joined: its last k_rinv is done when the batch is.
             HIP_OK(hipStreamWaitEvent(stream_, ev_rinv_, 0));
@@ -1109,7 +1114,7 @@ public:
             float ms = 0.f;
             HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
             ksec += double(ms) * 1e-3;
-            profCollect();
+            profCollect(double(ms) * 1e-3);
             ++launches_;
             done += n;
             readAllCtls();
@@ -1141,7 +1146,12 @@ public:
         HIP_OK(hipSetDevice(device_));
         const auto t0 = std::chrono::steady_clock::now();
         arm(0, c);
-        const uint64_t batch = c.steps_per_launch ? c.steps_per_launch : kDefaultBatch;
+        // RunControl::steps_per_launch (FastLattice.hpp:120-121): substeps per
+        // device batch, 0 for the whole run in one. The host enqueues a batch
+        // without waiting on the device; a run that stops early (its exit
+        // rule) turns the rest of its batch into no-ops, and abandon is polled
+        // between batches.
+        const uint64_t batch = c.steps_per_launch ? c.steps_per_launch : std::max<uint64_t>(1, c.max_steps);
         uint64_t done = 0;
         unsigned reason = 0;
         while (done < c.max_steps) {
@@ -1513,8 +1523,6 @@ public:
     }
 
 private:
-    static constexpr uint64_t kDefaultBatch = 240;
-
     int device_;
     std::vector<std::unique_ptr<Slot<R>>> slots_;
     Shared shared_;
@@ -1710,13 +1718,16 @@ private:
     }
     struct ProfRecord { uint32_t kernel, phase; hipEvent_t a, b; };
     bool prof_ = false;
+    bool sample_ = false;  // the substep being enqueued is the batch's phase-timed one
+    double sample_share_[BLX_PHASE_COUNT] = {};  // phase shares of the last timed substep
+    bool have_share_ = false;
     std::vector<hipEvent_t> pool_;
     std::vector<ProfRecord> records_;
     double kernel_seconds_[KCount] = {};
     uint64_t kernel_launches_[KCount] = {};
 
     void profBegin(uint32_t kernel, uint32_t phase, hipStream_t on = nullptr) {
-        if (!prof_) return;
+        if (!prof_ && !sample_) return;
         if (!on) on = stream_;
         const size_t need = 2 * (records_.size() + 1);
         while (pool_.size() < need) {
@@ -1729,22 +1740,49 @@ private:
         records_.push_back(r);
     }
     void profEnd(hipStream_t on = nullptr) {
-        if (!prof_) return;
+        if (!prof_ && !sample_) return;
         HIP_OK(hipEventRecord(records_.back().b, on ? on : stream_));
     }
     void profAddLaunches(uint32_t kernel, uint32_t extra) {
         if (prof_) kernel_launches_[kernel] += extra;
     }
-    void profCollect() {
-        for (const ProfRecord &r : records_) {
-            float ms = 0.f;
-            HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
-            kernel_seconds_[r.kernel] += double(ms) * 1e-3;
-            kernel_launches_[r.kernel] += 1;
-            for (auto &s : slots_)
-                if (s->armed) s->status.phase_seconds[r.phase] += double(ms) * 1e-3;
+    // RunStatus::phase_seconds (FastLattice.hpp:225-228). With profiling on,
+    // every launch is timed and each phase gets its kernels' event time. With
+    // it off, each batch times one substep phase by phase and splits the
+    // batch's device time (batch_seconds) in those proportions; kernels of
+    // overlapping streams share the wall time as they share the sampled sum.
+    void profCollect(double batch_seconds) {
+        if (prof_) {
+            for (const ProfRecord &r : records_) {
+                float ms = 0.f;
+                HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+                kernel_seconds_[r.kernel] += double(ms) * 1e-3;
+                kernel_launches_[r.kernel] += 1;
+                for (auto &s : slots_)
+                    if (s->armed) s->status.phase_seconds[r.phase] += double(ms) * 1e-3;
+            }
+            records_.clear();
+            return;
         }
-        records_.clear();
+        if (!records_.empty()) {
+            double ph[BLX_PHASE_COUNT] = {};
+            double total = 0.0;
+            for (const ProfRecord &r : records_) {
+                float ms = 0.f;
+                HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+                if (r.phase < BLX_PHASE_COUNT) ph[r.phase] += double(ms);
+                total += double(ms);
+            }
+            records_.clear();
+            if (total > 0.0) {
+                for (uint32_t p = 0; p < BLX_PHASE_COUNT; ++p) sample_share_[p] = ph[p] / total;
+                have_share_ = true;
+            }
+        }
+        if (!have_share_) return;
+        for (auto &s : slots_)
+            if (s->armed)
+                for (uint32_t p = 0; p < BLX_PHASE_COUNT; ++p) s->status.phase_seconds[p] += sample_share_[p] * batch_seconds;
     }
     uint64_t profilingBatch(uint64_t batch) const { return prof_ ? std::min<uint64_t>(batch, 4) : batch; }
 

@@ -248,40 +248,70 @@ def served_leg(args):
     return s
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: every core this process may run on
+    (its affinity), within the per-GPU CPU share the box sets in
+    OMP_NUM_THREADS when it sets one."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    limit = int(share) if share.isdigit() and int(share) > 0 else avail
+    return max(1, min(avail, limit)), avail
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
 def cpu_leg(args, sc):
-    """The reference's threaded CPU lane over a bounded window of the same workload."""
+    """The reference's threaded CPU lane over bounded windows of the same
+    workload: the first substeps from t = 0 (the first samples every node),
+    then a warm window that follows them, which is the reported value."""
     from banjo_amd import abi
     from banjo_amd.lattice import LatticeLibrary, control
     from banjo_amd.runner import prepare
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libbanjo_ref.so")
     orc_path = os.path.join(ROOT, "oracle", "build", "liboracle.so")
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, avail = cpu_threads()
+    cold_n, warm_n = args.cpu_substeps, args.cpu_warm_substeps
     if os.path.exists(ref_path):
-        lib, kind, extra, cores, window = LatticeLibrary(ref_path, "ref_", "ref"), "reference", (1, threads), threads, args.cpu_substeps
+        lib, kind, extra, cores = LatticeLibrary(ref_path, "ref_", "ref"), "reference", (1, threads), threads
     elif os.path.exists(orc_path):
-        lib, kind, extra, cores, window = LatticeLibrary(orc_path, "orc_", "oracle"), "port", (), 1, max(1, args.cpu_substeps // 5)
+        lib, kind, extra, cores = LatticeLibrary(orc_path, "orc_", "oracle"), "port", (), 1
+        cold_n, warm_n = max(1, cold_n // 5), max(1, warm_n // 10)
     else:
         return None
     sched, state = prepare(lib, sc)
     be = lib.backend(abi.PRECISION_FLOAT, sched, *extra)
     be.upload(state, sc.settings, sc.sphere)
     t0 = time.perf_counter()
-    st = be.run(control(window))
-    seconds = time.perf_counter() - t0
+    st = be.run(control(cold_n))
+    cold_s = time.perf_counter() - t0
+    cold_steps = st.total_steps
+    t0 = time.perf_counter()
+    st = be.run(control(warm_n))
+    warm_s = time.perf_counter() - t0
+    warm_steps = st.total_steps - cold_steps
     be.close()
-    value = sc.bond_count * st.total_steps / seconds
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    value = sc.bond_count * warm_steps / warm_s
+    lane = "ParallelCpuLatticeBackend" if kind == "reference" else "serial oracle"
     out = {"value": value, "unit": "bond-solves/s", "cores": cores, "kind": kind,
-           "sample": f"first {st.total_steps} substeps of the same 128x64x128 impact from t=0, float, "
-                     f"{'ParallelCpuLatticeBackend' if kind == 'reference' else 'serial oracle'} with {cores} "
-                     f"thread(s); {seconds:.2f} s; host {cpu_model}".strip()}
+           "sample": f"substeps {cold_steps}-{cold_steps + warm_steps} of the same 128x64x128 impact, float, {lane} "
+                     f"with {cores} thread(s): {warm_s:.2f} s (substeps 0-{cold_steps} from t=0, which include "
+                     f"the first substep's start sample of every node: "
+                     f"{sc.bond_count * cold_steps / cold_s:.3g} bond-solves/s in {cold_s:.2f} s)",
+           "cold_value": sc.bond_count * cold_steps / cold_s, "cold_substeps": cold_steps,
+           "warm_substeps": warm_steps, "host_cpus": os.cpu_count(), "affinity_cpus": avail,
+           "thread_rule": "every core of the process's affinity, within OMP_NUM_THREADS (the per-GPU CPU share)",
+           "cpu_model": cpu_model()}
     if kind == "reference" and not args.cpu_quick:
         out["more"] = cpu_more(lib, sc, threads)
     return out
@@ -320,6 +350,25 @@ def cpu_more(lib, c2, threads):
                 "(the reference has none), first 100 substeps (serial: 3); float, bond-solves/s; the first "
                 "substep after upload samples every node, so short serial windows read low",
     }
+
+
+def precision_leg(args):
+    """Float against double on the device (the double lane is the reference's
+    FP64 answer): config 1 in full, the impact before and into the regime where
+    its block's candidate list overflows, and the cut over its first substeps."""
+    from banjo_amd import precision, scenes
+    from banjo_amd.lattice import product
+
+    hip = product()
+    out = {"eps": precision.EPS,
+           "note": "max |du|, |dv| of the float lane against the double lane; bonds alive in one lane and broken "
+                   "in the other, and how many lie within eps (relative) of their threshold in the lane where "
+                   "they broke; the reference's bar is max |du| < 1e-6 m (tests/fast_lattice_tests.cpp:435-447)"}
+    out["c1_full"] = precision.report(hip, scenes.config1())
+    out["impact_first_9"] = precision.report(hip, scenes.config2(substeps=9))
+    out["impact_first_30"] = precision.report(hip, scenes.config2(substeps=30))
+    out["cut_first_72"] = precision.report(hip, scenes.config5(substeps=72))
+    return out
 
 
 def copy_probe(nbytes, repeats):
@@ -459,12 +508,14 @@ def main():
     ap.add_argument("--blocks", type=int, default=1)
     ap.add_argument("--profile-substeps", type=int, default=1200)
     ap.add_argument("--cpu-substeps", type=int, default=10)
+    ap.add_argument("--cpu-warm-substeps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-quick", action="store_true", help="only the config-2 window of the CPU baseline")
-    ap.add_argument("--forms", default="plain,batched,served,cut,contact,c1",
+    ap.add_argument("--forms", default="plain,batched,served,cut,contact,c1,precision",
                     help="comma list of plain (the headline impact), batched (many uneven lattices), "
                          "served (a stream of trials), cut (a plank cut by a blade, labels every step), "
-                         "contact (the cut with node-node contact on), c1 (the 16^3 cantilever in full)")
+                         "contact (the cut with node-node contact on), c1 (the 16^3 cantilever in full), "
+                         "precision (float against double on the device)")
     ap.add_argument("--cut-steps", type=int, default=600)
     ap.add_argument("--contact-steps", type=int, default=120)
     ap.add_argument("--batch-trials", type=int, default=512)
@@ -501,6 +552,8 @@ def main():
             out["contact"] = contact_leg(args)
         if "c1" in forms:
             out["c1"] = c1_leg(args)
+        if "precision" in forms:
+            out["precision"] = precision_leg(args)
         print(json.dumps(out))
         return
     t_build = time.perf_counter()
@@ -557,6 +610,8 @@ def main():
         out["contact"] = contact_leg(args)
     if "c1" in forms:
         out["c1"] = c1_leg(args)
+    if "precision" in forms:
+        out["precision"] = precision_leg(args)
     print(json.dumps(out))
 
 

@@ -122,7 +122,7 @@ typedef struct blx_run_control {
     double removable_energy_j;
     uint64_t capture_stride;
     uint32_t max_frames;
-    uint64_t steps_per_launch;   /* substeps per device batch; 0 = library default */
+    uint64_t steps_per_launch;   /* substeps per device batch; 0 = the whole run in one batch */
     const volatile uint8_t *abandon; /* polled between batches; NULL never stops */
 } blx_run_control;
 
@@ -466,9 +466,12 @@ int blx_batch_slot_bytes(const blx_batch *batch, uint32_t slot, uint64_t *bytes)
 /* ---- measurement (no counterpart in the reference's API) ---------------- */
 
 /* Per-launch HIP-event timing of every kernel (off by default: it shortens
- * device batches to 4 substeps). With it on, run() also fills
- * blx_run_status.phase_seconds with the reference's phase ids
- * (FastLattice.cpp:22-26, RunStatus::phase_seconds at FastLattice.hpp:226-228). */
+ * device batches to 4 substeps). blx_run_status.phase_seconds, with the
+ * reference's phase ids (FastLattice.cpp:22-26, RunStatus::phase_seconds at
+ * FastLattice.hpp:226-228), is filled on every run: with this on, each phase
+ * gets its kernels' event time; with it off, each device batch times one of
+ * its substeps phase by phase and splits the batch's device time in those
+ * proportions. */
 int blx_backend_set_profiling(blx_backend *backend, int on);
 /* Accumulated event time and launch count of kernel kind `index`; *count is
  * the number of kinds (call with index 0 to learn it). */

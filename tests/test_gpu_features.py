@@ -152,3 +152,18 @@ def test_refusals(gpu):
     with pytest.raises(ValueError):
         be.set_external_forces(bad, 10)
     be.close()
+
+
+def test_phase_seconds_on_every_run(gpu):
+    # RunStatus::phase_seconds (FastLattice.hpp:225-228) without profiling: each
+    # device batch times one substep phase by phase and splits its device time
+    # in those proportions, so the phases sum to the run's kernel time.
+    sc = scenes.small_impact(nx=12, ny=8, nz=12, substeps=60)
+    print("phase seconds: 60 substeps in batches of 20", flush=True)
+    h = run_scene(gpu, sc, F, control_kw=dict(steps_per_launch=20))
+    st = h["status"]
+    ph = list(st.phase_seconds)
+    assert st.launches == 3 and st.kernel_seconds > 0.0
+    assert abs(sum(ph) - st.kernel_seconds) <= 1e-9 + 1e-6 * st.kernel_seconds
+    for p in (3, 4, 5, 8, 10, 11, 12, 13):  # kick/classify, striker passes, sweep, velocity, sample, fold
+        assert ph[p] > 0.0, p
