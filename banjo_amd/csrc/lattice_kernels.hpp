@@ -1060,28 +1060,20 @@ __global__ __launch_bounds__(PASS == 1 ? kFinishThreads : kPass2Threads) void k_
 // XPBD bond projection, one colour stage (LatticePhysics.hpp:1132-1168). The
 // stage's bonds share no node, so every thread owns its two ends.
 
+// The arithmetic of one bond solve (LatticePhysics.hpp:1133-1168) on node
+// values held in registers: ua and ub move when the bond solves. lam is the
+// multiplier it reads (0 in the first iteration); a solve stores lam + dl in
+// lam_out and returns true, a bond shorter than 1e-12 returns false.
 template <class R, bool DIRECT, bool PLASTIC>
-__global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags) {
-    const WgMap *m = tileEntry(B);
-    if (!m) return;
-    const DevLattice<R> L = B.L[m->lat];
-    const u32 j = m->a + threadIdx.x;
-    if (j >= m->b) return;
-    const bool first = flags & kSolveFirst;
-    // The record is loaded with the alive flag and the run's stop flag, so the
-    // node gathers wait on one memory round trip, not two or three.
-    const BondRec<R> r = ldStream(&L.rec[j]);
-    const u8 live = ldStream(&L.alive[j]);
-    const R lam = first ? R(0) : ldg(&L.lambda[j]);
-    if (L.ctl->stop || !live) return;
-    const R4<R> ua = ldg(&L.u4[r.a]), ub = ldg(&L.u4[r.b]);
+__device__ __forceinline__ bool solveBond(const DevLattice<R> &L, const Step<R> &S, const BondRec<R> &r, u32 j, R lam,
+                                          R4<R> &ua, R4<R> &ub, R &lam_out) {
     const EdgePal<R> g = ldg(&L.epal[r.p & ~kNoDir]);
     const R4<R> ge = g.e;
     const V3<R> du = xyz(ub) - xyz(ua);
     const V3<R> e = xyz(ge);
     const V3<R> delta = DIRECT ? (xyz(ldg(&L.x04[r.b])) + xyz(ub)) - (xyz(ldg(&L.x04[r.a])) + xyz(ua)) : e + du;
     const R len = length(delta);
-    if (len <= R(1.0e-12)) return;
+    if (len <= R(1.0e-12)) return false;
     const R orig = ge.w;
     R C;
     if (PLASTIC) {
@@ -1096,11 +1088,34 @@ __global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags)
     const R wa = ua.w, wb = ub.w;
     const R alpha = r.c / (S.dt * S.dt);
     const R dl = (-C - alpha * lam) / (wa + wb + alpha);
+    lam_out = lam + dl;
+    ua = with(xyz(ua) - wa * dl * dir, ua.w);
+    ub = with(xyz(ub) + wb * dl * dir, ub.w);
+    return true;
+}
+
+template <class R, bool DIRECT, bool PLASTIC>
+__global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
+    const u32 j = m->a + threadIdx.x;
+    if (j >= m->b) return;
+    const bool first = flags & kSolveFirst;
+    // The record is loaded with the alive flag and the run's stop flag, so the
+    // node gathers wait on one memory round trip, not two or three.
+    const BondRec<R> r = ldStream(&L.rec[j]);
+    const u8 live = ldStream(&L.alive[j]);
+    const R lam = first ? R(0) : ldg(&L.lambda[j]);
+    if (L.ctl->stop || !live) return;
+    R4<R> ua = ldg(&L.u4[r.a]), ub = ldg(&L.u4[r.b]);
+    R lam_out;
+    if (!solveBond<R, DIRECT, PLASTIC>(L, S, r, j, lam, ua, ub, lam_out)) return;
     // The multiplier is read only by a later iteration, so a one-iteration run
     // never stores it.
-    if (flags & kSolveKeepLambda) stg(&L.lambda[j], lam + dl);
-    stStream(&L.u4[r.a], with(xyz(ua) - wa * dl * dir, ua.w));
-    stStream(&L.u4[r.b], with(xyz(ub) + wb * dl * dir, ub.w));
+    if (flags & kSolveKeepLambda) stg(&L.lambda[j], lam_out);
+    stStream(&L.u4[r.a], ua);
+    stStream(&L.u4[r.b], ub);
 }
 
 // Radial pair damping, one colour stage (LatticePhysics.hpp:1171-1184).
