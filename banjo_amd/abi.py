@@ -82,7 +82,8 @@ class RunControl(C.Structure):
                 ("no_failure_steps", u64), ("energy_flat_steps", u64),
                 ("energy_flat_fraction", f64), ("calm_steps", u64), ("calm_damage_margin", f64),
                 ("removable_energy_j", f64), ("capture_stride", u64), ("max_frames", u32),
-                ("steps_per_launch", u64), ("abandon", P(u8))]
+                ("steps_per_launch", u64), ("abandon", P(u8)),
+                ("abandon_poll", C.CFUNCTYPE(C.c_int, C.c_void_p)), ("abandon_context", C.c_void_p)]
 
 
 class ContactAccumulators(C.Structure):

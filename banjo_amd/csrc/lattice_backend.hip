@@ -1072,7 +1072,17 @@ public:
     // Advances every armed slot until it stops (its exit rule or its budget)
     // or `substeps` substeps have been enqueued; returns the number of slots
     // still running. Abandon is polled between device batches.
-    uint32_t advance(uint64_t substeps, uint64_t batch_len, const volatile uint8_t *abandon, double *kernel) {
+    // RunControl::abandon (FastLattice.hpp:122-127), read between device
+    // batches: a byte flag, or a caller's poll function (an adapter holding a
+    // std::atomic<bool> passes one that loads it).
+    struct Abandon {
+        const volatile uint8_t *flag = nullptr;
+        int (*poll)(const void *) = nullptr;
+        const void *context = nullptr;
+        bool operator()() const { return (flag && *flag) || (poll && poll(context) != 0); }
+    };
+
+    uint32_t advance(uint64_t substeps, uint64_t batch_len, const Abandon &abandon, double *kernel) {
         HIP_OK(hipSetDevice(device_));
         dropGraph();  // settings, maps or slots may have changed since the last call
         uint64_t done = 0;
@@ -1133,7 +1143,7 @@ public:
                 }
             }
             running = now;
-            if (abandon && *abandon) break;
+            if (abandon()) break;
         }
         if (kernel) *kernel = ksec;
         return running;
@@ -1152,6 +1162,7 @@ public:
         // rule) turns the rest of its batch into no-ops, and abandon is polled
         // between batches.
         const uint64_t batch = c.steps_per_launch ? c.steps_per_launch : std::max<uint64_t>(1, c.max_steps);
+        const Abandon abandon{c.abandon, c.abandon_poll, c.abandon_context};
         uint64_t done = 0;
         unsigned reason = 0;
         while (done < c.max_steps) {
@@ -1164,13 +1175,13 @@ public:
                 }
                 n = std::min(n, c.capture_stride - step % c.capture_stride);
             }
-            advance(n, batch, c.abandon, nullptr);
+            advance(n, batch, abandon, nullptr);
             done += sl.ctl_host.step - step;
             if (sl.ctl_host.stop && sl.ctl_host.exit_reason != kBudgetSpent) {
                 reason = sl.ctl_host.exit_reason;
                 break;
             }
-            if (c.abandon && *c.abandon && done < c.max_steps) {
+            if (done < c.max_steps && abandon()) {
                 reason = 7;
                 break;
             }
@@ -2319,7 +2330,8 @@ int blx_batch_advance(blx_batch *b, uint64_t substeps, uint64_t steps_per_launch
                       uint32_t *running) {
     return guard([&] {
         const uint32_t r = dispatch(H(b), [&](auto &e) {
-            return e.advance(substeps, steps_per_launch ? steps_per_launch : 240, abandon, nullptr);
+            return e.advance(substeps, steps_per_launch ? steps_per_launch : 240,
+                             typename std::decay_t<decltype(e)>::Abandon{abandon, nullptr, nullptr}, nullptr);
         });
         if (running) *running = r;
     });

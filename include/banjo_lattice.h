@@ -124,6 +124,12 @@ typedef struct blx_run_control {
     uint32_t max_frames;
     uint64_t steps_per_launch;   /* substeps per device batch; 0 = the whole run in one batch */
     const volatile uint8_t *abandon; /* polled between batches; NULL never stops */
+    /* RunControl::abandon as a callback, polled between batches when set:
+     * nonzero stops the run (exit reason 7). A caller holding the reference's
+     * std::atomic<bool> passes a function that loads it, with the atomic as
+     * the context. */
+    int (*abandon_poll)(const void *context);
+    const void *abandon_context;
 } blx_run_control;
 
 typedef struct blx_contact_accumulators {
