@@ -4,6 +4,8 @@ buildLatticeSchedule (LatticeSchedule.cpp:18-104) and buildLatticeState
 (FastLattice.cpp:47-169) fix the sweep order and the summation order of
 every later kernel, so they must agree array for array.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -81,3 +83,29 @@ def test_schedule_refuses_bad_slabs(hip, ref):
     for lib in (hip, ref):
         with pytest.raises(ValueError):
             lib.schedule_build(sc.asset, np.array([0, 16, 16, 32, 96], np.uint32))  # empty slab: bonds skip one
+
+
+def test_box26_pairs_by_brute_force():
+    # Every pair of distinct nodes whose grid coordinates differ by at most one
+    # on each axis, once, smaller index first, ordered by the smaller node (the
+    # node-major emission of Lattice.cpp:49-52), with rest length |offset| h:
+    # found by testing all pairs, so a missing, doubled or wrongly oriented
+    # half-offset fails.
+    import itertools
+    for dims in ((5, 3, 7), (4, 4, 4), (1, 3, 2), (2, 2, 2)):
+        nx, ny, nz = dims
+        pos, a, b, L = scenes.box26(nx, ny, nz, 0.01)
+        coords = [(i % nx, (i // nx) % ny, i // (nx * ny)) for i in range(nx * ny * nz)]
+        expect = []
+        for i, j in itertools.combinations(range(len(coords)), 2):
+            d = [abs(p - q) for p, q in zip(coords[i], coords[j])]
+            if max(d) == 1:
+                expect.append((i, j, 0.01 * math.sqrt(sum(x * x for x in d))))
+        got = list(zip(a.tolist(), b.tolist(), L.tolist()))
+        assert sorted((i, j) for i, j, _ in got) == [(i, j) for i, j, _ in expect]
+        assert len(set((i, j) for i, j, _ in got)) == len(got)
+        assert [i for i, _, _ in got] == sorted(i for i, _, _ in got)  # node-major
+        lengths = {(i, j): l for i, j, l in expect}
+        assert all(abs(l - lengths[(i, j)]) <= 1e-18 for i, j, l in got)
+        # The reference positions are the cell centres x + 1/2 - n/2 cells.
+        assert np.allclose(pos[:, 0], [(c[0] + 0.5 - 0.5 * nx) * 0.01 for c in coords], rtol=0, atol=1e-15)
