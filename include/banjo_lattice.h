@@ -362,6 +362,45 @@ int blx_backend_clear_frames(blx_backend *backend);
 int blx_backend_fragment_labels(blx_backend *backend, uint32_t *label,
                                 uint32_t *component_count);
 
+/* ---- fragment geometry for the rigid handoff (host) --------------------
+ * The geometric half of the reference's fragment build
+ * (buildFragmentRepresentations, fracture/FragmentGeometry.cpp:203-291); the
+ * mass half is blx_backend_fragment_mass_properties, over the canonical labels
+ * of blx_backend_fragment_labels. */
+
+/* buildExposedVoxelSurface (fracture/FragmentGeometry.cpp:145-199) of one
+ * fragment: for each of its `count` cells in the order given, each face of
+ * the cell (+x, -x, +y, -y, +z, -z) whose neighbouring cell is not in the
+ * fragment becomes four vertices at (position - center_of_mass) + voxel_size
+ * * corner with the face's normal, and two triangles. grid and position are
+ * the cells' grid coordinates and world positions (3 per cell). Writes up to
+ * face_capacity faces (12 doubles of vertex_position and of vertex_normal and
+ * 6 indices each) and sets *face_count to the total (capacity 0 to size). */
+int blx_exposed_voxel_surface(const int32_t *grid, const double *position, uint32_t count,
+                              const double center_of_mass[3], double voxel_size_m,
+                              double *vertex_position, double *vertex_normal, uint32_t *indices,
+                              uint64_t face_capacity, uint64_t *face_count);
+
+/* The collision proxy of a fragment (FragmentGeometry.cpp:54-131): its
+ * surface's support points in the 26 lattice directions, then an even sample
+ * of the vertices up to max(maximum_points, 8) distinct points, and the first
+ * cell's eight corners if fewer than four were found. vertex_position is
+ * 3 * vertex_count (the surface's vertices in order); first_voxel_center the
+ * first cell's centre relative to the centre of mass. Writes up to capacity
+ * points (3 doubles each) and sets *count (at most max(maximum_points, 8) + 8). */
+int blx_collision_points(const double *vertex_position, uint64_t vertex_count,
+                         const double first_voxel_center[3], double voxel_size_m,
+                         uint64_t maximum_points, double *points, uint64_t capacity,
+                         uint64_t *count);
+
+/* mergeCellsIntoBoxes (fracture/PieceGeometry.cpp:9-46): the cells (3 grid
+ * coordinates each) as inclusive boxes (min x, y, z, max x, y, z), greedily:
+ * the smallest free cell in (z, y, x) order grows along x, then by rows
+ * along y, then by slabs along z. Writes up to capacity boxes and sets
+ * *box_count. */
+int blx_merge_cells_into_boxes(const int32_t *cells, uint64_t count, int32_t *boxes, uint64_t capacity,
+                               uint64_t *box_count);
+
 /* blx_backend_fragment_labels without allocating. It has no counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * graph capture, tight serving loops, harnesses that forbid hipMalloc). The

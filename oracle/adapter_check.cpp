@@ -365,7 +365,11 @@ Result drive(LatticeBackend &be, const LatticeState &state0, const StepSettings<
     return r;
 }
 
-void compare(const Result &a, const Result &b, const std::string &tag) {
+bool close(double x, double y, double rtol) { return std::abs(x - y) <= rtol * std::max(std::abs(x), std::abs(y)); }
+
+// ledger_rtol: 0 for an exact external-load ledger; the device sums it in a
+// fixed tree order rather than node order and is held to 1e-12 relative.
+void compare(const Result &a, const Result &b, const std::string &tag, double ledger_rtol) {
     const LatticeState &x = a.state, &y = b.state;
     expect(sameVec(x.u, y.u) && sameVec(x.u_prev, y.u_prev) && sameVec(x.v, y.v), tag + ": u, u_prev, v");
     expect(sameVec(x.alive, y.alive) && sameVec(x.failure_mode, y.failure_mode) && sameVec(x.damage, y.damage),
@@ -387,8 +391,12 @@ void compare(const Result &a, const Result &b, const std::string &tag) {
                p.rank_deficient_nodes == q.rank_deficient_nodes,
            tag + ": run status counters");
     expect(sameBits(p.contact, q.contact), tag + ": striker contact ledger");
-    expect(p.external_load.steps == q.external_load.steps && p.external_load.work_j == q.external_load.work_j &&
-               sameBits(p.external_load.impulse_n_s, q.external_load.impulse_n_s),
+    const ExternalLoadLedger &la = p.external_load, &lb = q.external_load;
+    expect(la.steps == lb.steps && close(la.work_j, lb.work_j, ledger_rtol) &&
+               close(la.impulse_n_s.x, lb.impulse_n_s.x, ledger_rtol) &&
+               close(la.impulse_n_s.y, lb.impulse_n_s.y, ledger_rtol) &&
+               close(la.impulse_n_s.z, lb.impulse_n_s.z, ledger_rtol) &&
+               close(la.requested_impulse_n_s.y, lb.requested_impulse_n_s.y, ledger_rtol),
            tag + ": external load ledger");
     bool frames = a.frames.size() == b.frames.size();
     for (std::size_t k = 0; frames && k < a.frames.size(); ++k)
@@ -399,7 +407,7 @@ void compare(const Result &a, const Result &b, const std::string &tag) {
     expect(a.first == b.first, tag + ": first failure bonds");
 }
 
-void checkDriven(const Box &box, const LatticeSchedule &schedule, const char *tag, bool loads) {
+void checkDriven(const Box &box, const LatticeSchedule &schedule, const char *tag, bool loads, double ledger_rtol) {
     const double h = 0.01;
     double bottom = 1e9, top = -1e9;
     for (const Vec3 &p : box.matter.reference_positions_world_m) {
@@ -424,7 +432,7 @@ void checkDriven(const Box &box, const LatticeSchedule &schedule, const char *ta
         const Result b = drive(*cpu, state0, s, sp, loads);
         const std::string t = std::string(tag) + (prec == Precision::Float ? " float" : " double");
         expect(a.status.broken_bonds > 0 || loads, t + ": the run breaks bonds");
-        compare(a, b, t);
+        compare(a, b, t, ledger_rtol);
         std::printf("%s: %u broken, %llu contacts, %zu frames, identical to makeCpuLatticeBackend\n", t.c_str(),
                     a.status.broken_bonds, (unsigned long long)a.status.contact.impulse_contacts, a.frames.size());
     }
@@ -441,12 +449,15 @@ int main(int argc, char **argv) {
         checkNoDevice(one);
     } else if (mode == "product" || mode == "oracle") {
         std::printf("backend: %s\n", cudaLatticeDescription().c_str());
-        checkDriven(*box, one, "impact, one block", false);
+        // The oracle sums the load ledger in node order, as the reference does;
+        // the device in a fixed tree order.
+        const double ledger_rtol = mode == "oracle" ? 0.0 : 1e-12;
+        checkDriven(*box, one, "impact, one block", false, ledger_rtol);
         const std::uint32_t layer = 12 * 6;
         const std::vector<std::uint32_t> slabs{0, 4 * layer, 8 * layer, 12 * layer};
         const LatticeSchedule three = buildLatticeSchedule(box->asset, slabs);
-        checkDriven(*box, three, "impact, three slabs", false);
-        checkDriven(*box, one, "external loads", true);
+        checkDriven(*box, three, "impact, three slabs", false, ledger_rtol);
+        checkDriven(*box, one, "external loads", true, ledger_rtol);
     } else {
         std::printf("usage: adapter_check product|oracle\n");
         return 2;

@@ -12,6 +12,8 @@
 //   ref_components       -> findConnectedComponents (ConnectedComponents.cpp:51-87)
 //   ref_fragment_mass    -> calculateFragmentMassProperties
 //                           (FragmentMassProperties.cpp:7-80)
+//   ref_fragment_build   -> buildFragmentRepresentations (FragmentGeometry.cpp:203-291)
+//   ref_merge_cells_into_boxes -> mergeCellsIntoBoxes (PieceGeometry.cpp:9-46)
 //   ref_box_slabs        -> boxLatticeSlabs         (BoxLattice.cpp:175-195)
 //   ref_state_substep_limit, ref_state_elastic_energy, ref_state_kinetic_energy
 //                        -> latticeStateSubstepLimit / ElasticEnergy / KineticEnergy
@@ -22,7 +24,9 @@
 #include "fastlattice/FastLattice.hpp"
 #include "fastlattice/LatticeSchedule.hpp"
 #include "fracture/ConnectedComponents.hpp"
+#include "fracture/FragmentGeometry.hpp"
 #include "fracture/FragmentMassProperties.hpp"
+#include "fracture/PieceGeometry.hpp"
 #include "matter/BoxLattice.hpp"
 
 #include <atomic>
@@ -307,6 +311,108 @@ int ref_components(uint32_t N, uint32_t B, const uint32_t *ba, const uint32_t *b
         const auto comps = findConnectedComponents(m);
         for (const auto &c : comps) for (std::uint32_t n : c.node_indices) label[n] = c.id;
         *count = std::uint32_t(comps.size());
+    });
+}
+
+// buildFragmentRepresentations over the components of `label` (canonical ids
+// 0 .. components-1, each component's nodes ascending), on a matter of N cells
+// with grid coordinates, world positions, velocities, masses and spins (NULL:
+// none), the asset's voxel size and the material's density. settings:
+// first_body_id, maximum_rigid_fragments, minimum_nodes_per_rigid_fragment,
+// maximum_collision_points, friction, restitution, rolling_resistance.
+// The result is flattened into doubles (written up to capacity; *size is the
+// total):
+//   rigid count, debris count, total mass, rigid mass, debris mass,
+//   total linear momentum (3), total angular momentum about the origin (3),
+//   coarsening kinetic loss;
+//   per rigid fragment: body id, source node count, mass, centre of mass (3),
+//   face count, vertex count, index count, collision point count, voxel centre
+//   count, vertices (position 3, normal 3 each), indices, points (3 each),
+//   voxel centres (3 each), friction, restitution, rolling resistance, voxel size;
+//   per debris particle: position (3), velocity (3), angular velocity (3),
+//   mass, radius, inertia (9).
+int ref_fragment_build(uint32_t N, const int32_t *grid, const double *pos, const double *vel, const double *mass,
+                       const double *spin, double cell, double density, const uint32_t *label, uint32_t components,
+                       const double *settings, double *out, uint64_t capacity, uint64_t *size) {
+    return guard([&] {
+        LatticeAsset asset;
+        asset.recipe.voxel_size_m = cell;
+        asset.nodes.resize(N);
+        ActiveMatter m;
+        m.asset = &asset;
+        m.material.density_kg_m3 = density;
+        m.nodes.resize(N);
+        for (std::uint32_t i = 0; i < N; ++i) {
+            asset.nodes[i].grid = {grid[3 * i], grid[3 * i + 1], grid[3 * i + 2]};
+            ActiveNodeState &n = m.nodes[i];
+            n.position_world_m = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+            n.velocity_m_s = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+            n.mass_kg = mass[i];
+            if (spin) n.spin_angular_velocity_rad_s = {spin[3 * i], spin[3 * i + 1], spin[3 * i + 2]};
+        }
+        std::vector<FragmentComponent> comps(components);
+        for (std::uint32_t f = 0; f < components; ++f) comps[f].id = f;
+        for (std::uint32_t i = 0; i < N; ++i) {
+            if (label[i] >= components) throw std::invalid_argument("label out of range");
+            comps[label[i]].node_indices.push_back(i);
+        }
+        FragmentBuildSettings fs;
+        fs.first_body_id = MatterBodyId(settings[0]);
+        fs.maximum_rigid_fragments = std::size_t(settings[1]);
+        fs.minimum_nodes_per_rigid_fragment = std::size_t(settings[2]);
+        fs.maximum_collision_points = std::size_t(settings[3]);
+        fs.friction = settings[4];
+        fs.restitution = settings[5];
+        fs.rolling_resistance = settings[6];
+        const FragmentBuildResult r = buildFragmentRepresentations(m, comps, fs);
+        std::vector<double> o;
+        auto v3 = [&](const Vec3 &v) { o.insert(o.end(), {v.x, v.y, v.z}); };
+        o.push_back(double(r.rigid_fragments.size()));
+        o.push_back(double(r.debris_particles.size()));
+        o.insert(o.end(), {r.total_mass_kg, r.rigid_fragment_mass_kg, r.debris_mass_kg});
+        v3(r.total_linear_momentum_kg_m_s);
+        v3(r.total_angular_momentum_about_origin_kg_m2_s);
+        o.push_back(r.coarsening_kinetic_loss_j);
+        for (const RigidFragmentDescription &f : r.rigid_fragments) {
+            o.insert(o.end(), {double(f.body_id), double(f.source_node_count), f.mass_properties.mass_kg});
+            v3(f.mass_properties.center_of_mass_world_m);
+            o.insert(o.end(), {double(f.surface_mesh.exposed_face_count), double(f.surface_mesh.vertices.size()),
+                               double(f.surface_mesh.indices.size()), double(f.collision_points_local_m.size()),
+                               double(f.voxel_centers_local_m.size())});
+            for (const SurfaceVertex &v : f.surface_mesh.vertices) {
+                v3(v.position_local_m);
+                v3(v.normal_local);
+            }
+            for (std::uint32_t k : f.surface_mesh.indices) o.push_back(double(k));
+            for (const Vec3 &p : f.collision_points_local_m) v3(p);
+            for (const Vec3 &p : f.voxel_centers_local_m) v3(p);
+            o.insert(o.end(), {f.friction, f.restitution, f.rolling_resistance, f.voxel_size_m});
+        }
+        for (const DebrisParticleDescription &d : r.debris_particles) {
+            v3(d.position_world_m);
+            v3(d.velocity_m_s);
+            v3(d.angular_velocity_rad_s);
+            o.insert(o.end(), {d.mass_kg, d.radius_m});
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) o.push_back(d.inertia_world_kg_m2.m[a][b]);
+        }
+        *size = o.size();
+        if (out) std::memcpy(out, o.data(), std::min<std::uint64_t>(capacity, o.size()) * sizeof(double));
+    });
+}
+
+int ref_merge_cells_into_boxes(const int32_t *cells, uint64_t count, int32_t *boxes, uint64_t capacity,
+                               uint64_t *box_count) {
+    return guard([&] {
+        std::vector<GridCoord> c(count);
+        for (uint64_t k = 0; k < count; ++k) c[k] = {cells[3 * k], cells[3 * k + 1], cells[3 * k + 2]};
+        const std::vector<VoxelBox> b = mergeCellsIntoBoxes(c);
+        *box_count = b.size();
+        for (uint64_t k = 0; k < std::min<uint64_t>(capacity, b.size()); ++k) {
+            const int32_t v[6] = {b[k].minimum.x, b[k].minimum.y, b[k].minimum.z, b[k].maximum.x, b[k].maximum.y,
+                                  b[k].maximum.z};
+            std::memcpy(boxes + 6 * k, v, sizeof v);
+        }
     });
 }
 
