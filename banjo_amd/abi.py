@@ -142,6 +142,22 @@ class RunStatus(C.Structure):
                 ("node_contact", NodeContactAccumulators)]
 
 
+class FragmentPose(C.Structure):
+    """FragmentPose, src/fastlattice/Refracture.hpp:262-268."""
+    _fields_ = [("center_of_mass", f64 * 3), ("orientation_wxyz", f64 * 4), ("linear_velocity", f64 * 3),
+                ("angular_velocity", f64 * 3)]
+
+
+class FragmentLatticeView(C.Structure):
+    """FragmentLattice, src/fastlattice/Refracture.hpp:241-260, as caller-owned arrays."""
+    _fields_ = [("node_count", u32), ("bond_count", u32), ("node_capacity", u32), ("bond_capacity", u32),
+                ("origin", f64 * 3), ("carried_spin", f64 * 3), ("parent_node", P(u32)), ("parent_bond", P(u32)),
+                ("reference_position", P(f64)), ("position", P(f64)), ("velocity", P(f64)), ("mass", P(f64)),
+                ("bond_a", P(u32)), ("bond_b", P(u32)), ("rest_length", P(f64)), ("compliance", P(f64)),
+                ("threshold", P(f64)), ("alive", P(u8)), ("damage", P(f64)), ("failure_mode", P(u8)),
+                ("plastic_extension", P(f64)), ("plastic_strain", P(f64))]
+
+
 class AssetView(C.Structure):
     """LatticeAsset bonds + ActiveMatter state, src/matter/Lattice.hpp:26-44,82-93."""
     _fields_ = [("node_count", u32), ("bond_count", u32), ("reference_position", P(f64)),

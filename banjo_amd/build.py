@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIBRARY = os.path.join(LIBDIR, "libbanjo_lattice.so")
 
-SOURCES = ["lattice_backend.hip", "lattice_host.cpp", "fragment_geometry.cpp"]
+SOURCES = ["lattice_backend.hip", "lattice_host.cpp", "fragment_geometry.cpp", "fragment_lattice.cpp"]
 HEADERS = ["lattice_device.hpp", "lattice_kernels.hpp", "lattice_errors.hpp"]
 
 FP_FLAGS = ["-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",

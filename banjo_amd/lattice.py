@@ -271,6 +271,51 @@ class LatticeLibrary:
             out.ctypes.data_as(C.POINTER(abi.FragmentMass))), "fragment_mass")
         return out
 
+    def fragment_lattice(self, asset: Asset, nodes, cell_offset, center_of_mass, orientation_wxyz=(1.0, 0.0, 0.0, 0.0),
+                         linear_velocity=(0.0, 0.0, 0.0), angular_velocity=(0.0, 0.0, 0.0), plastic_extension=None,
+                         plastic_strain=None):
+        """buildFragmentLattice (src/fastlattice/Refracture.cpp:267-393): the
+        sub-lattice of the parent's cells `nodes`, placed on the pose. Returns
+        (Asset of the sub-lattice, dict with parent_node, parent_bond, origin,
+        carried_spin, plastic_extension, plastic_strain)."""
+        av, _keep = asset.view()
+        nd = np.ascontiguousarray(nodes, np.uint32)
+        off = np.ascontiguousarray(cell_offset, np.float64).reshape(-1)
+        if len(off) != 3 * asset.node_count:
+            raise ValueError("cell_offset holds three components per parent node")
+        pose = abi.FragmentPose()
+        pose.center_of_mass[:] = center_of_mass
+        pose.orientation_wxyz[:] = orientation_wxyz
+        pose.linear_velocity[:] = linear_velocity
+        pose.angular_velocity[:] = angular_velocity
+        pe = None if plastic_extension is None else np.ascontiguousarray(plastic_extension, np.float64)
+        ps = None if plastic_strain is None else np.ascontiguousarray(plastic_strain, np.float64)
+        fn = self.fn("fragment_lattice")
+        out = abi.FragmentLatticeView()
+        args = (C.byref(av), _ptr(nd, C.c_uint32), C.c_uint32(len(nd)), _ptr(off, C.c_double), C.byref(pose),
+                None if pe is None else _ptr(pe, C.c_double), None if ps is None else _ptr(ps, C.c_double))
+        self.check(fn(*args, C.byref(out)), "fragment_lattice")
+        n, k = out.node_count, out.bond_count
+        arr = dict(parent_node=np.zeros(n, np.uint32), parent_bond=np.zeros(k, np.uint32),
+                   reference_position=np.zeros((n, 3)), position=np.zeros((n, 3)), velocity=np.zeros((n, 3)),
+                   mass=np.zeros(n), bond_a=np.zeros(k, np.uint32), bond_b=np.zeros(k, np.uint32),
+                   rest_length=np.zeros(k), compliance=np.zeros(k), threshold=np.zeros((k, 6)),
+                   alive=np.zeros(k, np.uint8), damage=np.zeros(k), failure_mode=np.zeros(k, np.uint8),
+                   plastic_extension=np.zeros(k), plastic_strain=np.zeros(k))
+        out.node_capacity, out.bond_capacity = n, k
+        ctype = {np.dtype(np.uint32): C.c_uint32, np.dtype(np.float64): C.c_double, np.dtype(np.uint8): C.c_uint8}
+        for name, a in arr.items():
+            setattr(out, name, _ptr(a, ctype[a.dtype]))
+        self.check(fn(*args, C.byref(out)), "fragment_lattice")
+        sub = Asset(reference_position=arr["reference_position"], mass=arr["mass"], bond_a=arr["bond_a"],
+                    bond_b=arr["bond_b"], rest_length=arr["rest_length"], compliance=arr["compliance"],
+                    threshold=arr["threshold"], position=arr["position"], velocity=arr["velocity"],
+                    alive=arr["alive"], damage=arr["damage"], failure_mode=arr["failure_mode"])
+        info = dict(parent_node=arr["parent_node"], parent_bond=arr["parent_bond"], origin=tuple(out.origin),
+                    carried_spin=tuple(out.carried_spin), plastic_extension=arr["plastic_extension"],
+                    plastic_strain=arr["plastic_strain"])
+        return sub, info
+
     def state_metrics(self, state: "State"):
         """(substep limit s, elastic energy J, kinetic energy J) of a state:
         latticeStateSubstepLimit / ElasticEnergy / KineticEnergy

@@ -240,6 +240,39 @@ typedef struct blx_state_view {
     double *plastic_extension, *plastic_strain;                       /* B */
 } blx_state_view;
 
+/* FragmentPose (fastlattice/Refracture.hpp:262-268): where a fragment that
+ * re-enters the lattice sits now. Cells go to center_of_mass + orientation *
+ * offset; velocities are linear_velocity + angular_velocity x r. */
+typedef struct blx_fragment_pose {
+    double center_of_mass[3];
+    double orientation_wxyz[4];
+    double linear_velocity[3];
+    double angular_velocity[3];
+} blx_fragment_pose;
+
+/* The sub-lattice of one fragment (FragmentLattice, Refracture.hpp:241-260) as
+ * the arrays of a blx_asset_view plus the maps back to the parent. The caller
+ * owns the arrays and states their capacities; the call sets the counts. */
+typedef struct blx_fragment_lattice_view {
+    uint32_t node_count, bond_count;        /* set by the call */
+    uint32_t node_capacity, bond_capacity;  /* of the arrays below */
+    double origin[3];                       /* the fragment's centre of mass now */
+    double carried_spin[3];                 /* the pose's angular velocity, carried as cell spin */
+    uint32_t *parent_node;                  /* n: local -> parent node (ascending) */
+    uint32_t *parent_bond;                  /* k: local -> parent bond (ascending) */
+    double *reference_position;             /* 3n world: the rest shape placed on the pose */
+    double *position;                       /* 3n world */
+    double *velocity;                       /* 3n */
+    double *mass;                           /* n */
+    uint32_t *bond_a, *bond_b;              /* k, local node indices */
+    double *rest_length, *compliance;       /* k */
+    double *threshold;                      /* 6k */
+    uint8_t *alive;                         /* k */
+    double *damage;                         /* k */
+    uint8_t *failure_mode;                  /* k */
+    double *plastic_extension, *plastic_strain; /* k */
+} blx_fragment_lattice_view;
+
 /* FragmentMassProperties (fracture/FragmentMassProperties.hpp:11-21) of one
  * fragment, with the fragment's node count and smallest node. */
 typedef struct blx_fragment_mass {
@@ -400,6 +433,22 @@ int blx_collision_points(const double *vertex_position, uint64_t vertex_count,
  * *box_count. */
 int blx_merge_cells_into_boxes(const int32_t *cells, uint64_t count, int32_t *boxes, uint64_t capacity,
                                uint64_t *box_count);
+
+/* buildFragmentLattice (fastlattice/Refracture.cpp:267-393): the sub-lattice
+ * of a fragment re-entering the lattice lane. Its cells are `nodes` of the
+ * parent (any order, sorted and made unique here), its bonds every parent
+ * bond with both ends among them (dead ones included), in parent order, with
+ * their state (parent alive / damage / failure_mode, NULL for alive, 0, none).
+ * The reference configuration is the parent's rest shape, turned by the
+ * rotation closest to the frozen shape's (cell_offset, parent-indexed, 3 per
+ * parent node) and placed on the pose; positions are the pose applied to the
+ * frozen offsets. Plastic state comes from the parent's per-bond arrays (NULL
+ * for zero). Call with capacities 0 to learn the counts; the sub-lattice's
+ * schedule is blx_schedule_build of the result as one block. */
+int blx_fragment_lattice(const blx_asset_view *parent, const uint32_t *nodes, uint32_t count,
+                         const double *cell_offset, const blx_fragment_pose *pose,
+                         const double *plastic_extension, const double *plastic_strain,
+                         blx_fragment_lattice_view *out);
 
 /* blx_backend_fragment_labels without allocating. It has no counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under

@@ -14,6 +14,7 @@
 //                           (FragmentMassProperties.cpp:7-80)
 //   ref_fragment_build   -> buildFragmentRepresentations (FragmentGeometry.cpp:203-291)
 //   ref_merge_cells_into_boxes -> mergeCellsIntoBoxes (PieceGeometry.cpp:9-46)
+//   ref_fragment_lattice -> buildFragmentLattice     (Refracture.cpp:267-393)
 //   ref_box_slabs        -> boxLatticeSlabs         (BoxLattice.cpp:175-195)
 //   ref_state_substep_limit, ref_state_elastic_energy, ref_state_kinetic_energy
 //                        -> latticeStateSubstepLimit / ElasticEnergy / KineticEnergy
@@ -23,6 +24,7 @@
 
 #include "fastlattice/FastLattice.hpp"
 #include "fastlattice/LatticeSchedule.hpp"
+#include "fastlattice/Refracture.hpp"
 #include "fracture/ConnectedComponents.hpp"
 #include "fracture/FragmentGeometry.hpp"
 #include "fracture/FragmentMassProperties.hpp"
@@ -398,6 +400,62 @@ int ref_fragment_build(uint32_t N, const int32_t *grid, const double *pos, const
         }
         *size = o.size();
         if (out) std::memcpy(out, o.data(), std::min<std::uint64_t>(capacity, o.size()) * sizeof(double));
+    });
+}
+
+// buildFragmentLattice on a parent matter built from the asset view (reference
+// positions as the asset's and the matter's reference positions, masses, bonds
+// and bond state); fills the same view the product's blx_fragment_lattice does.
+int ref_fragment_lattice(const blx_asset_view *a, const uint32_t *nodes, uint32_t count, const double *offset,
+                         const blx_fragment_pose *pose, const double *pext, const double *pstr,
+                         blx_fragment_lattice_view *out) {
+    return guard([&] {
+        const LatticeAsset asset = makeAsset(*a);
+        ActiveMatter m;
+        fillMatter(m, asset, *a);
+        std::vector<Vec3> off(a->node_count);
+        for (std::uint32_t i = 0; i < a->node_count; ++i) off[i] = {offset[3 * i], offset[3 * i + 1], offset[3 * i + 2]};
+        FragmentPose p;
+        p.center_of_mass_world_m = {pose->center_of_mass[0], pose->center_of_mass[1], pose->center_of_mass[2]};
+        p.orientation_world = {pose->orientation_wxyz[0], pose->orientation_wxyz[1], pose->orientation_wxyz[2],
+                               pose->orientation_wxyz[3]};
+        p.linear_velocity_m_s = {pose->linear_velocity[0], pose->linear_velocity[1], pose->linear_velocity[2]};
+        p.angular_velocity_rad_s = {pose->angular_velocity[0], pose->angular_velocity[1], pose->angular_velocity[2]};
+        const std::vector<double> ext = pext ? std::vector<double>(pext, pext + a->bond_count) : std::vector<double>{};
+        const std::vector<double> str = pstr ? std::vector<double>(pstr, pstr + a->bond_count) : std::vector<double>{};
+        const FragmentLattice f = buildFragmentLattice(m, std::span<const std::uint32_t>(nodes, count), off, p, ext, str);
+        const std::uint32_t n = std::uint32_t(f.parent_node.size()), k = std::uint32_t(f.parent_bond.size());
+        out->node_count = n;
+        out->bond_count = k;
+        if (out->node_capacity < n || out->bond_capacity < k) return;
+        out->origin[0] = f.origin.x; out->origin[1] = f.origin.y; out->origin[2] = f.origin.z;
+        out->carried_spin[0] = f.carried_spin_rad_s.x; out->carried_spin[1] = f.carried_spin_rad_s.y;
+        out->carried_spin[2] = f.carried_spin_rad_s.z;
+        for (std::uint32_t i = 0; i < n; ++i) {
+            out->parent_node[i] = f.parent_node[i];
+            const Vec3 r = f.matter.reference_positions_world_m[i], x = f.matter.nodes[i].position_world_m,
+                       v = f.matter.nodes[i].velocity_m_s;
+            out->reference_position[3 * i] = r.x; out->reference_position[3 * i + 1] = r.y; out->reference_position[3 * i + 2] = r.z;
+            out->position[3 * i] = x.x; out->position[3 * i + 1] = x.y; out->position[3 * i + 2] = x.z;
+            out->velocity[3 * i] = v.x; out->velocity[3 * i + 1] = v.y; out->velocity[3 * i + 2] = v.z;
+            out->mass[i] = f.matter.nodes[i].mass_kg;
+        }
+        for (std::uint32_t j = 0; j < k; ++j) {
+            const BondRest &b = f.asset->bonds[j];
+            out->parent_bond[j] = f.parent_bond[j];
+            out->bond_a[j] = b.node_a;
+            out->bond_b[j] = b.node_b;
+            out->rest_length[j] = b.rest_length_m;
+            out->compliance[j] = b.compliance;
+            const double t[6] = {b.damage_start_stretch, b.damage_end_stretch, b.compression_damage_start_strain,
+                                 b.compression_damage_end_strain, b.shear_damage_start_strain, b.shear_damage_end_strain};
+            std::memcpy(out->threshold + 6 * std::size_t(j), t, sizeof t);
+            out->alive[j] = f.matter.bonds[j].alive ? 1 : 0;
+            out->damage[j] = f.matter.bonds[j].damage;
+            out->failure_mode[j] = std::uint8_t(f.matter.bonds[j].failure_mode);
+            out->plastic_extension[j] = f.plastic_extension_m[j];
+            out->plastic_strain[j] = f.plastic_strain_m[j];
+        }
     });
 }
 
