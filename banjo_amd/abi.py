@@ -139,7 +139,25 @@ class RunStatus(C.Structure):
                 ("phase_seconds", f64 * PHASE_COUNT), ("max_tensile_stretch", f32),
                 ("max_compressive_strain", f32), ("max_shear_strain", f32),
                 ("rank_deficient_nodes", u32), ("plastic_work_j", f64), ("max_plastic_stretch", f32),
-                ("node_contact", NodeContactAccumulators)]
+                ("node_contact", NodeContactAccumulators), ("shared_memory_positions", u8),
+                ("shared_memory_bytes", u64)]
+
+
+class Tuning(C.Structure):
+    """blx_tuning: launch tuning of a backend or batch (same bits for every setting)."""
+    _fields_ = [("colours_per_launch", u32), ("lds_sweep", u32), ("lds_threads", u32), ("graph_substeps", u32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+TUNE_MAX = 32
+
+
+class AutotuneReport(C.Structure):
+    """blx_autotune_report: the candidates tried, their host seconds per substep, the one kept."""
+    _fields_ = [("chosen", Tuning), ("candidates", u32), ("tried", Tuning * TUNE_MAX),
+                ("seconds_per_substep", f64 * TUNE_MAX)]
 
 
 class FragmentPose(C.Structure):

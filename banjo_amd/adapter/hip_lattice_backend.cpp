@@ -260,6 +260,8 @@ void statusBack(const blx_run_status &in, RunStatus &out) {
     out.rank_deficient_nodes = in.rank_deficient_nodes;
     out.plastic_work_j = in.plastic_work_j;
     out.max_plastic_stretch = in.max_plastic_stretch;
+    out.shared_memory_positions = in.shared_memory_positions != 0;
+    out.shared_memory_bytes = std::size_t(in.shared_memory_bytes);
 }
 
 } // namespace hip_adapter

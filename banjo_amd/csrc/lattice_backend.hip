@@ -315,9 +315,17 @@ public:
     // is a list of bond ranges of at most one workgroup each.
     std::vector<std::vector<uint2>> stages;
     size_t interior_stages = 0;
+    // The fused sweep (k_sweep) of a one-iteration run: per launch group, its
+    // map entries (lat filled in when the maps are built) and its first stage.
+    std::vector<std::vector<WgMap>> sweep_groups;
+    // The stages as merged bond ranges for the one-workgroup sweep
+    // (k_sweep_lds): stage s is sweep_ranges[stage_end[s - 1] .. stage_end[s]).
+    std::vector<uint2> sweep_ranges;
+    std::vector<uint32_t> stage_end;
+    std::vector<uint32_t> sweep_first_stage;
     // Offsets of the device arrays in the arena.
     struct Off {
-        size_t x04, u4, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
+        size_t x04, u4, u4b, tasks, extras, stage_ranges, stage_end, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
         size_t nbr_key, spal, sdelta, nbr_bond, epal, nodir, rec, alive, lambda, pext;
         size_t bnd, alive_n, thr, thr2, pv, pyz, mode, pstr, slots;
         size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
@@ -356,6 +364,18 @@ public:
         interior_stages = stages.size();
         add(1, 0, 2);
         add(1, 1, 2);
+        sweep_ranges.clear();
+        stage_end.clear();
+        for (const auto &st : stages) {
+            for (const uint2 &r : st) {
+                if (!sweep_ranges.empty() && sweep_ranges.size() > (stage_end.empty() ? 0 : stage_end.back()) &&
+                    sweep_ranges.back().y == r.x)
+                    sweep_ranges.back().y = r.y;
+                else
+                    sweep_ranges.push_back(r);
+            }
+            stage_end.push_back(uint32_t(sweep_ranges.size()));
+        }
         scheduled = true;
         uploaded = false;
         armed = false;
@@ -375,6 +395,12 @@ public:
         l.group_words = group_words;
         l.tensile_only = tensile_only ? 1U : 0U;
         l.u4 = arena.at<R4<R>>(o.u4);
+        l.u4b = arena.at<R4<R>>(o.u4b);
+        l.stage_ranges = arena.at<uint2>(o.stage_ranges);
+        l.stage_end = arena.at<uint32_t>(o.stage_end);
+        l.stage_count = uint32_t(stage_end.size());
+        l.tasks = arena.at<uint4>(o.tasks);
+        l.extras = arena.at<uint32_t>(o.extras);
         l.up4 = arena.at<R4<R>>(o.up4);
         l.v4 = arena.at<R4<R>>(o.v4);
         l.x04 = arena.at<R4<R>>(o.x04);
@@ -462,6 +488,10 @@ public:
         sch = src.sch;
         stages = src.stages;
         interior_stages = src.interior_stages;
+        sweep_groups = src.sweep_groups;
+        sweep_first_stage = src.sweep_first_stage;
+        sweep_ranges = src.sweep_ranges;
+        stage_end = src.stage_end;
         scheduled = true;
         N = src.N;
         B = src.B;
@@ -495,6 +525,77 @@ public:
         status.first_failure_step = status.last_failure_step = status.last_energy_gain_step = ~0ULL;
         uploaded = true;
         armed = false;
+    }
+
+    // The fused sweep's launch groups, tasks and extra entries (k_sweep) from
+    // the bond ends in schedule order. The stages are grouped as one stage in
+    // place, then an even number of pairs, then the stages left over, so the
+    // pairs' ping-pong ends in u4.
+    void buildSweep(const uint32_t *end_a, const uint32_t *end_b, std::vector<uint4> &tasks,
+                    std::vector<uint32_t> &extras) {
+        sweep_groups.clear();
+        sweep_first_stage.clear();
+        tasks.clear();
+        extras.clear();
+        const size_t S = stages.size();
+        const size_t pairs = (S / 2) & ~size_t(1);
+        const size_t singles = S - 2 * pairs;
+        auto single = [&](size_t st) {
+            std::vector<WgMap> g;
+            for (const uint2 &r : stages[st]) g.push_back(WgMap{0, r.x, r.y, kSweepSingle << 16});
+            sweep_groups.push_back(std::move(g));
+            sweep_first_stage.push_back(uint32_t(st));
+        };
+        std::vector<uint32_t> at(N, kNone);   // node -> its stage-s bond
+        std::vector<uint8_t> next(N, 0);      // node is an end of a stage-(s + 1) bond
+        size_t st = 0;
+        if (singles > 0) single(st++);
+        for (size_t p = 0; p < pairs; ++p, st += 2) {
+            const uint32_t dir = (p & 1) ? kSweepReverse : 0U;
+            std::vector<WgMap> g;
+            for (const uint2 &r : stages[st])
+                for (uint32_t j = r.x; j < r.y; ++j) at[end_a[j]] = at[end_b[j]] = j;
+            for (const uint2 &r : stages[st + 1])
+                for (uint32_t j = r.x; j < r.y; ++j) next[end_a[j]] = next[end_b[j]] = 1;
+            auto far = [&](uint32_t v, uint32_t &far_node) -> uint32_t {
+                const uint32_t j = at[v];
+                if (j == kNone) {
+                    far_node = 0;
+                    return kNoTaskBond;
+                }
+                far_node = end_a[j] == v ? end_b[j] : end_a[j];
+                return j | (next[far_node] ? 0U : kTaskWrite);
+            };
+            for (const uint2 &r : stages[st + 1])
+                for (uint32_t j0 = r.x; j0 < r.y; j0 += kWG) {
+                    const uint32_t n = std::min<uint32_t>(kWG, r.y - j0);
+                    g.push_back(WgMap{0, uint32_t(tasks.size()), j0, n | (kSweepPair << 16) | dir});
+                    for (uint32_t j = j0; j < j0 + n; ++j) {
+                        uint4 t{};
+                        t.x = far(end_a[j], t.z);
+                        t.y = far(end_b[j], t.w);
+                        tasks.push_back(t);
+                    }
+                }
+            const size_t e0 = extras.size();
+            for (const uint2 &r : stages[st])
+                for (uint32_t j = r.x; j < r.y; ++j)
+                    if (!next[end_a[j]] && !next[end_b[j]]) extras.push_back(j);
+            for (uint32_t v = 0; v < N; ++v)
+                if (at[v] == kNone && !next[v]) extras.push_back(v | kExtraCopy);
+            for (size_t q = e0; q < extras.size(); q += kWG)
+                g.push_back(WgMap{0, uint32_t(q), 0, uint32_t(std::min<size_t>(kWG, extras.size() - q)) |
+                                                         (kSweepExtra << 16) | dir});
+            for (const uint2 &r : stages[st])
+                for (uint32_t j = r.x; j < r.y; ++j) at[end_a[j]] = at[end_b[j]] = kNone;
+            for (const uint2 &r : stages[st + 1])
+                for (uint32_t j = r.x; j < r.y; ++j) next[end_a[j]] = next[end_b[j]] = 0;
+            sweep_groups.push_back(std::move(g));
+            sweep_first_stage.push_back(uint32_t(st));
+        }
+        while (st < S) single(st++);
+        if (tasks.size() >= 0x80000000ULL || extras.size() >= 0x80000000ULL)
+            throw Invalid("the fused sweep holds at most 2^31 tasks");
     }
 
     // Builds every device array of this lattice on `stream` (and waits for
@@ -626,6 +727,10 @@ public:
         }
         tensile_only = tensile;
 
+        std::vector<uint4> sweep_tasks;
+        std::vector<uint32_t> sweep_extras;
+        buildSweep(st.bond_a, st.bond_b, sweep_tasks, sweep_extras);
+
         // Layout in the arena.
         arena.reset();
         const uint32_t np = edge_palette;
@@ -633,6 +738,11 @@ public:
         o = Off{};
         o.x04 = arena.reserve<R4<R>>(N);
         o.u4 = arena.reserve<R4<R>>(N);
+        o.u4b = arena.reserve<R4<R>>(N);
+        o.stage_ranges = arena.reserve<uint2>(sweep_ranges.size());
+        o.stage_end = arena.reserve<uint32_t>(stage_end.size());
+        o.tasks = arena.reserve<uint4>(sweep_tasks.size());
+        o.extras = arena.reserve<uint32_t>(sweep_extras.size());
         o.up4 = arena.reserve<R4<R>>(N);
         o.v4 = arena.reserve<R4<R>>(N);
         o.rinv = arena.reserve<R>(9 * size_t(N));
@@ -737,6 +847,10 @@ public:
         for (uint32_t i = 0; i < N; ++i) all[i] = i;
         putv(o.x04, x04);
         putv(o.u4, u4);
+        putv(o.tasks, sweep_tasks);
+        putv(o.stage_ranges, sweep_ranges);
+        putv(o.stage_end, stage_end);
+        putv(o.extras, sweep_extras);
         putv(o.up4, up4);
         putv(o.v4, v4);
         putv(o.nbr_key, key);
@@ -851,10 +965,10 @@ public:
             HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
         HIP_OK(hipEventCreate(&ev0_));
         HIP_OK(hipEventCreate(&ev1_));
-        if (const char *e = std::getenv("BLX_GRAPHS")) use_graphs_ = e[0] != '\0' && e[0] != '0';
         hipDeviceProp_t prop{};
         HIP_OK(hipGetDeviceProperties(&prop, device_));
         cus_ = uint32_t(prop.multiProcessorCount);
+        lds_limit_ = prop.sharedMemPerBlock;
         for (auto &s : slots_) s = std::make_unique<Slot<R>>();
         ls_.alloc(size_t(capacity) * sizeof(DevLattice<R>));
         ctls_.alloc(size_t(capacity) * sizeof(DevCtl));
@@ -1098,9 +1212,10 @@ public:
             const uint64_t n = std::min<uint64_t>(profilingBatch(batch_len), substeps - done);
             HIP_OK(hipEventRecord(ev0_, stream_));
             uint64_t q = 0;
-            if (use_graphs_ && S_.nc.mode == 0U && !prof_ && n >= kGraphSubsteps) {
-                if (!graph_exec_) captureSubsteps();
-                for (; q + kGraphSubsteps <= n; q += kGraphSubsteps) HIP_OK(hipGraphLaunch(graph_exec_, stream_));
+            const uint64_t gl = tuning_.graph_substeps;
+            if (gl != 0 && S_.nc.mode == 0U && !prof_ && n >= gl) {
+                if (!graph_exec_ || graph_len_ != gl) captureSubsteps(gl);
+                for (; q + gl <= n; q += gl) HIP_OK(hipGraphLaunch(graph_exec_, stream_));
             }
             const uint64_t q_sample = q;
             for (; q < n; ++q) {
@@ -1197,10 +1312,154 @@ public:
         out.exit_reason = reason;
     }
 
+    // Launch tuning. Every setting gives the same bits.
+    const blx_tuning &tuning() const { return tuning_; }
+    void setTuning(const blx_tuning &t) {
+        if (t.colours_per_launch != 1 && t.colours_per_launch != 2)
+            throw Invalid("colours_per_launch is 1 or 2");
+        if (t.lds_threads != 256 && t.lds_threads != 512 && t.lds_threads != 1024)
+            throw Invalid("lds_threads is 256, 512 or 1024");
+        if (t.graph_substeps > kMaxGraphSubsteps) throw Invalid("graph_substeps is at most 256");
+        tuning_ = t;
+        maps_dirty_ = true;
+        dropGraph();
+    }
+
+    // The autotuner: every candidate tuning advances the armed slots (after
+    // `prepare`, which may arm them) from the same state, once by warm
+    // substeps and once by warm + probe, on the host clock; the difference is
+    // the probe's time without the set-up both pay (a graph capture, a map
+    // build). Two rounds over the candidates, and the fastest is kept. The
+    // state of every slot, on the device and on the host, is put back as it
+    // was before, so tuning changes no result. Takes a device copy of every
+    // uploaded slot.
+    template <class Prepare> void autotune(uint64_t warm, uint64_t probe, Prepare prepare, blx_autotune_report &rep) {
+        if (probe == 0) throw Invalid("the autotuner needs at least one probe substep");
+        HIP_OK(hipSetDevice(device_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        HIP_OK(hipStreamSynchronize(side_));
+        HIP_OK(hipStreamSynchronize(early_));
+        struct Saved {
+            DevBuf arena;
+            DevCtl ctl;
+            Sphere<R> sphere;
+            blx_run_status status;
+            bool armed, pristine;
+        };
+        std::vector<std::unique_ptr<Saved>> saved(slots_.size());
+        size_t need = 0;
+        for (auto &sl : slots_)
+            if (sl->uploaded) need += sl->used;
+        size_t free_bytes = 0, total_bytes = 0;
+        HIP_OK(hipMemGetInfo(&free_bytes, &total_bytes));
+        if (need + (size_t(64) << 20) > free_bytes)
+            throw Runtime("the autotuner's copy of the lattices (" + std::to_string(need >> 20) +
+                          " MiB) does not fit in free device memory");
+        for (uint32_t k = 0; k < slots_.size(); ++k) {
+            Slot<R> &sl = *slots_[k];
+            if (!sl.uploaded) continue;
+            auto sv = std::make_unique<Saved>();
+            sv->arena.alloc(sl.used);
+            HIP_OK(hipMemcpyAsync(sv->arena.p, sl.arena.template at<char>(0), sl.used, hipMemcpyDeviceToDevice, stream_));
+            sv->ctl = readCtl(k);
+            sv->sphere = readSphere(k);
+            sv->status = sl.status;
+            sv->armed = sl.armed;
+            sv->pristine = sl.pristine;
+            saved[k] = std::move(sv);
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+        auto restore = [&] {
+            HIP_OK(hipStreamSynchronize(stream_));
+            for (uint32_t k = 0; k < slots_.size(); ++k) {
+                if (!saved[k]) continue;
+                Slot<R> &sl = *slots_[k];
+                const Saved &sv = *saved[k];
+                HIP_OK(hipMemcpyAsync(sl.arena.template at<char>(0), sv.arena.p, sl.used, hipMemcpyDeviceToDevice,
+                                      stream_));
+                writeCtl(k, sv.ctl);
+                HIP_OK(hipMemcpy(spheres_.as<Sphere<R>>() + k, &sv.sphere, sizeof(Sphere<R>), hipMemcpyHostToDevice));
+                sl.ctl_host = sv.ctl;
+                sl.status = sv.status;
+                sl.armed = sv.armed;
+                sl.pristine = sv.pristine;
+            }
+            HIP_OK(hipStreamSynchronize(stream_));
+            maps_dirty_ = true;
+            dropGraph();
+        };
+        // The candidates: colour stages per launch, the LDS sweep and its
+        // workgroup size where a lattice fits, and substeps per graph launch.
+        const blx_tuning before = tuning_;
+        bool any_fits = false, all_fit = true, any_run = false;
+        prepare();
+        for (auto &sl : slots_) {
+            if (!sl->armed) continue;
+            any_run = true;
+            blx_tuning t = tuning_;
+            t.lds_sweep = 1U;
+            std::swap(t, tuning_);
+            const bool fits = ldsSweeps(*sl);
+            std::swap(t, tuning_);
+            any_fits |= fits;
+            all_fit &= fits;
+        }
+        restore();
+        if (!any_run) throw Logic("the autotuner needs an armed lattice");
+        std::vector<blx_tuning> cand;
+        const uint32_t graphs[] = {0U, 8U, 32U};
+        for (uint32_t g : graphs) {
+            if (g != 0 && (g > warm + probe || S_.nc.mode != 0U)) continue;
+            for (uint32_t c = 1; c <= 2; ++c) cand.push_back(blx_tuning{c, 0U, 1024U, g});
+            if (any_fits)
+                for (uint32_t th : {256U, 512U, 1024U})
+                    for (uint32_t c = 1; c <= (all_fit ? 1U : 2U); ++c) cand.push_back(blx_tuning{c, 1U, th, g});
+        }
+        if (cand.size() > BLX_TUNE_MAX) throw Logic("too many tuning candidates");
+        std::vector<double> best(cand.size(), std::numeric_limits<double>::infinity());
+        const Abandon none{};
+        auto timed = [&](uint64_t n) {
+            prepare();
+            const auto t0 = std::chrono::steady_clock::now();
+            advance(n, n, none, nullptr);
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            restore();
+            return sec;
+        };
+        try {
+            for (int round = 0; round < 2; ++round)
+                for (size_t i = 0; i < cand.size(); ++i) {
+                    setTuning(cand[i]);
+                    const double a = warm ? timed(warm) : 0.0;
+                    const double b = timed(warm + probe);
+                    best[i] = std::min(best[i], std::max(0.0, b - a) / double(probe));
+                }
+        } catch (...) {
+            restore();
+            tuning_ = before;
+            throw;
+        }
+        size_t pick = 0;
+        for (size_t i = 1; i < cand.size(); ++i)
+            if (best[i] < best[pick]) pick = i;
+        setTuning(cand[pick]);
+        rep = blx_autotune_report{};
+        rep.chosen = cand[pick];
+        rep.candidates = uint32_t(cand.size());
+        for (size_t i = 0; i < cand.size(); ++i) {
+            rep.tried[i] = cand[i];
+            rep.seconds_per_substep[i] = best[i];
+        }
+    }
+
     void status(uint32_t k, blx_run_status &o) const {
         const Slot<R> &sl = slot(k);
         const DevCtl &c = sl.ctl_host;
         o = sl.status;
+        // RunStatus::shared_memory_* (FastLattice.hpp:229-231): whether this
+        // lattice's sweep keeps its nodes in LDS, and how many bytes.
+        o.shared_memory_positions = ldsSweeps(sl) ? 1U : 0U;
+        o.shared_memory_bytes = o.shared_memory_positions ? uint64_t(sl.N) * sizeof(R4<R>) : 0U;
         o.total_steps = c.step;
         o.first_failure_step = c.first_failure_step;
         o.last_failure_step = c.last_failure_step;
@@ -1560,6 +1819,13 @@ private:
     MapRef node_tiles_map_, bond_tiles_map_, list_map_, small_map_, mark_map_, lattice_map_;
     std::vector<MapRef> stage_maps_;
     size_t interior_stages_ = 0;
+    std::vector<MapRef> sweep_maps_;     // the fused sweep's launch groups
+    std::vector<uint32_t> sweep_phase_;  // reference phase id of each group (5 interior, 6 boundary)
+    // Launch tuning (blx_tuning). Every setting gives the same bits.
+    blx_tuning tuning_{1U, 1U, 1024U, 0U};
+    MapRef lds_map_;                     // one entry per lattice that sweeps in LDS (k_sweep_lds)
+    uint32_t lds_bytes_ = 0;             // that launch's dynamic LDS: the largest such lattice's nodes
+    size_t lds_limit_ = 0;               // LDS one workgroup may hold on this device
     bool any_loads_ = false;
 
     uint32_t gridFor(uint32_t n) const {
@@ -1637,27 +1903,70 @@ private:
         begin(lattice_map_);
         for (uint32_t k : run) all.push_back(WgMap{k, 0, 0, 0});
         end(lattice_map_);
+        // A one-iteration run sweeps every lattice whose nodes fit in one
+        // workgroup's LDS in one launch (k_sweep_lds, one workgroup each); the
+        // rest sweep by stages, or by stage pairs.
+        begin(lds_map_);
+        lds_bytes_ = 0;
+        std::vector<uint32_t> staged;
+        for (uint32_t k : run) {
+            const size_t bytes = size_t(slots_[k]->N) * sizeof(R4<R>);
+            if (ldsSweeps(*slots_[k])) {
+                all.push_back(WgMap{k, 0, 0, 0});
+                lds_bytes_ = std::max(lds_bytes_, uint32_t(bytes));
+            } else {
+                staged.push_back(k);
+            }
+        }
+        end(lds_map_);
         // Stage s of the batch is stage s of every running slot: lattices are
         // independent, so only each slot's own stage order matters.
         size_t stages = 0;
         interior_stages_ = 0;
-        for (uint32_t k : run) {
+        for (uint32_t k : staged) {
             stages = std::max(stages, slots_[k]->stages.size());
             interior_stages_ = std::max(interior_stages_, slots_[k]->interior_stages);
         }
         stage_maps_.assign(stages, MapRef{});
         for (size_t s = 0; s < stages; ++s) {
             begin(stage_maps_[s]);
-            for (uint32_t k : run) {
+            for (uint32_t k : staged) {
                 const Slot<R> &sl = *slots_[k];
                 if (s < sl.stages.size())
                     for (const uint2 &r : sl.stages[s]) all.push_back(WgMap{k, r.x, r.y, 0});
             }
             end(stage_maps_[s]);
         }
+        // The fused sweep's launch group g of the batch is group g of every
+        // running slot (each slot's groups keep its own stage order and its
+        // own even number of ping-pong pairs).
+        size_t groups = 0;
+        for (uint32_t k : staged) groups = std::max(groups, slots_[k]->sweep_groups.size());
+        sweep_maps_.assign(groups, MapRef{});
+        sweep_phase_.assign(groups, 5U);
+        for (size_t g = 0; g < groups; ++g) {
+            begin(sweep_maps_[g]);
+            for (uint32_t k : staged) {
+                const Slot<R> &sl = *slots_[k];
+                if (g >= sl.sweep_groups.size()) continue;
+                if (sl.sweep_first_stage[g] >= sl.interior_stages) sweep_phase_[g] = 6U;
+                for (WgMap e : sl.sweep_groups[g]) {
+                    e.lat = k;
+                    all.push_back(e);
+                }
+            }
+            end(sweep_maps_[g]);
+        }
         maps_.alloc(std::max<size_t>(1, all.size()) * sizeof(WgMap));
         if (!all.empty()) HIP_OK(hipMemcpy(maps_.p, all.data(), all.size() * sizeof(WgMap), hipMemcpyHostToDevice));
         maps_dirty_ = false;
+    }
+
+    // Whether slot sl sweeps in LDS (k_sweep_lds): on request, in a
+    // one-iteration run, with its nodes within one workgroup's LDS.
+    bool ldsSweeps(const Slot<R> &sl) const {
+        return tuning_.lds_sweep != 0 && shared_.iterations == 1 && sl.N > 0 &&
+               size_t(sl.N) * sizeof(R4<R>) <= lds_limit_;
     }
 
     Batch<R> batch(const MapRef &r) const {
@@ -1797,30 +2106,29 @@ private:
     }
     uint64_t profilingBatch(uint64_t batch) const { return prof_ ? std::min<uint64_t>(batch, 4) : batch; }
 
-    // Substep graphs, on request (BLX_GRAPHS=1 in the environment). Every
-    // launch argument of a substep is constant (the run state lives on the
-    // device), so kGraphSubsteps substeps are captured once per advance call
+    // Substep graphs (blx_tuning::graph_substeps). Every launch argument of a
+    // substep is constant (the run state lives on the device), so a block of
+    // substeps is captured once per advance call (and after every map rebuild)
     // and replayed; the side streams fork from stream_ and join back into it
     // through the substep's own events, which become graph edges. Nothing in a
-    // substep records a memset. Replays measured slower than enqueuing every
-    // substep on every workload here (the host stays ahead of the device), so
-    // enqueuing is the default.
-    static constexpr uint64_t kGraphSubsteps = 8;
-    bool use_graphs_ = false;
+    // substep records a memset. Replays pay where the host, not the device,
+    // sets the pace: small lattices with short kernels.
+    static constexpr uint32_t kMaxGraphSubsteps = 256;
     bool capturing_ = false;
+    uint64_t graph_len_ = 0;  // substeps in graph_exec_
     hipGraphExec_t graph_exec_ = nullptr;
 
     void dropGraph() {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
     }
-    void captureSubsteps() {
+    void captureSubsteps(uint64_t len) {
         dropGraph();
         HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
         capturing_ = true;
         hipError_t err = hipSuccess;
         try {
-            for (uint64_t q = 0; q < kGraphSubsteps; ++q) {
+            for (uint64_t q = 0; q < len; ++q) {
                 enqueueSubstepA();
                 enqueueSubstepB();
             }
@@ -1848,6 +2156,7 @@ private:
             inst = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
         if (g) (void)hipGraphDestroy(g);
         if (inst != hipSuccess || left) graph_exec_ = nullptr;
+        graph_len_ = graph_exec_ ? len : 0;
         HIP_OK(err);
         HIP_OK(end);
         if (left) throw Runtime("a side stream was left unjoined in the substep capture");
@@ -1858,17 +2167,18 @@ private:
 // event rides on the launch as its stop event: a separate record puts a marker
 // between two kernels of the stream, which costs about 5 us. While a substep
 // graph is being captured the record is a graph edge instead.
-#define BLX_SIGNAL(kernel, grid, block, stream, ev, ...)                                         \
-    do {                                                                                         \
-        hipEvent_t ev_ = (ev);                                                                   \
-        if (capturing_) {                                                                        \
-            hipLaunchKernelGGL(kernel, grid, block, 0, stream, __VA_ARGS__);                     \
-            HIP_OK(hipGetLastError());                                                           \
-            if (ev_) HIP_OK(hipEventRecord(ev_, stream));                                        \
-        } else {                                                                                 \
-            hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, nullptr, ev_, 0, __VA_ARGS__); \
-        }                                                                                        \
+#define BLX_SIGNAL_LDS(kernel, grid, block, lds, stream, ev, ...)                                  \
+    do {                                                                                           \
+        hipEvent_t ev_ = (ev);                                                                     \
+        if (capturing_) {                                                                          \
+            hipLaunchKernelGGL(kernel, grid, block, lds, stream, __VA_ARGS__);                     \
+            HIP_OK(hipGetLastError());                                                             \
+            if (ev_) HIP_OK(hipEventRecord(ev_, stream));                                          \
+        } else {                                                                                   \
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, nullptr, ev_, 0, __VA_ARGS__); \
+        }                                                                                          \
     } while (0)
+#define BLX_SIGNAL(kernel, grid, block, stream, ev, ...) BLX_SIGNAL_LDS(kernel, grid, block, 0, stream, ev, __VA_ARGS__)
 
 #define BLX_LAUNCH(kernel, phase, ...) \
     do {                                \
@@ -1934,7 +2244,36 @@ private:
             profEnd(early_);
         }
         const uint32_t iterations = shared_.iterations;
-        for (uint32_t it = 0; it < iterations; ++it) {
+        const bool pairs = iterations == 1 && tuning_.colours_per_launch == 2;
+        if (lds_map_.count > 0) {
+            // Every lattice that fits sweeps in LDS, one workgroup each.
+            const bool last = own && (pairs ? sweep_maps_.empty() : stage_maps_.empty());
+            const Batch<R> lb = batch(lds_map_);
+            const dim3 g(lds_map_.count), t(tuning_.lds_threads);
+            profBegin(KSolve, 5);
+            if (last)
+                BLX_SIGNAL_LDS((k_sweep_lds<R, DIRECT, false>), g, t, lds_bytes_, stream_, ev_sweep_, lb, S);
+            else if (plastic)
+                hipLaunchKernelGGL((k_sweep_lds<R, DIRECT, true>), g, t, lds_bytes_, stream_, lb, S);
+            else
+                hipLaunchKernelGGL((k_sweep_lds<R, DIRECT, false>), g, t, lds_bytes_, stream_, lb, S);
+            profEnd();
+        }
+        if (pairs) {
+            // The fused sweep: stage pairs in one launch each (k_sweep).
+            for (size_t g = 0; g < sweep_maps_.size(); ++g) {
+                const MapRef &m = sweep_maps_[g];
+                profBegin(KSolve, sweep_phase_[g]);
+                if (own && g + 1 == sweep_maps_.size())
+                    BLX_SIGNAL((k_sweep<R, DIRECT, false>), tiles(m), dim3(kWG), stream_, ev_sweep_, batch(m), S);
+                else if (plastic)
+                    hipLaunchKernelGGL((k_sweep<R, DIRECT, true>), tiles(m), dim3(kWG), 0, stream_, batch(m), S);
+                else
+                    hipLaunchKernelGGL((k_sweep<R, DIRECT, false>), tiles(m), dim3(kWG), 0, stream_, batch(m), S);
+                profEnd();
+            }
+        }
+        for (uint32_t it = 0; it < (pairs ? 0U : iterations); ++it) {
             // With more than one iteration every solve stores its multiplier, as
             // the reference does (LatticePhysics.hpp:1163): a bond that skips
             // iteration 0 (a zero-length bond) reads the previous substep's
@@ -2183,8 +2522,9 @@ int blx_description(char *out, size_t cap) {
             hipDeviceProp_t p{};
             HIP_OK(hipGetDeviceProperties(&p, d));
             char line[512];
-            std::snprintf(line, sizeof line, "%sdevice %d: %s (%s), %d CUs, %.1f GiB", d ? "; " : "", d, p.name,
-                          p.gcnArchName, p.multiProcessorCount, double(p.totalGlobalMem) / (1024.0 * 1024.0 * 1024.0));
+            std::snprintf(line, sizeof line, "%sdevice %d: %s (%s), %d CUs, %.1f GiB, %zu KiB LDS per workgroup",
+                          d ? "; " : "", d, p.name, p.gcnArchName, p.multiProcessorCount,
+                          double(p.totalGlobalMem) / (1024.0 * 1024.0 * 1024.0), size_t(p.sharedMemPerBlock) >> 10);
             s += line;
         }
         copyName(s, out, cap);
@@ -2272,6 +2612,46 @@ int blx_backend_fragment_mass_properties(blx_backend *b, double cell_size_m, con
 }
 int blx_backend_set_profiling(blx_backend *b, int on) {
     return guard([&] { dispatch(H(b), [&](auto &e) { e.setProfiling(on != 0); }); });
+}
+int blx_backend_get_tuning(const blx_backend *b, blx_tuning *t) {
+    return guard([&] {
+        if (!t) throw Invalid("null argument");
+        *t = dispatch(CH(b), [&](auto &e) { return e.tuning(); });
+    });
+}
+int blx_backend_set_tuning(blx_backend *b, const blx_tuning *t) {
+    return guard([&] {
+        if (!t) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) { e.setTuning(*t); });
+    });
+}
+int blx_backend_autotune(blx_backend *b, uint64_t warm, uint64_t probe, blx_autotune_report *rep) {
+    return guard([&] {
+        if (!rep) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) {
+            blx_run_control c{};
+            c.max_steps = warm + probe;
+            e.autotune(warm, probe, [&] { e.arm(0, c); }, *rep);
+        });
+    });
+}
+int blx_batch_get_tuning(const blx_batch *b, blx_tuning *t) {
+    return guard([&] {
+        if (!t) throw Invalid("null argument");
+        *t = dispatch(CH(b), [&](auto &e) { return e.tuning(); });
+    });
+}
+int blx_batch_set_tuning(blx_batch *b, const blx_tuning *t) {
+    return guard([&] {
+        if (!t) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) { e.setTuning(*t); });
+    });
+}
+int blx_batch_autotune(blx_batch *b, uint64_t warm, uint64_t probe, blx_autotune_report *rep) {
+    return guard([&] {
+        if (!rep) throw Invalid("null argument");
+        dispatch(H(b), [&](auto &e) { e.autotune(warm, probe, [] {}, *rep); });
+    });
 }
 int blx_backend_kernel_stats(const blx_backend *b, uint32_t index, char *name, size_t cap, double *seconds,
                              uint64_t *launches, uint32_t *count) {

@@ -106,6 +106,12 @@ template <class R> struct DevLattice {
     u32 N, B, D, blocks, node_tiles, bond_tiles, group_words;
     u32 tensile_only; // every bond's compression and shear ramps are off: thr2 replaces thr
     R4<R> *u4;        // displacement, w = inv_mass
+    R4<R> *u4b;       // the fused sweep's second node buffer
+    const uint4 *tasks;  // fused sweep: per stage-(s + 1) bond, its stage-s bonds and their far ends
+    const u32 *extras;   // fused sweep: leftover stage-s bonds and untouched nodes (| kExtraCopy)
+    const uint2 *stage_ranges;  // the colour stages as bond ranges [x, y), in sweep order
+    const u32 *stage_end;       // stage s is stage_ranges[stage_end[s - 1] .. stage_end[s])
+    u32 stage_count;
     R4<R> *up4;       // displacement at substep start
     R4<R> *v4;        // velocity
     const R4<R> *x04; // reference position, w = mass
@@ -1116,6 +1122,145 @@ __global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags)
     if (flags & kSolveKeepLambda) stg(&L.lambda[j], lam_out);
     stStream(&L.u4[r.a], ua);
     stStream(&L.u4[r.b], ub);
+}
+
+// The fused colour sweep of a one-iteration run: two consecutive colour
+// stages s, s + 1 in one launch. Each task owns one bond of stage s + 1 and
+// first solves the stage-s bonds at its two ends (at most one each: a stage
+// shares no node), then its own bond, all on register copies of the nodes.
+// A stage-s bond between the ends of two tasks is solved by both, from the
+// same inputs, so both get the bits the stage-by-stage sweep gives. Tasks read
+// one node buffer and write the other (u4 <-> u4b), so no task reads a node
+// another task has already moved. Every node is written once: the ends of the
+// stage-(s + 1) bonds by their task; the far end of a stage-s bond whose far
+// end is in no stage-(s + 1) bond by the task at its near end (kTaskWrite);
+// the ends of a stage-s bond with neither end in stage s + 1, and the nodes in
+// neither stage, by extra entries. Pairs alternate direction, and a sweep has
+// an even number of them, so it ends in u4; a stage left over from the pairing
+// runs in place on u4 like k_solve. Map entries (one per workgroup):
+//   single: a, b = bond range of one stage, in place on u4;
+//   pair:   a = first task, b = schedule index of its stage-(s + 1) bond;
+//   extra:  a = first entry of extras.
+// pad = count (bits 0-15) | kind (16-19) | direction (bit 20: u4b -> u4).
+constexpr u32 kSweepSingle = 0U, kSweepPair = 1U, kSweepExtra = 2U;
+constexpr u32 kSweepReverse = 1U << 20;
+constexpr u32 kNoTaskBond = 0x7fffffffU;  // a task end with no stage-s bond
+constexpr u32 kTaskWrite = 0x80000000U;   // the task writes that stage-s bond's far end
+constexpr u32 kExtraCopy = 0x80000000U;   // an extra entry that copies node (e & ~kExtraCopy)
+
+template <class R, bool DIRECT, bool PLASTIC>
+__global__ __launch_bounds__(kWG) void k_sweep(Batch<R> B, Step<R> S) {
+    const WgMap *m = tileEntry(B);
+    if (!m) return;
+    const DevLattice<R> L = B.L[m->lat];
+    const u32 kind = (m->pad >> 16) & 0xfU;
+    R lam_out;
+    if (kind == kSweepSingle) {
+        const u32 j = m->a + threadIdx.x;
+        if (j >= m->b) return;
+        const BondRec<R> r = ldStream(&L.rec[j]);
+        const u8 live = ldStream(&L.alive[j]);
+        if (L.ctl->stop || !live) return;
+        R4<R> ua = ldg(&L.u4[r.a]), ub = ldg(&L.u4[r.b]);
+        if (!solveBond<R, DIRECT, PLASTIC>(L, S, r, j, R(0), ua, ub, lam_out)) return;
+        stStream(&L.u4[r.a], ua);
+        stStream(&L.u4[r.b], ub);
+        return;
+    }
+    if (threadIdx.x >= (m->pad & 0xffffU)) return;
+    const bool rev = m->pad & kSweepReverse;
+    const R4<R> *src = rev ? L.u4b : L.u4;
+    R4<R> *dst = rev ? L.u4 : L.u4b;
+    if (kind == kSweepPair) {
+        const uint4 t = ldStream(&L.tasks[m->a + threadIdx.x]);
+        const u32 j2 = m->b + threadIdx.x;
+        const BondRec<R> r2 = ldStream(&L.rec[j2]);
+        const u8 live2 = ldStream(&L.alive[j2]);
+        const u32 ja = t.x & kNoTaskBond, jb = t.y & kNoTaskBond;
+        BondRec<R> ra{}, rb{};
+        u8 live_a = 0, live_b = 0;
+        if (ja != kNoTaskBond) {
+            ra = ldg(&L.rec[ja]);
+            live_a = ldg(&L.alive[ja]);
+        }
+        if (jb != kNoTaskBond) {
+            rb = ldg(&L.rec[jb]);
+            live_b = ldg(&L.alive[jb]);
+        }
+        if (L.ctl->stop) return;
+        R4<R> ua = ldg(&src[r2.a]), ub = ldg(&src[r2.b]);
+        R4<R> fa = ja != kNoTaskBond ? ldg(&src[t.z]) : R4<R>{}, fb = jb != kNoTaskBond ? ldg(&src[t.w]) : R4<R>{};
+        // Stage s at each end: the bond's own a side is whichever of its ends
+        // the task holds there.
+        if (live_a) {
+            if (ra.a == r2.a) solveBond<R, DIRECT, PLASTIC>(L, S, ra, ja, R(0), ua, fa, lam_out);
+            else solveBond<R, DIRECT, PLASTIC>(L, S, ra, ja, R(0), fa, ua, lam_out);
+        }
+        if (live_b) {
+            if (rb.a == r2.b) solveBond<R, DIRECT, PLASTIC>(L, S, rb, jb, R(0), ub, fb, lam_out);
+            else solveBond<R, DIRECT, PLASTIC>(L, S, rb, jb, R(0), fb, ub, lam_out);
+        }
+        if (live2) solveBond<R, DIRECT, PLASTIC>(L, S, r2, j2, R(0), ua, ub, lam_out);
+        stStream(&dst[r2.a], ua);
+        stStream(&dst[r2.b], ub);
+        if (t.x & kTaskWrite) stStream(&dst[t.z], fa);
+        if (t.y & kTaskWrite) stStream(&dst[t.w], fb);
+        return;
+    }
+    const u32 e = ldg(&L.extras[m->a + threadIdx.x]);
+    if (L.ctl->stop) return;
+    if (e & kExtraCopy) {
+        const u32 v = e & ~kExtraCopy;
+        stStream(&dst[v], ldg(&src[v]));
+        return;
+    }
+    const BondRec<R> r = ldg(&L.rec[e]);
+    const u8 live = ldg(&L.alive[e]);
+    R4<R> ua = ldg(&src[r.a]), ub = ldg(&src[r.b]);
+    if (live) solveBond<R, DIRECT, PLASTIC>(L, S, r, e, R(0), ua, ub, lam_out);
+    stStream(&dst[r.a], ua);
+    stStream(&dst[r.b], ub);
+}
+
+// The whole colour sweep of a one-iteration run for a lattice whose nodes fit
+// in LDS (FastLattice.hpp:229-231, the single-block shared-memory mode): one
+// workgroup per lattice loads every node's displacement into LDS, runs the
+// stages in order with a workgroup barrier between them, and writes the nodes
+// back. A stage's bonds share no node, so the threads of a stage never touch
+// the same LDS entry, and each solve is solveBond on the same values the
+// stage-by-stage launches read from memory: the same bits. Launch one
+// workgroup per map entry (lat = the lattice) with N * sizeof(R4<R>) bytes of
+// dynamic LDS and 256, 512 or 1024 threads.
+constexpr int kSweepLdsThreads = 1024;
+template <class R, bool DIRECT, bool PLASTIC>
+__global__ __launch_bounds__(kSweepLdsThreads) void k_sweep_lds(Batch<R> B, Step<R> S) {
+    extern __shared__ __align__(16) unsigned char sweep_lds[];
+    R4<R> *u = reinterpret_cast<R4<R> *>(sweep_lds);
+    const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
+    // Only the end of a substep (k_finish) sets stop, so it holds for the sweep.
+    if (ldg(&L.ctl->stop)) return;
+    const u32 N = L.N, T = blockDim.x;
+    for (u32 i = threadIdx.x; i < N; i += T) u[i] = ldg(&L.u4[i]);
+    __syncthreads();
+    u32 q0 = 0;
+    for (u32 st = 0; st < L.stage_count; ++st) {
+        const u32 q1 = ldg(&L.stage_end[st]);
+        for (u32 q = q0; q < q1; ++q) {
+            const uint2 rg = ldg(&L.stage_ranges[q]);
+            for (u32 j = rg.x + threadIdx.x; j < rg.y; j += T) {
+                const BondRec<R> r = ldg(&L.rec[j]);
+                if (!ldg(&L.alive[j])) continue;
+                R4<R> ua = u[r.a], ub = u[r.b];
+                R lam_out;
+                if (!solveBond<R, DIRECT, PLASTIC>(L, S, r, j, R(0), ua, ub, lam_out)) continue;
+                u[r.a] = ua;
+                u[r.b] = ub;
+            }
+        }
+        q0 = q1;
+        __syncthreads();
+    }
+    for (u32 i = threadIdx.x; i < N; i += T) stg(&L.u4[i], u[i]);
 }
 
 // Radial pair damping, one colour stage (LatticePhysics.hpp:1171-1184).

@@ -210,6 +210,12 @@ class LatticeLibrary:
     def has(self, name):
         return hasattr(self.lib, self.prefix + name)
 
+    def description(self) -> str:
+        """The visible devices (blx_description): name, arch, CUs, memory, LDS per workgroup."""
+        buf = C.create_string_buffer(4096)
+        self.check(self.fn("description")(buf, C.c_size_t(len(buf))), "description")
+        return buf.value.decode()
+
     def check(self, rc, what):
         if rc != abi.BLX_OK:
             msg = self._err().decode(errors="replace")
@@ -344,7 +350,35 @@ def _mass_properties(library, name, head, node_count, cell_size, spin):
     return out[:count.value].copy()
 
 
-class Backend:
+class _Tunable:
+    """Launch tuning and the autotuner (blx_*_tuning, blx_*_autotune); _kind is
+    "backend" or "batch"."""
+    _kind = "backend"
+
+    def tuning(self) -> dict:
+        t = abi.Tuning()
+        self.L.check(self.L.fn(f"{self._kind}_get_tuning")(self.h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = abi.Tuning(**self.tuning())
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise TypeError(f"unknown tuning field {k}")
+            setattr(t, k, int(v))
+        self.L.check(self.L.fn(f"{self._kind}_set_tuning")(self.h, C.byref(t)), "set_tuning")
+
+    def autotune(self, warm_substeps=32, probe_substeps=64) -> dict:
+        """Runs the autotuner; returns the tuning it kept and what it measured."""
+        rep = abi.AutotuneReport()
+        self.L.check(self.L.fn(f"{self._kind}_autotune")(self.h, C.c_uint64(warm_substeps),
+                                                         C.c_uint64(probe_substeps), C.byref(rep)), "autotune")
+        tried = [dict(rep.tried[i].as_dict(), us_per_substep=rep.seconds_per_substep[i] * 1e6)
+                 for i in range(rep.candidates)]
+        return {"chosen": rep.chosen.as_dict(), "tried": tried}
+
+
+class Backend(_Tunable):
     """LatticeBackend (src/fastlattice/FastLattice.hpp:255-369) over the C ABI."""
 
     def __init__(self, library: LatticeLibrary, precision, schedule: Schedule, *extra):
@@ -449,10 +483,11 @@ class Backend:
                                 cell_size, spin)
 
 
-class Batch:
+class Batch(_Tunable):
     """A batch of independent lattices advanced together (blx_batch_* of
     include/banjo_lattice.h). Slots share the step settings; each slot has
     its own state, sphere, loads, run control and status."""
+    _kind = "batch"
 
     def __init__(self, library: LatticeLibrary, precision, capacity, device=0):
         self.L = library

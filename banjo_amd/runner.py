@@ -28,12 +28,15 @@ def prepare(lib: LatticeLibrary, scene, origin=(0.0, 0.0, 0.0)):
 
 
 def run_scene(lib: LatticeLibrary, scene, precision=abi.PRECISION_FLOAT, substeps=None, backend_args=(),
-              labels="host", chunks=None, control_kw=None, prepared=None, keep_backend=False):
+              labels="host", chunks=None, control_kw=None, prepared=None, keep_backend=False, tuning=None,
+              autotune=None):
     """Advance `scene` and return the results in asset order.
 
     labels: "host" (library's host components), "device" (backend fragment
     labels), or None. chunks: list of substep counts for successive run()
-    calls (tests the status carrying over between calls).
+    calls (tests the status carrying over between calls). tuning: launch
+    tuning fields to set (blx_tuning); autotune: (warm, probe) substeps to
+    run the autotuner with before the run (its report is res["autotune"]).
     """
     sched, state = prepared if prepared is not None else prepare(lib, scene)
     be = lib.backend(precision, sched, *backend_args)
@@ -42,6 +45,9 @@ def run_scene(lib: LatticeLibrary, scene, precision=abi.PRECISION_FLOAT, substep
         be.set_blade(scene.info["blade"])
     if scene.forces is not None:
         be.set_external_forces(scene.forces, scene.force_substeps)
+    if tuning:
+        be.set_tuning(**tuning)
+    report = be.autotune(*autotune) if autotune else None
     total = scene.substeps if substeps is None else substeps
     plan = chunks if chunks is not None else [total]
     t0 = time.perf_counter()
@@ -61,7 +67,7 @@ def run_scene(lib: LatticeLibrary, scene, precision=abi.PRECISION_FLOAT, substep
     mode[order] = out.failure_mode
     res = dict(u=out.u.reshape(-1, 3), v=out.v.reshape(-1, 3), u_prev=out.u_prev.reshape(-1, 3),
                alive=alive, damage=damage, mode=mode, status=st, sphere=sphere, wall=wall,
-               first_failure=np.sort(order[be.first_failure_bonds()]), schedule=sched, state=out)
+               first_failure=np.sort(order[be.first_failure_bonds()]), schedule=sched, state=out, autotune=report)
     if labels == "host":
         res["labels"], res["components"] = lib.components(scene.node_count, scene.asset.bond_a, scene.asset.bond_b,
                                                           alive)
@@ -94,9 +100,10 @@ def collect(lib: LatticeLibrary, scene, sched, state_out, sphere, status, first_
 
 
 def run_batch(lib: LatticeLibrary, scenes_, precision=abi.PRECISION_FLOAT, substeps=None, control_kw=None,
-              steps_per_launch=0, capacity=None, labels=True):
+              steps_per_launch=0, capacity=None, labels=True, tuning=None, autotune=None):
     """Advance several scenes together in one batch (they must share step
-    settings) and return each one's results as run_scene would."""
+    settings) and return each one's results as run_scene would. tuning and
+    autotune as for run_scene (the autotuner runs once every trial is armed)."""
     from .lattice import Batch
     cap = capacity or len(scenes_)
     bt = Batch(lib, precision, cap)
@@ -113,6 +120,9 @@ def run_batch(lib: LatticeLibrary, scenes_, precision=abi.PRECISION_FLOAT, subst
             kw = control_kw[k] if isinstance(control_kw, (list, tuple)) else (control_kw or {})
             bt.arm(k, make_control(n, **kw))
             prepared.append((sched, state))
+        if tuning:
+            bt.set_tuning(**tuning)
+        report = bt.autotune(*autotune) if autotune else None
         t0 = time.perf_counter()
         while bt.advance(1 << 62, steps_per_launch):
             pass
@@ -125,6 +135,7 @@ def run_batch(lib: LatticeLibrary, scenes_, precision=abi.PRECISION_FLOAT, subst
             res = collect(lib, sc, sched, st_out, sphere, bt.status(k), bt.first_failure_bonds(k),
                           bt.fragment_labels(k) if labels else None)
             res["wall"] = wall
+            res["autotune"] = report
             out.append(res)
         return out
     finally:

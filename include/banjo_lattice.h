@@ -187,6 +187,11 @@ typedef struct blx_run_status {
     double plastic_work_j;
     float max_plastic_stretch;
     blx_node_contact_accumulators node_contact;
+    /* RunStatus::shared_memory_positions / shared_memory_bytes
+     * (FastLattice.hpp:229-231): the sweep kept this lattice's nodes in LDS,
+     * and the bytes they took (0: none). */
+    uint8_t shared_memory_positions;
+    uint64_t shared_memory_bytes;
 } blx_run_status;
 
 /* LatticeAsset bonds + ActiveMatter nodes and bond state, asset order. */
@@ -571,6 +576,55 @@ int blx_backend_set_profiling(blx_backend *backend, int on);
  * the number of kinds (call with index 0 to learn it). */
 int blx_backend_kernel_stats(const blx_backend *backend, uint32_t index, char *name, size_t capacity,
                              double *seconds, uint64_t *launches, uint32_t *count);
+
+/* Launch tuning of a backend or a batch. No counterpart in the reference,
+ * whose CUDA lane fixed its launch shape when it was built. Every setting
+ * gives the same bits; they differ in speed, which depends on the lattices'
+ * sizes. Defaults: 1, 1, 1024, 0.
+ *  colours_per_launch  colour stages per sweep launch in a one-iteration run:
+ *                      1, or 2 (stage pairs, each bond of the second stage
+ *                      solving the first stage's bonds at its ends itself);
+ *  lds_sweep           nonzero: in a one-iteration run, a lattice whose nodes
+ *                      fit in one workgroup's LDS (RunStatus
+ *                      shared_memory_bytes) runs its whole sweep in one
+ *                      workgroup, the nodes held in LDS;
+ *  lds_threads         that workgroup's size: 256, 512 or 1024;
+ *  graph_substeps      substeps per captured graph launch, 0..256 (0 enqueues
+ *                      every substep; node contact always enqueues). */
+typedef struct blx_tuning {
+    uint32_t colours_per_launch;
+    uint32_t lds_sweep;
+    uint32_t lds_threads;
+    uint32_t graph_substeps;
+} blx_tuning;
+
+#define BLX_TUNE_MAX 32
+/* What the autotuner tried and measured: host seconds per substep of each
+ * candidate (the better of two rounds), and the one it kept. */
+typedef struct blx_autotune_report {
+    blx_tuning chosen;
+    uint32_t candidates;
+    blx_tuning tried[BLX_TUNE_MAX];
+    double seconds_per_substep[BLX_TUNE_MAX];
+} blx_autotune_report;
+
+int blx_backend_get_tuning(const blx_backend *backend, blx_tuning *tuning);
+int blx_backend_set_tuning(blx_backend *backend, const blx_tuning *tuning);
+int blx_batch_get_tuning(const blx_batch *batch, blx_tuning *tuning);
+int blx_batch_set_tuning(blx_batch *batch, const blx_tuning *tuning);
+/* The autotuner (no counterpart in the reference). Each candidate tuning
+ * (colour stages per launch, the LDS sweep and its workgroup size where a
+ * lattice fits, substeps per graph launch) advances the lattice from its
+ * current state by warm substeps and, again from that state, by warm + probe
+ * substeps; the difference of the two host times is its cost. The fastest is
+ * set, and the state (device and host, run status included) is put back as it
+ * was: autotuning changes no result. The backend's lattice must be uploaded;
+ * a batch tunes over its armed slots, whose budgets should cover
+ * warm + probe. Takes a device copy of every uploaded lattice while it runs. */
+int blx_backend_autotune(blx_backend *backend, uint64_t warm_substeps, uint64_t probe_substeps,
+                         blx_autotune_report *report);
+int blx_batch_autotune(blx_batch *batch, uint64_t warm_substeps, uint64_t probe_substeps,
+                       blx_autotune_report *report);
 
 /* A streaming device-to-device copy of `bytes` (read once, written once),
  * repeated; the mean seconds per copy. Used to calibrate the hardware

@@ -271,6 +271,8 @@ void checkValueTypes() {
     st.plastic_work_j = next();
     st.max_plastic_stretch = float(next());
     st.node_contact = {nu(), nu(), nu(), nu(), nu(), next(), next(), next(), {next(), next(), next()}};
+    st.shared_memory_positions = 1;
+    st.shared_memory_bytes = nu();
     RunStatus r;
     hip_adapter::statusBack(st, r);
     bool sok = r.total_steps == st.total_steps && r.first_failure_step == st.first_failure_step &&
@@ -311,7 +313,8 @@ void checkValueTypes() {
                r.node_contact.maximum_position_correction_m == st.node_contact.maximum_position_correction_m &&
                r.node_contact.momentum_residual_x == st.node_contact.momentum_residual[0] &&
                r.node_contact.momentum_residual_z == st.node_contact.momentum_residual[2] &&
-               !r.shared_memory_positions && r.shared_memory_bytes == 0 && r.bond_integrator == kBondXpbd;
+               r.shared_memory_positions && r.shared_memory_bytes == st.shared_memory_bytes &&
+               r.bond_integrator == kBondXpbd;
     for (unsigned p = 0; p < kPhaseCount; ++p) sok = sok && r.phase_seconds[p] == st.phase_seconds[p];
     expect(sok, "blx_run_status -> RunStatus field for field");
     std::printf("value types: settings, sphere, run control, run status\n");

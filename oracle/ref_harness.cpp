@@ -223,6 +223,8 @@ void statusToC(const RunStatus &s, blx_run_status &c) {
     c.node_contact.momentum_residual[0] = n.momentum_residual_x;
     c.node_contact.momentum_residual[1] = n.momentum_residual_y;
     c.node_contact.momentum_residual[2] = n.momentum_residual_z;
+    c.shared_memory_positions = s.shared_memory_positions ? 1 : 0;
+    c.shared_memory_bytes = s.shared_memory_bytes;
 }
 
 struct Handle {

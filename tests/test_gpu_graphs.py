@@ -1,7 +1,7 @@
 """Substep graphs against enqueued substeps, on the device.
 
-A run replays a captured block of substeps (BLX_GRAPHS=1) or enqueues every
-substep (the default). The two must give the same bits after every
+A run replays a captured block of substeps (blx_tuning graph_substeps 8) or
+enqueues every substep (the default). The two must give the same bits after every
 run() call: calls that are whole graphs, graphs plus an enqueued remainder,
 and calls shorter than one graph, with the host's synchronisations and
 state reads between them. The same holds for a batch advanced in chunks.
@@ -18,23 +18,20 @@ F = abi.PRECISION_FLOAT
 CALLS = [8, 13, 3, 40, 16, 7]
 
 
-def backend(gpu, sc, monkeypatch, graphs):
-    if graphs:
-        monkeypatch.setenv("BLX_GRAPHS", "1")
-    else:
-        monkeypatch.delenv("BLX_GRAPHS", raising=False)
+def backend(gpu, sc, graphs):
     sched, state = prepare(gpu, sc)
     be = gpu.backend(F, sched, 0)
     be.upload(state, sc.settings, sc.sphere)
+    be.set_tuning(graph_substeps=8 if graphs else 0)
     return be, state
 
 
 @pytest.mark.parametrize("scene", ["impact", "bridge"])
-def test_graph_replays_equal_enqueued_substeps(gpu, monkeypatch, scene):
+def test_graph_replays_equal_enqueued_substeps(gpu, scene):
     sc = scenes.small_impact(nx=14, ny=8, nz=14, substeps=90) if scene == "impact" else scenes.bridge_impact()
     print(f"{scene}: calls of {CALLS} substeps, graphs against enqueued", flush=True)
-    g, state = backend(gpu, sc, monkeypatch, True)
-    e, _ = backend(gpu, sc, monkeypatch, False)
+    g, state = backend(gpu, sc, True)
+    e, _ = backend(gpu, sc, False)
     try:
         for n in CALLS:
             sg, se = g.run(control(n)), e.run(control(n))
@@ -56,19 +53,17 @@ def test_graph_replays_equal_enqueued_substeps(gpu, monkeypatch, scene):
         e.close()
 
 
-def test_batch_graphs_equal_enqueued_substeps(gpu, monkeypatch):
+def test_batch_graphs_equal_enqueued_substeps(gpu):
     trials = [scenes.config3_trial(t, substeps=96) for t in range(4)]
-    monkeypatch.delenv("BLX_GRAPHS", raising=False)
     eager = run_batch(gpu, trials, F, steps_per_launch=24)
-    monkeypatch.setenv("BLX_GRAPHS", "1")
-    graphs = run_batch(gpu, trials, F, steps_per_launch=24)
+    graphs = run_batch(gpu, trials, F, steps_per_launch=24, tuning={"graph_substeps": 8})
     for a, b in zip(graphs, eager):
         for k in ("u", "v", "alive", "damage", "labels"):
             assert np.array_equal(a[k], b[k]), k
         assert a["status"].broken_bonds == b["status"].broken_bonds
 
 
-def test_batch_graphs_with_uneven_budgets_equal_enqueued(gpu, monkeypatch):
+def test_batch_graphs_with_uneven_budgets_equal_enqueued(gpu):
     # Trials stop at different substeps, so the launch maps are rebuilt between
     # graph replays; a rebuild must drop the captured graph (its launch
     # arguments hold the old maps) and capture again.
@@ -76,10 +71,9 @@ def test_batch_graphs_with_uneven_budgets_equal_enqueued(gpu, monkeypatch):
     steps = [40, 96, 17, 64]
     kws = [{}, {"no_failure_steps": 30}, {}, {"calm_steps": 20}]
     print(f"batch of {len(trials)} with budgets {steps}, graphs against enqueued", flush=True)
-    monkeypatch.delenv("BLX_GRAPHS", raising=False)
     eager = run_batch(gpu, trials, F, substeps=steps, control_kw=kws, steps_per_launch=8)
-    monkeypatch.setenv("BLX_GRAPHS", "1")
-    graphs = run_batch(gpu, trials, F, substeps=steps, control_kw=kws, steps_per_launch=8)
+    graphs = run_batch(gpu, trials, F, substeps=steps, control_kw=kws, steps_per_launch=8,
+                       tuning={"graph_substeps": 8})
     for a, b in zip(graphs, eager):
         for k in ("u", "v", "alive", "damage", "labels"):
             assert np.array_equal(a[k], b[k]), k
