@@ -1818,6 +1818,7 @@ private:
     DevBuf maps_;
     MapRef node_tiles_map_, bond_tiles_map_, list_map_, small_map_, mark_map_, lattice_map_;
     std::vector<MapRef> stage_maps_;
+    std::vector<MapRef> damp_maps_;      // colour stages of every running slot, for radial damping
     size_t interior_stages_ = 0;
     std::vector<MapRef> sweep_maps_;     // the fused sweep's launch groups
     std::vector<uint32_t> sweep_phase_;  // reference phase id of each group (5 interior, 6 boundary)
@@ -1936,6 +1937,21 @@ private:
                     for (const uint2 &r : sl.stages[s]) all.push_back(WgMap{k, r.x, r.y, 0});
             }
             end(stage_maps_[s]);
+        }
+        // Radial damping runs by colour stages on every running slot, the
+        // ones that sweep in LDS included.
+        size_t damp_stages = 0;
+        if (S_.damping > R(0))
+            for (uint32_t k : run) damp_stages = std::max(damp_stages, slots_[k]->stages.size());
+        damp_maps_.assign(damp_stages, MapRef{});
+        for (size_t s = 0; s < damp_stages; ++s) {
+            begin(damp_maps_[s]);
+            for (uint32_t k : run) {
+                const Slot<R> &sl = *slots_[k];
+                if (s < sl.stages.size())
+                    for (const uint2 &r : sl.stages[s]) all.push_back(WgMap{k, r.x, r.y, 0});
+            }
+            end(damp_maps_[s]);
         }
         // The fused sweep's launch group g of the batch is group g of every
         // running slot (each slot's groups keep its own stage order and its
@@ -2338,7 +2354,7 @@ private:
         else BLX_SIGNAL((k_velocity<R, false>), ng, dim3(kWG), stream_, after_velocity, nodes, S, Batch<R>{});
         profEnd();
         if (S.damping > R(0)) {
-            for (const MapRef &m : stage_maps_)
+            for (const MapRef &m : damp_maps_)
                 BLX_LAUNCH(KDamp, 9, hipLaunchKernelGGL((k_damp<R, DIRECT>), tiles(m), dim3(kWG), 0, stream_, batch(m), S));
             BLX_LAUNCH(KSupportVelocity, 9, hipLaunchKernelGGL((k_support_velocity<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
