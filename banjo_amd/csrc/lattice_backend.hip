@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -315,7 +316,7 @@ public:
     // is a list of bond ranges of at most one workgroup each.
     std::vector<std::vector<uint2>> stages;
     size_t interior_stages = 0;
-    // The fused sweep (k_sweep) of a one-iteration run: per launch group, its
+    // The paired sweep (k_sweep_pair) of a one-iteration run: per launch group, its
     // map entries (lat filled in when the maps are built) and its first stage.
     std::vector<std::vector<WgMap>> sweep_groups;
     // The stages as merged bond ranges for the one-workgroup sweep
@@ -323,9 +324,10 @@ public:
     std::vector<uint2> sweep_ranges;
     std::vector<uint32_t> stage_end;
     std::vector<uint32_t> sweep_first_stage;
+    std::vector<uint32_t> sweep_lds_nodes;  // per launch group: the most nodes one of its workgroups holds in LDS
     // Offsets of the device arrays in the arena.
     struct Off {
-        size_t x04, u4, u4b, tasks, extras, stage_ranges, stage_end, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
+        size_t x04, u4, u4b, pnodes, pbonds, stage_ranges, stage_end, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
         size_t nbr_key, spal, sdelta, nbr_bond, epal, nodir, rec, alive, lambda, pext;
         size_t bnd, alive_n, thr, thr2, pv, pyz, mode, pstr, slots;
         size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
@@ -399,8 +401,8 @@ public:
         l.stage_ranges = arena.at<uint2>(o.stage_ranges);
         l.stage_end = arena.at<uint32_t>(o.stage_end);
         l.stage_count = uint32_t(stage_end.size());
-        l.tasks = arena.at<uint4>(o.tasks);
-        l.extras = arena.at<uint32_t>(o.extras);
+        l.pnodes = arena.at<uint32_t>(o.pnodes);
+        l.pbonds = arena.at<PairBond<R>>(o.pbonds);
         l.up4 = arena.at<R4<R>>(o.up4);
         l.v4 = arena.at<R4<R>>(o.v4);
         l.x04 = arena.at<R4<R>>(o.x04);
@@ -490,6 +492,7 @@ public:
         interior_stages = src.interior_stages;
         sweep_groups = src.sweep_groups;
         sweep_first_stage = src.sweep_first_stage;
+        sweep_lds_nodes = src.sweep_lds_nodes;
         sweep_ranges = src.sweep_ranges;
         stage_end = src.stage_end;
         scheduled = true;
@@ -527,75 +530,168 @@ public:
         armed = false;
     }
 
-    // The fused sweep's launch groups, tasks and extra entries (k_sweep) from
-    // the bond ends in schedule order. The stages are grouped as one stage in
-    // place, then an even number of pairs, then the stages left over, so the
-    // pairs' ping-pong ends in u4.
-    void buildSweep(const uint32_t *end_a, const uint32_t *end_b, std::vector<uint4> &tasks,
-                    std::vector<uint32_t> &extras) {
+    // The paired sweep's plan (k_sweep_pair) of a one-iteration run. Nodes are
+    // grouped in tiles of kPairTile (kPairTileSmall below kPairWideNodes
+    // nodes) cells of the reference grid (the cell is
+    // the shortest rest length), ordered z, y, x, so that a workgroup's owned
+    // nodes fill whole cache lines and the workgroups next to it in the launch
+    // (which run on the same XCD) share its halo. Per stage pair, the
+    // workgroup of each tile lists its owned nodes, the halo they depend on and
+    // the bonds to solve; a tile whose lists exceed one workgroup is split in
+    // halves by node order until they fit (one owned node needs at most four
+    // nodes and two bonds). The stages are grouped as the stages left over
+    // from an even number of pairs, each in place, then the pairs, so that the
+    // pairs' ping-pong ends in u4. Bond entries get their palette index and
+    // compliance when the record is built.
+    void buildSweep(const blx_state_view &st, std::vector<uint32_t> &nodes, std::vector<PairBond<R>> &bonds) {
         sweep_groups.clear();
         sweep_first_stage.clear();
-        tasks.clear();
-        extras.clear();
+        sweep_lds_nodes.clear();
+        nodes.clear();
+        bonds.clear();
         const size_t S = stages.size();
         const size_t pairs = (S / 2) & ~size_t(1);
         const size_t singles = S - 2 * pairs;
-        auto single = [&](size_t st) {
+        const uint32_t *end_a = st.bond_a, *end_b = st.bond_b;
+        auto single = [&](size_t s) {
+            std::vector<uint2> runs;
+            for (const uint2 &r : stages[s]) {
+                if (!runs.empty() && runs.back().y == r.x) runs.back().y = r.y;
+                else runs.push_back(r);
+            }
             std::vector<WgMap> g;
-            for (const uint2 &r : stages[st]) g.push_back(WgMap{0, r.x, r.y, kSweepSingle << 16});
-            sweep_groups.push_back(std::move(g));
-            sweep_first_stage.push_back(uint32_t(st));
-        };
-        std::vector<uint32_t> at(N, kNone);   // node -> its stage-s bond
-        std::vector<uint8_t> next(N, 0);      // node is an end of a stage-(s + 1) bond
-        size_t st = 0;
-        if (singles > 0) single(st++);
-        for (size_t p = 0; p < pairs; ++p, st += 2) {
-            const uint32_t dir = (p & 1) ? kSweepReverse : 0U;
-            std::vector<WgMap> g;
-            for (const uint2 &r : stages[st])
-                for (uint32_t j = r.x; j < r.y; ++j) at[end_a[j]] = at[end_b[j]] = j;
-            for (const uint2 &r : stages[st + 1])
-                for (uint32_t j = r.x; j < r.y; ++j) next[end_a[j]] = next[end_b[j]] = 1;
-            auto far = [&](uint32_t v, uint32_t &far_node) -> uint32_t {
-                const uint32_t j = at[v];
-                if (j == kNone) {
-                    far_node = 0;
-                    return kNoTaskBond;
+            for (const uint2 &r : runs)
+                for (uint32_t j = r.x; j < r.y; j += kPairThreads) {
+                    g.push_back(WgMap{0, j, std::min(r.y, j + kPairThreads), kSweepSingle << 16});
+                    g.push_back(WgMap{0, 0, 0, 0});
                 }
-                far_node = end_a[j] == v ? end_b[j] : end_a[j];
-                return j | (next[far_node] ? 0U : kTaskWrite);
-            };
-            for (const uint2 &r : stages[st + 1])
-                for (uint32_t j0 = r.x; j0 < r.y; j0 += kWG) {
-                    const uint32_t n = std::min<uint32_t>(kWG, r.y - j0);
-                    g.push_back(WgMap{0, uint32_t(tasks.size()), j0, n | (kSweepPair << 16) | dir});
-                    for (uint32_t j = j0; j < j0 + n; ++j) {
-                        uint4 t{};
-                        t.x = far(end_a[j], t.z);
-                        t.y = far(end_b[j], t.w);
-                        tasks.push_back(t);
+            sweep_groups.push_back(std::move(g));
+            sweep_first_stage.push_back(uint32_t(s));
+            sweep_lds_nodes.push_back(0);
+        };
+        size_t s = 0;
+        while (s < singles) single(s++);
+        if (pairs > 0) {
+            // Nodes in tile order (ties in node order).
+            double cell = std::numeric_limits<double>::infinity();
+            for (uint32_t k = 0; k < B; ++k)
+                if (st.rest_length[k] > 0.0) cell = std::min(cell, st.rest_length[k]);
+            if (!(cell < std::numeric_limits<double>::infinity())) cell = 1.0;
+            double lo[3] = {0.0, 0.0, 0.0};
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::numeric_limits<double>::infinity();
+                for (uint32_t i = 0; i < N; ++i) lo[c] = std::min(lo[c], st.x0[3 * size_t(i) + c]);
+            }
+            std::vector<uint64_t> key(N);
+            const uint32_t *tile = N >= kPairWideNodes ? kPairTile : kPairTileSmall;
+            for (uint32_t i = 0; i < N; ++i) {
+                uint64_t k = 0;
+                for (int c = 2; c >= 0; --c) {
+                    const double g = std::floor((st.x0[3 * size_t(i) + c] - lo[c]) / cell + 0.5);
+                    const uint64_t q = g > 0.0 ? std::min<uint64_t>(uint64_t(g) / tile[c], (1ULL << 21) - 1) : 0;
+                    k = (k << 21) | q;
+                }
+                key[i] = k;
+            }
+            std::vector<uint32_t> order(N);
+            for (uint32_t i = 0; i < N; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
+            std::vector<uint32_t> tile_begin;
+            for (uint32_t q = 0; q < N; ++q)
+                if (q == 0 || key[order[q]] != key[order[q - 1]]) tile_begin.push_back(q);
+            tile_begin.push_back(N);
+
+            std::vector<uint32_t> p1(N, kNone), p2(N, kNone), nstamp(N, 0), nslot(N, 0), bstamp(B, 0);
+            uint32_t stamp = 0;
+            std::vector<uint32_t> halo, level1, b1, b2;
+            auto other = [&](uint32_t j, uint32_t v) { return end_a[j] == v ? end_b[j] : end_a[j]; };
+            uint32_t dir = 0, widest = 0;
+            std::vector<WgMap> g;
+            // One workgroup's lists for the owned nodes own[0 .. n), or two
+            // halves' when they do not fit.
+            std::function<void(const uint32_t *, uint32_t)> emit = [&](const uint32_t *own, uint32_t n) {
+                if (++stamp == 0) throw Logic("paired sweep plan stamp wrapped");
+                halo.clear();
+                level1.clear();
+                b1.clear();
+                b2.clear();
+                for (uint32_t q = 0; q < n; ++q) {
+                    nstamp[own[q]] = stamp;
+                    nslot[own[q]] = q;
+                }
+                auto need = [&](uint32_t v) {
+                    if (nstamp[v] != stamp) {
+                        nstamp[v] = stamp;
+                        halo.push_back(v);
+                    }
+                };
+                for (uint32_t q = 0; q < n; ++q) {
+                    const uint32_t j = p2[own[q]];
+                    if (j == kNone) continue;
+                    const uint32_t y = other(j, own[q]);
+                    if (nstamp[y] != stamp) level1.push_back(y);
+                    need(y);
+                    if (bstamp[j] != stamp) {
+                        bstamp[j] = stamp;
+                        b2.push_back(j);
                     }
                 }
-            const size_t e0 = extras.size();
-            for (const uint2 &r : stages[st])
-                for (uint32_t j = r.x; j < r.y; ++j)
-                    if (!next[end_a[j]] && !next[end_b[j]]) extras.push_back(j);
-            for (uint32_t v = 0; v < N; ++v)
-                if (at[v] == kNone && !next[v]) extras.push_back(v | kExtraCopy);
-            for (size_t q = e0; q < extras.size(); q += kWG)
-                g.push_back(WgMap{0, uint32_t(q), 0, uint32_t(std::min<size_t>(kWG, extras.size() - q)) |
-                                                         (kSweepExtra << 16) | dir});
-            for (const uint2 &r : stages[st])
-                for (uint32_t j = r.x; j < r.y; ++j) at[end_a[j]] = at[end_b[j]] = kNone;
-            for (const uint2 &r : stages[st + 1])
-                for (uint32_t j = r.x; j < r.y; ++j) next[end_a[j]] = next[end_b[j]] = 0;
-            sweep_groups.push_back(std::move(g));
-            sweep_first_stage.push_back(uint32_t(st));
+                auto stage_s = [&](uint32_t v) {
+                    const uint32_t j = p1[v];
+                    if (j == kNone) return;
+                    need(other(j, v));
+                    if (bstamp[j] != stamp) {
+                        bstamp[j] = stamp;
+                        b1.push_back(j);
+                    }
+                };
+                for (uint32_t q = 0; q < n; ++q) stage_s(own[q]);
+                for (uint32_t v : level1) stage_s(v);
+                if (n + halo.size() > kPairMaxNodes || b1.size() > kPairMaxBonds || b2.size() > kPairMaxBonds) {
+                    const uint32_t h = n / 2;
+                    emit(own, h);
+                    emit(own + h, n - h);
+                    return;
+                }
+                std::sort(halo.begin(), halo.end());
+                for (size_t q = 0; q < halo.size(); ++q) nslot[halo[q]] = uint32_t(n + q);
+                std::sort(b1.begin(), b1.end());
+                std::sort(b2.begin(), b2.end());
+                const uint32_t all = uint32_t(n + halo.size());
+                g.push_back(WgMap{0, uint32_t(nodes.size()), uint32_t(bonds.size()), (kSweepPair << 16) | dir});
+                g.push_back(WgMap{n | (all << 16), uint32_t(b1.size()) | (uint32_t(b2.size()) << 16), 0, 0});
+                nodes.insert(nodes.end(), own, own + n);
+                nodes.insert(nodes.end(), halo.begin(), halo.end());
+                for (const std::vector<uint32_t> *list : {&b1, &b2})
+                    for (uint32_t j : *list) {
+                        PairBond<R> e{};
+                        e.j = j;
+                        e.slots = nslot[end_a[j]] | (nslot[end_b[j]] << 16);
+                        bonds.push_back(e);
+                    }
+                if (nodes.size() >= 0xffffffffULL || bonds.size() >= 0xffffffffULL)
+                    throw Invalid("the paired sweep holds at most 2^32 - 1 node and bond entries");
+                widest = std::max(widest, all);
+            };
+            for (size_t p = 0; p < pairs; ++p, s += 2) {
+                for (const uint2 &r : stages[s])
+                    for (uint32_t j = r.x; j < r.y; ++j) p1[end_a[j]] = p1[end_b[j]] = j;
+                for (const uint2 &r : stages[s + 1])
+                    for (uint32_t j = r.x; j < r.y; ++j) p2[end_a[j]] = p2[end_b[j]] = j;
+                dir = (p & 1) ? kSweepReverse : 0U;
+                widest = 0;
+                g.clear();
+                for (size_t t = 0; t + 1 < tile_begin.size(); ++t)
+                    emit(order.data() + tile_begin[t], tile_begin[t + 1] - tile_begin[t]);
+                for (const uint2 &r : stages[s])
+                    for (uint32_t j = r.x; j < r.y; ++j) p1[end_a[j]] = p1[end_b[j]] = kNone;
+                for (const uint2 &r : stages[s + 1])
+                    for (uint32_t j = r.x; j < r.y; ++j) p2[end_a[j]] = p2[end_b[j]] = kNone;
+                sweep_groups.push_back(g);
+                sweep_first_stage.push_back(uint32_t(s));
+                sweep_lds_nodes.push_back(widest);
+            }
         }
-        while (st < S) single(st++);
-        if (tasks.size() >= 0x80000000ULL || extras.size() >= 0x80000000ULL)
-            throw Invalid("the fused sweep holds at most 2^31 tasks");
     }
 
     // Builds every device array of this lattice on `stream` (and waits for
@@ -727,9 +823,9 @@ public:
         }
         tensile_only = tensile;
 
-        std::vector<uint4> sweep_tasks;
-        std::vector<uint32_t> sweep_extras;
-        buildSweep(st.bond_a, st.bond_b, sweep_tasks, sweep_extras);
+        std::vector<uint32_t> sweep_nodes;
+        std::vector<PairBond<R>> sweep_bonds;
+        buildSweep(st, sweep_nodes, sweep_bonds);
 
         // Layout in the arena.
         arena.reset();
@@ -741,8 +837,8 @@ public:
         o.u4b = arena.reserve<R4<R>>(N);
         o.stage_ranges = arena.reserve<uint2>(sweep_ranges.size());
         o.stage_end = arena.reserve<uint32_t>(stage_end.size());
-        o.tasks = arena.reserve<uint4>(sweep_tasks.size());
-        o.extras = arena.reserve<uint32_t>(sweep_extras.size());
+        o.pnodes = arena.reserve<uint32_t>(sweep_nodes.size());
+        o.pbonds = arena.reserve<PairBond<R>>(sweep_bonds.size());
         o.up4 = arena.reserve<R4<R>>(N);
         o.v4 = arena.reserve<R4<R>>(N);
         o.rinv = arena.reserve<R>(9 * size_t(N));
@@ -847,10 +943,9 @@ public:
         for (uint32_t i = 0; i < N; ++i) all[i] = i;
         putv(o.x04, x04);
         putv(o.u4, u4);
-        putv(o.tasks, sweep_tasks);
+        putv(o.pnodes, sweep_nodes);
         putv(o.stage_ranges, sweep_ranges);
         putv(o.stage_end, stage_end);
-        putv(o.extras, sweep_extras);
         putv(o.up4, up4);
         putv(o.v4, v4);
         putv(o.nbr_key, key);
@@ -902,6 +997,11 @@ public:
                 mode[q] = st.failure_mode[k];
             }
             putv(o.rec, rec);
+            for (PairBond<R> &e : sweep_bonds) {
+                e.p = pal_of[e.j];
+                e.c = rec[e.j].c;
+            }
+            putv(o.pbonds, sweep_bonds);
             putv(o.bnd, bnd);
             putv(o.thr, thr);
             putv(o.thr2, thr2);
@@ -961,7 +1061,7 @@ public:
         // release is device-scope; the default system-scope fence makes every
         // signal also write the caches back for the host (about 1% of the step).
         for (hipEvent_t *e : {&ev_bond_end_, &ev_rinv_, &ev_pass1_, &ev_velocity_, &ev_early_, &ev_predict_, &ev_start_,
-                              &ev_sweep_})
+                              &ev_sweep_, &ev_strain_})
             HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
         HIP_OK(hipEventCreate(&ev0_));
         HIP_OK(hipEventCreate(&ev1_));
@@ -983,7 +1083,7 @@ public:
         for (hipStream_t s : {stream_, side_, early_})
             if (s) (void)hipStreamSynchronize(s);
         for (hipEvent_t e : {ev_bond_end_, ev_rinv_, ev_pass1_, ev_velocity_, ev_early_, ev_predict_, ev_start_, ev_sweep_,
-                             ev0_, ev1_})
+                             ev_strain_, ev0_, ev1_})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
         for (hipStream_t s : {early_, side_, stream_})
@@ -1803,7 +1903,7 @@ private:
     hipStream_t stream_ = nullptr, side_ = nullptr, early_ = nullptr;
     hipEvent_t ev_predict_ = nullptr, ev_start_ = nullptr, ev_sweep_ = nullptr;
     hipEvent_t ev_bond_end_ = nullptr, ev_rinv_ = nullptr, ev_pass1_ = nullptr, ev_velocity_ = nullptr,
-               ev_early_ = nullptr;
+               ev_early_ = nullptr, ev_strain_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t cus_ = 256;
     uint64_t launches_ = 0;
@@ -1820,10 +1920,16 @@ private:
     std::vector<MapRef> stage_maps_;
     std::vector<MapRef> damp_maps_;      // colour stages of every running slot, for radial damping
     size_t interior_stages_ = 0;
-    std::vector<MapRef> sweep_maps_;     // the fused sweep's launch groups
+    std::vector<MapRef> sweep_maps_;     // the paired sweep's launch groups
     std::vector<uint32_t> sweep_phase_;  // reference phase id of each group (5 interior, 6 boundary)
+    std::vector<uint32_t> sweep_lds_;    // LDS bytes of each group's launch
+    std::vector<int64_t> sweep_stage_;   // the stage a group of single entries runs by stage map, or -1
     // Launch tuning (blx_tuning). Every setting gives the same bits.
-    blx_tuning tuning_{1U, 1U, 1024U, 0U};
+    // A batch's default: colour stages one per launch and no LDS sweep, the
+    // faster setting for many small lattices advancing together (a lattice
+    // sweeping in LDS holds its launch for the whole sweep); a single-lattice
+    // backend starts from kBackendTuning instead.
+    blx_tuning tuning_{1U, 0U, 1024U, 0U};
     MapRef lds_map_;                     // one entry per lattice that sweeps in LDS (k_sweep_lds)
     uint32_t lds_bytes_ = 0;             // that launch's dynamic LDS: the largest such lattice's nodes
     size_t lds_limit_ = 0;               // LDS one workgroup may hold on this device
@@ -1953,26 +2059,55 @@ private:
             }
             end(damp_maps_[s]);
         }
-        // The fused sweep's launch group g of the batch is group g of every
+        // The paired sweep's launch group g of the batch is group g of every
         // running slot (each slot's groups keep its own stage order and its
         // own even number of ping-pong pairs).
         size_t groups = 0;
         for (uint32_t k : staged) groups = std::max(groups, slots_[k]->sweep_groups.size());
         sweep_maps_.assign(groups, MapRef{});
         sweep_phase_.assign(groups, 5U);
+        sweep_lds_.assign(groups, 0U);
         for (size_t g = 0; g < groups; ++g) {
             begin(sweep_maps_[g]);
             for (uint32_t k : staged) {
                 const Slot<R> &sl = *slots_[k];
                 if (g >= sl.sweep_groups.size()) continue;
                 if (sl.sweep_first_stage[g] >= sl.interior_stages) sweep_phase_[g] = 6U;
-                for (WgMap e : sl.sweep_groups[g]) {
+                sweep_lds_[g] = std::max(sweep_lds_[g], uint32_t(sl.sweep_lds_nodes[g] * sizeof(R4<R>)));
+                // Two entries per workgroup: the second holds counts, not a lattice.
+                const std::vector<WgMap> &es = sl.sweep_groups[g];
+                for (size_t q = 0; q < es.size(); q += 2) {
+                    WgMap e = es[q];
                     e.lat = k;
                     all.push_back(e);
+                    all.push_back(es[q + 1]);
                 }
             }
             end(sweep_maps_[g]);
         }
+        // A group that is the same single stage s in every staged slot that
+        // has a stage s runs as k_solve over stage s's map, the lighter kernel.
+        sweep_stage_.assign(groups, -1);
+        for (size_t g = 0; g < groups; ++g) {
+            int64_t st = -1;
+            bool ok = true;
+            for (uint32_t k : staged) {
+                const Slot<R> &sl = *slots_[k];
+                if (g >= sl.sweep_groups.size()) continue;
+                const bool single = ((sl.sweep_groups[g][0].pad >> 16) & 0xfU) == kSweepSingle;
+                if (!single || (st >= 0 && st != int64_t(sl.sweep_first_stage[g]))) ok = false;
+                st = sl.sweep_first_stage[g];
+            }
+            for (uint32_t k : staged) {
+                const Slot<R> &sl = *slots_[k];
+                if (!ok || st < 0 || size_t(st) >= sl.stages.size()) continue;
+                if (g >= sl.sweep_groups.size() || sl.sweep_first_stage[g] != uint32_t(st) ||
+                    ((sl.sweep_groups[g][0].pad >> 16) & 0xfU) != kSweepSingle)
+                    ok = false;
+            }
+            if (ok && st >= 0 && size_t(st) < stage_maps_.size()) sweep_stage_[g] = st;
+        }
+
         maps_.alloc(std::max<size_t>(1, all.size()) * sizeof(WgMap));
         if (!all.empty()) HIP_OK(hipMemcpy(maps_.p, all.data(), all.size() * sizeof(WgMap), hipMemcpyHostToDevice));
         maps_dirty_ = false;
@@ -2276,16 +2411,32 @@ private:
             profEnd();
         }
         if (pairs) {
-            // The fused sweep: stage pairs in one launch each (k_sweep).
+            // The paired sweep: two stages per launch on LDS tiles (k_sweep_pair).
             for (size_t g = 0; g < sweep_maps_.size(); ++g) {
                 const MapRef &m = sweep_maps_[g];
+                const uint32_t lds = sweep_lds_[g];
+                const bool last = own && g + 1 == sweep_maps_.size();
                 profBegin(KSolve, sweep_phase_[g]);
-                if (own && g + 1 == sweep_maps_.size())
-                    BLX_SIGNAL((k_sweep<R, DIRECT, false>), tiles(m), dim3(kWG), stream_, ev_sweep_, batch(m), S);
-                else if (plastic)
-                    hipLaunchKernelGGL((k_sweep<R, DIRECT, true>), tiles(m), dim3(kWG), 0, stream_, batch(m), S);
-                else
-                    hipLaunchKernelGGL((k_sweep<R, DIRECT, false>), tiles(m), dim3(kWG), 0, stream_, batch(m), S);
+                const dim3 grid(std::max(1U, xcdGrid(m.count / 2)));
+                if (sweep_stage_[g] >= 0) {
+                    const MapRef &sm = stage_maps_[size_t(sweep_stage_[g])];
+                    if (last)
+                        BLX_SIGNAL((k_solve<R, DIRECT, false>), tiles(sm), dim3(kWG), stream_, ev_sweep_, batch(sm), S,
+                                   kSolveFirst);
+                    else if (plastic)
+                        hipLaunchKernelGGL((k_solve<R, DIRECT, true>), tiles(sm), dim3(kWG), 0, stream_, batch(sm), S,
+                                           kSolveFirst);
+                    else
+                        hipLaunchKernelGGL((k_solve<R, DIRECT, false>), tiles(sm), dim3(kWG), 0, stream_, batch(sm), S,
+                                           kSolveFirst);
+                } else if (last) {
+                    BLX_SIGNAL_LDS((k_sweep_pair<R, DIRECT, false>), grid, dim3(kPairThreads), lds, stream_, ev_sweep_,
+                                   batch(m), S);
+                } else if (plastic) {
+                    hipLaunchKernelGGL((k_sweep_pair<R, DIRECT, true>), grid, dim3(kPairThreads), lds, stream_, batch(m), S);
+                } else {
+                    hipLaunchKernelGGL((k_sweep_pair<R, DIRECT, false>), grid, dim3(kPairThreads), lds, stream_, batch(m), S);
+                }
                 profEnd();
             }
         }
@@ -2318,30 +2469,44 @@ private:
                 BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
         if (own) {
-            // Striker pass 2 follows the sweep directly and takes its listed
-            // candidates' velocity update itself. The velocity update of the
-            // rest and the end sample of what pass 2 cannot reach run on the
-            // early stream; the strain of the affected nodes waits for both.
-            HIP_OK(hipStreamWaitEvent(early_, ev_start_, 0));
+            // Striker pass 2 forks onto the early stream after the sweep and
+            // takes its listed candidates' velocity update itself; the strain
+            // of the nodes it can reach follows it there. The velocity update
+            // of the rest, the end sample of what pass 2 cannot reach and the
+            // affected bonds' sample stay on stream_, the longer branch, which
+            // joins the early stream only before that last sample: a join the
+            // early stream has usually reached by then, so stream_ does not
+            // wait on a cross-stream signal at either end of the branch. The
+            // host enqueues k_velocity (which signals ev_velocity_) before the
+            // early stream's wait on it.
             HIP_OK(hipStreamWaitEvent(early_, ev_sweep_, 0));
-            profBegin(KVelocity, 8, early_);
+            profBegin(KPass2, 10, early_);
+            hipLaunchKernelGGL((k_pass<R, 2, true>), fg, dim3(kPass2Threads), 0, early_, lats, S);
+            profEnd(early_);
+            HIP_OK(hipStreamWaitEvent(stream_, ev_start_, 0));
+            profBegin(KVelocity, 8);
             const Batch<R> mark = batch(mark_map_);
             if (S.plane_count > 0 || S.blade.enabled)
-                BLX_SIGNAL((k_velocity<R, true, true>), ng, dim3(kWG), early_, ev_velocity_, nodes, S, mark);
-            else BLX_SIGNAL((k_velocity<R, false, true>), ng, dim3(kWG), early_, ev_velocity_, nodes, S, mark);
+                BLX_SIGNAL((k_velocity<R, true, true>), ng, dim3(kWG), stream_, ev_velocity_, nodes, S, mark);
+            else BLX_SIGNAL((k_velocity<R, false, true>), ng, dim3(kWG), stream_, ev_velocity_, nodes, S, mark);
+            profEnd();
+            HIP_OK(hipStreamWaitEvent(early_, ev_velocity_, 0));
+            profBegin(KStrainLate, 11, early_);
+            hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), sg, dim3(kWG), 0, early_, small);
             profEnd(early_);
-            profBegin(KStrain, 11, early_);
-            hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainUnaffected>), ng, dim3(kWG), 0, early_, nodes);
+            BLX_LAUNCH(KStrain, 11, BLX_SIGNAL((k_strain<R, DIRECT, kStrainUnaffected>), ng, dim3(kWG), stream_, ev_strain_, nodes));
+            // The affected bonds' sample reads the strains of both ends, so it
+            // follows both strain passes; it shares no bond with the unaffected
+            // sample (whose shared words it updates only with atomics), so the
+            // two run side by side and stream_ joins the early stream once,
+            // before the finish.
+            HIP_OK(hipStreamWaitEvent(early_, ev_strain_, 0));
+            profBegin(KBondEndLate, 12, early_);
+            BLX_SIGNAL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), early_, ev_early_, small, S);
             profEnd(early_);
-            profBegin(KBondEnd, 12, early_);
-            BLX_SIGNAL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), early_, ev_early_, bonds, S);
-            profEnd(early_);
-            BLX_LAUNCH(KPass2, 10, hipLaunchKernelGGL((k_pass<R, 2, true>), fg, dim3(kPass2Threads), 0, stream_, lats, S));
-            HIP_OK(hipStreamWaitEvent(stream_, ev_velocity_, 0));
-            BLX_LAUNCH(KStrainLate, 11, hipLaunchKernelGGL((k_strain<R, DIRECT, kStrainAffected>), sg, dim3(kWG), 0, stream_, small));
+            BLX_LAUNCH(KBondEnd, 12,
+                       BLX_SIGNAL((k_bond_end<R, DIRECT, false, kBondUnaffected>), bg, dim3(kWG), stream_, ev_bond_end_, bonds, S));
             HIP_OK(hipStreamWaitEvent(stream_, ev_early_, 0));
-            BLX_LAUNCH(KBondEndLate, 12,
-                       BLX_SIGNAL((k_bond_end<R, DIRECT, false, kBondAffected>), sg, dim3(kWG), stream_, ev_bond_end_, small, S));
             HIP_OK(hipGetLastError());
             return;
         }
@@ -2403,6 +2568,9 @@ private:
         // on side_.
         if (!split) HIP_OK(hipEventRecord(ev_bond_end_, stream_));
         HIP_OK(hipStreamWaitEvent(side_, ev_bond_end_, 0));
+        // Striker pass 2's branch ends with the affected bonds' sample on the
+        // early stream, beside the unaffected one.
+        if (split && !(S.damping > R(0))) HIP_OK(hipStreamWaitEvent(side_, ev_early_, 0));
         profBegin(KRinv, 1, side_);
         BLX_SIGNAL((k_rinv<R>), lg, dim3(kWG), side_, ev_rinv_, list);
         profEnd(side_);
@@ -2556,11 +2724,18 @@ uint32_t blx_multiprocessor_count(void) {
 
 // ---- single-lattice backend: an engine with one slot ----------------------
 
+// A single lattice's default tuning: stage pairs for a lattice too large for
+// LDS, the whole sweep in one workgroup for one that fits.
+constexpr blx_tuning kBackendTuning{2U, 1U, 1024U, 0U};
+
 int blx_backend_create(int device, int precision, const blx_schedule_view *sv, blx_backend **out) {
     return guard([&] {
         if (!sv || !out) throw Invalid("null argument");
         std::unique_ptr<Handle> h(makeHandle(device, precision, 1));
-        dispatch(h.get(), [&](auto &e) { e.setSchedule(0, scheduleOf(*sv)); });
+        dispatch(h.get(), [&](auto &e) {
+            e.setSchedule(0, scheduleOf(*sv));
+            e.setTuning(kBackendTuning);
+        });
         *out = reinterpret_cast<blx_backend *>(h.release());
     });
 }

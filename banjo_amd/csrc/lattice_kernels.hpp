@@ -106,9 +106,11 @@ template <class R> struct DevLattice {
     u32 N, B, D, blocks, node_tiles, bond_tiles, group_words;
     u32 tensile_only; // every bond's compression and shear ramps are off: thr2 replaces thr
     R4<R> *u4;        // displacement, w = inv_mass
-    R4<R> *u4b;       // the fused sweep's second node buffer
-    const uint4 *tasks;  // fused sweep: per stage-(s + 1) bond, its stage-s bonds and their far ends
-    const u32 *extras;   // fused sweep: leftover stage-s bonds and untouched nodes (| kExtraCopy)
+    R4<R> *u4b;       // the paired sweep's second node buffer
+    // The paired sweep's node lists (per workgroup its owned nodes, then the
+    // halo) and stage bond lists.
+    const u32 *pnodes;
+    const PairBond<R> *pbonds;
     const uint2 *stage_ranges;  // the colour stages as bond ranges [x, y), in sweep order
     const u32 *stage_end;       // stage s is stage_ranges[stage_end[s - 1] .. stage_end[s])
     u32 stage_count;
@@ -1067,17 +1069,18 @@ __global__ __launch_bounds__(PASS == 1 ? kFinishThreads : kPass2Threads) void k_
 // stage's bonds share no node, so every thread owns its two ends.
 
 // The arithmetic of one bond solve (LatticePhysics.hpp:1133-1168) on node
-// values held in registers: ua and ub move when the bond solves. lam is the
-// multiplier it reads (0 in the first iteration); a solve stores lam + dl in
-// lam_out and returns true, a bond shorter than 1e-12 returns false.
+// values held in registers (or LDS): ua and ub move when the bond solves. g is
+// the bond's edge palette entry, lam the multiplier it reads (0 in the first
+// iteration); a solve stores lam + dl in lam_out and returns true, a bond
+// shorter than 1e-12 returns false. r.a and r.b (the global node indices) are
+// read only with DIRECT, c always and p only through g.
 template <class R, bool DIRECT, bool PLASTIC>
-__device__ __forceinline__ bool solveBond(const DevLattice<R> &L, const Step<R> &S, const BondRec<R> &r, u32 j, R lam,
-                                          R4<R> &ua, R4<R> &ub, R &lam_out) {
-    const EdgePal<R> g = ldg(&L.epal[r.p & ~kNoDir]);
+__device__ __forceinline__ bool solveBondWith(const DevLattice<R> &L, const Step<R> &S, const EdgePal<R> &g, u32 a,
+                                              u32 b, R compliance, u32 j, R lam, R4<R> &ua, R4<R> &ub, R &lam_out) {
     const R4<R> ge = g.e;
     const V3<R> du = xyz(ub) - xyz(ua);
     const V3<R> e = xyz(ge);
-    const V3<R> delta = DIRECT ? (xyz(ldg(&L.x04[r.b])) + xyz(ub)) - (xyz(ldg(&L.x04[r.a])) + xyz(ua)) : e + du;
+    const V3<R> delta = DIRECT ? (xyz(ldg(&L.x04[b])) + xyz(ub)) - (xyz(ldg(&L.x04[a])) + xyz(ua)) : e + du;
     const R len = length(delta);
     if (len <= R(1.0e-12)) return false;
     const R orig = ge.w;
@@ -1092,12 +1095,19 @@ __device__ __forceinline__ bool solveBond(const DevLattice<R> &L, const Step<R> 
     }
     const V3<R> dir = delta / len;
     const R wa = ua.w, wb = ub.w;
-    const R alpha = r.c / (S.dt * S.dt);
+    const R alpha = compliance / (S.dt * S.dt);
     const R dl = (-C - alpha * lam) / (wa + wb + alpha);
     lam_out = lam + dl;
     ua = with(xyz(ua) - wa * dl * dir, ua.w);
     ub = with(xyz(ub) + wb * dl * dir, ub.w);
     return true;
+}
+
+template <class R, bool DIRECT, bool PLASTIC>
+__device__ __forceinline__ bool solveBond(const DevLattice<R> &L, const Step<R> &S, const BondRec<R> &r, u32 j, R lam,
+                                          R4<R> &ua, R4<R> &ub, R &lam_out) {
+    const EdgePal<R> g = ldg(&L.epal[r.p & ~kNoDir]);
+    return solveBondWith<R, DIRECT, PLASTIC>(L, S, g, r.a, r.b, r.c, j, lam, ua, ub, lam_out);
 }
 
 template <class R, bool DIRECT, bool PLASTIC>
@@ -1124,102 +1134,129 @@ __global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags)
     stStream(&L.u4[r.b], ub);
 }
 
-// The fused colour sweep of a one-iteration run: two consecutive colour
-// stages s, s + 1 in one launch. Each task owns one bond of stage s + 1 and
-// first solves the stage-s bonds at its two ends (at most one each: a stage
-// shares no node), then its own bond, all on register copies of the nodes.
-// A stage-s bond between the ends of two tasks is solved by both, from the
-// same inputs, so both get the bits the stage-by-stage sweep gives. Tasks read
-// one node buffer and write the other (u4 <-> u4b), so no task reads a node
-// another task has already moved. Every node is written once: the ends of the
-// stage-(s + 1) bonds by their task; the far end of a stage-s bond whose far
-// end is in no stage-(s + 1) bond by the task at its near end (kTaskWrite);
-// the ends of a stage-s bond with neither end in stage s + 1, and the nodes in
-// neither stage, by extra entries. Pairs alternate direction, and a sweep has
-// an even number of them, so it ends in u4; a stage left over from the pairing
-// runs in place on u4 like k_solve. Map entries (one per workgroup):
-//   single: a, b = bond range of one stage, in place on u4;
-//   pair:   a = first task, b = schedule index of its stage-(s + 1) bond;
-//   extra:  a = first entry of extras.
-// pad = count (bits 0-15) | kind (16-19) | direction (bit 20: u4b -> u4).
-constexpr u32 kSweepSingle = 0U, kSweepPair = 1U, kSweepExtra = 2U;
+// The paired colour sweep of a one-iteration run: two consecutive colour
+// stages s, s + 1 in one launch, each workgroup on a tile of nodes staged in
+// LDS. A workgroup owns a set of nodes and loads, besides them, every node the
+// owned nodes' final values depend on: for an owned node x with stage-s bond
+// partner p1(x) and stage-(s + 1) partner p2(x), those are p1(x), p2(x) and
+// p1(p2(x)). It solves every stage-s bond at x or at p2(x), then every
+// stage-(s + 1) bond at x, on its LDS copies (a stage shares no node, so no
+// two threads touch one slot), and writes back only the owned nodes. A bond
+// solved by two workgroups is solved from the same inputs in both, so each
+// owned node gets the bits the stage-by-stage sweep gives. Workgroups read
+// one node buffer and write the other (u4 <-> u4b), so none reads a node
+// another has already written; every node is owned by one workgroup, so the
+// written buffer is whole. Pairs alternate direction and a sweep has an even
+// number of them, so it ends in u4; the stages left over run in place on u4
+// as single entries. Each workgroup has two map entries, read together:
+//   single: {lat, a, b, pad}: a, b = bond range of one stage (at most
+//           kPairThreads bonds); the second entry unused;
+//   pair:   {lat, node list offset, bond list offset, pad},
+//           {owned | all nodes << 16, stage-s | stage-(s + 1) bonds << 16, 0, 0};
+// pad = kind (bits 16-19) | direction (bit 20: u4b -> u4). A launch has
+// xcdGrid(count / 2) workgroups.
+constexpr u32 kSweepSingle = 0U, kSweepPair = 1U;
 constexpr u32 kSweepReverse = 1U << 20;
-constexpr u32 kNoTaskBond = 0x7fffffffU;  // a task end with no stage-s bond
-constexpr u32 kTaskWrite = 0x80000000U;   // the task writes that stage-s bond's far end
-constexpr u32 kExtraCopy = 0x80000000U;   // an extra entry that copies node (e & ~kExtraCopy)
+constexpr u32 kPairThreads = 512;
+constexpr u32 kPairNodeRegs = 4;  // node slots per thread: a workgroup holds at most 2048 nodes
+constexpr u32 kPairBondRegs = 2;  // bonds per thread per stage: at most 1024 per stage
+constexpr u32 kPairMaxNodes = kPairThreads * kPairNodeRegs;
+// A workgroup's tile in cells of the reference grid (x, y, z): 1,024 owned
+// nodes of a full lattice, rows of 16 nodes along the fastest axis of the box
+// lattices' node order; lattices below kPairWideNodes take 512-node tiles, so
+// that a launch still has several workgroups per CU.
+constexpr u32 kPairTile[3] = {16, 8, 8};
+constexpr u32 kPairTileSmall[3] = {8, 8, 8};
+constexpr u32 kPairWideNodes = 1U << 19;
+constexpr u32 kPairMaxBonds = kPairThreads * kPairBondRegs;
 
 template <class R, bool DIRECT, bool PLASTIC>
-__global__ __launch_bounds__(kWG) void k_sweep(Batch<R> B, Step<R> S) {
-    const WgMap *m = tileEntry(B);
-    if (!m) return;
+__device__ __forceinline__ void pairStage(const DevLattice<R> &L, const Step<R> &S, const u32 *nodes, R4<R> *u,
+                                          const PairBond<R> (&e)[kPairBondRegs], const u8 (&live)[kPairBondRegs],
+                                          u32 n) {
+#pragma unroll
+    for (u32 q = 0; q < kPairBondRegs; ++q) {
+        if (threadIdx.x + q * kPairThreads >= n || !live[q]) continue;
+        const EdgePal<R> g = ldg(&L.epal[e[q].p & ~kNoDir]);
+        const u32 sa = e[q].slots & 0xffffU, sb = e[q].slots >> 16;
+        R4<R> ua = u[sa], ub = u[sb];
+        const u32 a = DIRECT ? ldg(&nodes[sa]) : 0U, b = DIRECT ? ldg(&nodes[sb]) : 0U;
+        R lam_out;
+        if (!solveBondWith<R, DIRECT, PLASTIC>(L, S, g, a, b, e[q].c, e[q].j, R(0), ua, ub, lam_out)) continue;
+        u[sa] = ua;
+        u[sb] = ub;
+    }
+}
+
+// Eight waves per SIMD in float: the launch's workgroups of a full lattice
+// are all resident at once, so none waits for a second round.
+template <class R, bool DIRECT, bool PLASTIC>
+__global__ __launch_bounds__(kPairThreads, sizeof(R) == 4 ? 8 : 4) void k_sweep_pair(Batch<R> B, Step<R> S) {
+    extern __shared__ __align__(16) unsigned char pair_lds[];
+    R4<R> *u = reinterpret_cast<R4<R> *>(pair_lds);
+    const u32 wgs = B.count / 2, w = xcdTile(blockIdx.x, wgs);
+    if (w >= wgs) return;
+    const WgMap m0 = ldg(&B.map[2 * w]), m1 = ldg(&B.map[2 * w + 1]);
+    const WgMap *m = &m0;
     const DevLattice<R> L = B.L[m->lat];
-    const u32 kind = (m->pad >> 16) & 0xfU;
-    R lam_out;
-    if (kind == kSweepSingle) {
-        const u32 j = m->a + threadIdx.x;
+    const u32 t = threadIdx.x;
+    if (((m->pad >> 16) & 0xfU) == kSweepSingle) {
+        const u32 j = m->a + t;
         if (j >= m->b) return;
         const BondRec<R> r = ldStream(&L.rec[j]);
         const u8 live = ldStream(&L.alive[j]);
         if (L.ctl->stop || !live) return;
         R4<R> ua = ldg(&L.u4[r.a]), ub = ldg(&L.u4[r.b]);
+        R lam_out;
         if (!solveBond<R, DIRECT, PLASTIC>(L, S, r, j, R(0), ua, ub, lam_out)) return;
         stStream(&L.u4[r.a], ua);
         stStream(&L.u4[r.b], ub);
         return;
     }
-    if (threadIdx.x >= (m->pad & 0xffffU)) return;
     const bool rev = m->pad & kSweepReverse;
     const R4<R> *src = rev ? L.u4b : L.u4;
     R4<R> *dst = rev ? L.u4 : L.u4b;
-    if (kind == kSweepPair) {
-        const uint4 t = ldStream(&L.tasks[m->a + threadIdx.x]);
-        const u32 j2 = m->b + threadIdx.x;
-        const BondRec<R> r2 = ldStream(&L.rec[j2]);
-        const u8 live2 = ldStream(&L.alive[j2]);
-        const u32 ja = t.x & kNoTaskBond, jb = t.y & kNoTaskBond;
-        BondRec<R> ra{}, rb{};
-        u8 live_a = 0, live_b = 0;
-        if (ja != kNoTaskBond) {
-            ra = ldg(&L.rec[ja]);
-            live_a = ldg(&L.alive[ja]);
-        }
-        if (jb != kNoTaskBond) {
-            rb = ldg(&L.rec[jb]);
-            live_b = ldg(&L.alive[jb]);
-        }
-        if (L.ctl->stop) return;
-        R4<R> ua = ldg(&src[r2.a]), ub = ldg(&src[r2.b]);
-        R4<R> fa = ja != kNoTaskBond ? ldg(&src[t.z]) : R4<R>{}, fb = jb != kNoTaskBond ? ldg(&src[t.w]) : R4<R>{};
-        // Stage s at each end: the bond's own a side is whichever of its ends
-        // the task holds there.
-        if (live_a) {
-            if (ra.a == r2.a) solveBond<R, DIRECT, PLASTIC>(L, S, ra, ja, R(0), ua, fa, lam_out);
-            else solveBond<R, DIRECT, PLASTIC>(L, S, ra, ja, R(0), fa, ua, lam_out);
-        }
-        if (live_b) {
-            if (rb.a == r2.b) solveBond<R, DIRECT, PLASTIC>(L, S, rb, jb, R(0), ub, fb, lam_out);
-            else solveBond<R, DIRECT, PLASTIC>(L, S, rb, jb, R(0), fb, ub, lam_out);
-        }
-        if (live2) solveBond<R, DIRECT, PLASTIC>(L, S, r2, j2, R(0), ua, ub, lam_out);
-        stStream(&dst[r2.a], ua);
-        stStream(&dst[r2.b], ub);
-        if (t.x & kTaskWrite) stStream(&dst[t.z], fa);
-        if (t.y & kTaskWrite) stStream(&dst[t.w], fb);
-        return;
+    // The node ids and bond entries go out with the stop flag.
+    const u32 n_own = m1.lat & 0xffffU, n_all = m1.lat >> 16, n1 = m1.a & 0xffffU, n2 = m1.a >> 16;
+    const u32 *nodes = L.pnodes + m->a;
+    const PairBond<R> *bonds = L.pbonds + m->b;
+    u32 id[kPairNodeRegs];
+#pragma unroll
+    for (u32 q = 0; q < kPairNodeRegs; ++q) {
+        const u32 k = t + q * kPairThreads;
+        id[q] = k < n_all ? ldStream(&nodes[k]) : 0U;
     }
-    const u32 e = ldg(&L.extras[m->a + threadIdx.x]);
+    PairBond<R> e1[kPairBondRegs] = {}, e2[kPairBondRegs] = {};
+    u8 l1[kPairBondRegs] = {}, l2[kPairBondRegs] = {};
+#pragma unroll
+    for (u32 q = 0; q < kPairBondRegs; ++q) {
+        const u32 k = t + q * kPairThreads;
+        if (k < n1) e1[q] = ldStream(&bonds[k]);
+        if (k < n2) e2[q] = ldStream(&bonds[n1 + k]);
+    }
+    // Only the end of a substep (k_finish) sets stop, so it holds for the launch.
     if (L.ctl->stop) return;
-    if (e & kExtraCopy) {
-        const u32 v = e & ~kExtraCopy;
-        stStream(&dst[v], ldg(&src[v]));
-        return;
+#pragma unroll
+    for (u32 q = 0; q < kPairNodeRegs; ++q) {
+        const u32 k = t + q * kPairThreads;
+        if (k < n_all) u[k] = ldg(&src[id[q]]);
     }
-    const BondRec<R> r = ldg(&L.rec[e]);
-    const u8 live = ldg(&L.alive[e]);
-    R4<R> ua = ldg(&src[r.a]), ub = ldg(&src[r.b]);
-    if (live) solveBond<R, DIRECT, PLASTIC>(L, S, r, e, R(0), ua, ub, lam_out);
-    stStream(&dst[r.a], ua);
-    stStream(&dst[r.b], ub);
+#pragma unroll
+    for (u32 q = 0; q < kPairBondRegs; ++q) {
+        const u32 k = t + q * kPairThreads;
+        if (k < n1) l1[q] = ldg(&L.alive[e1[q].j]);
+        if (k < n2) l2[q] = ldg(&L.alive[e2[q].j]);
+    }
+    __syncthreads();
+    pairStage<R, DIRECT, PLASTIC>(L, S, nodes, u, e1, l1, n1);
+    __syncthreads();
+    pairStage<R, DIRECT, PLASTIC>(L, S, nodes, u, e2, l2, n2);
+    __syncthreads();
+#pragma unroll
+    for (u32 q = 0; q < kPairNodeRegs; ++q) {
+        const u32 k = t + q * kPairThreads;
+        if (k < n_own) stStream(&dst[id[q]], u[k]);
+    }
 }
 
 // The whole colour sweep of a one-iteration run for a lattice whose nodes fit

@@ -92,6 +92,13 @@ def measured_traffic(kernel):
             "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
 
 
+def tuned(obj, args):
+    """Applies the --tuning fields (blx_tuning) to a backend or batch."""
+    if args.tuning:
+        obj.set_tuning(**json.loads(args.tuning))
+    return obj
+
+
 def make_workload(args):
     from banjo_amd import scenes
     sc = scenes.impact(128, 64, 128, substeps=args.steps * SUBSTEPS_PER_STEP, blocks=args.blocks,
@@ -108,7 +115,7 @@ def gpu_leg(args, sc):
     if not hip.lib.blx_available():
         raise SystemExit("bench.py: no gfx950 device visible to the HIP lattice library")
     sched, state = prepare(hip, sc)
-    be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+    be = tuned(hip.backend(abi.PRECISION_FLOAT, sched, 0), args)
     substeps = args.steps * SUBSTEPS_PER_STEP
     if args.warmup:
         be.upload(state, sc.settings, sc.sphere)
@@ -167,13 +174,23 @@ def batched_leg(args):
     scs = [scenes.config3_trial(t, edges=edges[t], substeps=sub) for t in range(T)]
     prepared = [prepare(hip, sc) for sc in scs]
     t_build = time.perf_counter() - t0
-    bt = Batch(hip, abi.PRECISION_FLOAT, T)
+    bt = tuned(Batch(hip, abi.PRECISION_FLOAT, T), args)
     t0 = time.perf_counter()
     for k, (sc, (sched, state)) in enumerate(zip(scs, prepared)):
         bt.upload(k, sched, state, sc.settings, sc.sphere)
         bt.set_external_forces(k, sc.forces, sc.force_substeps)
         bt.arm(k, control(sub))
     t_upload = time.perf_counter() - t0
+    # The autotuner picks colour stages per launch, the LDS sweep and its
+    # workgroup size, and substeps per graph from probes of this batch (and
+    # puts every lattice back as it was).
+    tune = None
+    if args.batch_autotune:
+        warm, probe = (int(x) for x in args.batch_autotune.split(","))
+        t0 = time.perf_counter()
+        rep = bt.autotune(warm, probe)
+        tune = {"chosen": rep["chosen"], "candidates": len(rep["tried"]), "warm_substeps": warm,
+                "probe_substeps": probe, "seconds": time.perf_counter() - t0}
     t0 = time.perf_counter()
     while bt.advance(1 << 62, args.batch_steps_per_launch):
         pass
@@ -185,7 +202,7 @@ def batched_leg(args):
     identical = True
     for k in sample:
         sc, (sched, state) = scs[k], prepared[k]
-        be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+        be = tuned(hip.backend(abi.PRECISION_FLOAT, sched, 0), args)
         be.upload(state, sc.settings, sc.sphere)
         be.set_external_forces(sc.forces, sc.force_substeps)
         t0 = time.perf_counter()
@@ -203,7 +220,7 @@ def batched_leg(args):
             "broken_bonds": broken, "seconds": t_run, "trials_per_s": T / t_run,
             "bond_solves_per_s": bonds * sub / t_run, "plain_trials_per_s": plain_tps,
             "speedup_vs_plain": (T / t_run) / plain_tps, "plain_sample": len(sample),
-            "identical_to_plain": identical, "host_build_s": t_build, "upload_s": t_upload,
+            "identical_to_plain": identical, "host_build_s": t_build, "upload_s": t_upload, "autotune": tune,
             "workload": "cantilevers, edges log-uniform in [8, 48] per axis, 26-neighbour, Weibull(6, 1.5%) "
                         "break strain, log-uniform compliance, tip load U(1, 50) N, dt 1/1920 s"}
 
@@ -224,6 +241,7 @@ def served_leg(args):
     lats = [scenes.served_lattice(e, seed=4242 + k) for k, e in enumerate(edges)]
     server = TrialServer(hip, lats, slots=args.serve_slots, hbm_budget_bytes=args.serve_hbm_gib << 30,
                          chunk=args.serve_chunk)
+    tuned(server.session.batch, args)
     t_setup = time.perf_counter() - t0
     stream = scenes.served_stream(971, args.serve_rate, args.serve_seconds, pool)
     trials = [Trial(id=i, arrival=a, template=k, sphere=scenes.served_marble(edges[k], v, tilt, az),
@@ -399,7 +417,7 @@ def contact_leg(args):
         if mode:
             scenes.with_node_contact(sc.settings, sc.cell, sc.node_count, mode)
         sched, state = prepare(hip, sc)
-        be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+        be = tuned(hip.backend(abi.PRECISION_FLOAT, sched, 0), args)
         be.upload(state, sc.settings, sc.sphere)
         be.set_blade(sc.info["blade"])
         t0 = time.perf_counter()
@@ -429,7 +447,7 @@ def c1_leg(args):
     hip = product()
     sc = scenes.config1()
     sched, state = prepare(hip, sc)
-    be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+    be = tuned(hip.backend(abi.PRECISION_FLOAT, sched, 0), args)
     out = {}
     for label in ("warm", "timed"):
         be.upload(state, sc.settings, sc.sphere)
@@ -460,7 +478,7 @@ def cut_leg(args):
     sc = scenes.config5(substeps=steps * per)
     sched, state = prepare(hip, sc)
     t_build = time.perf_counter() - t0
-    be = hip.backend(abi.PRECISION_FLOAT, sched, 0)
+    be = tuned(hip.backend(abi.PRECISION_FLOAT, sched, 0), args)
     be.upload(state, sc.settings, sc.sphere)
     be.set_blade(sc.info["blade"])
     t_sim = t_label = 0.0
@@ -521,6 +539,10 @@ def main():
     ap.add_argument("--batch-trials", type=int, default=512)
     ap.add_argument("--batch-substeps", type=int, default=8 * 240)
     ap.add_argument("--batch-steps-per-launch", type=int, default=0)
+    ap.add_argument("--batch-autotune", default="8,24",
+                    help="warm,probe substeps of the batched form's autotuner run before its timed region ('' for none)")
+    ap.add_argument("--tuning", default="",
+                    help="launch tuning fields (blx_tuning) as JSON, set on every backend and batch")
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=60.0)

@@ -580,10 +580,11 @@ int blx_backend_kernel_stats(const blx_backend *backend, uint32_t index, char *n
 /* Launch tuning of a backend or a batch. No counterpart in the reference,
  * whose CUDA lane fixed its launch shape when it was built. Every setting
  * gives the same bits; they differ in speed, which depends on the lattices'
- * sizes. Defaults: 1, 1, 1024, 0.
+ * sizes. Defaults: a backend 2, 1, 1024, 0; a batch 1, 0, 1024, 0.
  *  colours_per_launch  colour stages per sweep launch in a one-iteration run:
- *                      1, or 2 (stage pairs, each bond of the second stage
- *                      solving the first stage's bonds at its ends itself);
+ *                      1, or 2 (stage pairs: each workgroup holds a tile of
+ *                      nodes and the halo its two stages reach in LDS, solves
+ *                      both stages there and writes the tile back once);
  *  lds_sweep           nonzero: in a one-iteration run, a lattice whose nodes
  *                      fit in one workgroup's LDS (RunStatus
  *                      shared_memory_bytes) runs its whole sweep in one

@@ -47,6 +47,13 @@ def scene(case):
         return scenes.config1(substeps=160)
     if case == "large":
         return scenes.small_impact(nx=28, ny=12, nz=28, substeps=60)
+    if case == "prestressed":
+        # Rest lengths 1.6 times the node spacing: the paired sweep's tiles
+        # (16 x 8 x 8 cells of the shortest rest length) then hold 2,880
+        # nodes each, more than a workgroup, and its plan splits them in halves.
+        sc = scenes.small_impact(nx=24, ny=10, nz=24, substeps=40)
+        sc.asset.rest_length = sc.asset.rest_length * 1.6
+        return sc
     raise KeyError(case)
 
 
@@ -61,7 +68,7 @@ def oracle_runs(oracle):
 
 
 @pytest.mark.parametrize("precision", [F, D], ids=["float", "double"])
-@pytest.mark.parametrize("case", ["impact", "impact_3slabs", "plastic", "direct", "config1", "large"])
+@pytest.mark.parametrize("case", ["impact", "impact_3slabs", "plastic", "direct", "config1", "large", "prestressed"])
 def test_every_tuning_equals_the_oracle(gpu, oracle, oracle_runs, case, precision):
     sc = scene(case)
     key = (case, precision)
@@ -165,6 +172,6 @@ def test_tuning_is_validated(gpu):
         for bad in ({"colours_per_launch": 3}, {"lds_threads": 100}, {"graph_substeps": 1000}):
             with pytest.raises(Exception):
                 be.set_tuning(**bad)
-        assert be.tuning() == {"colours_per_launch": 1, "lds_sweep": 1, "lds_threads": 1024, "graph_substeps": 0}
+        assert be.tuning() == {"colours_per_launch": 2, "lds_sweep": 1, "lds_threads": 1024, "graph_substeps": 0}
     finally:
         be.close()
