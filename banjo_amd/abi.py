@@ -145,7 +145,8 @@ class RunStatus(C.Structure):
 
 class Tuning(C.Structure):
     """blx_tuning: launch tuning of a backend or batch (same bits for every setting)."""
-    _fields_ = [("colours_per_launch", u32), ("lds_sweep", u32), ("lds_threads", u32), ("graph_substeps", u32)]
+    _fields_ = [("colours_per_launch", u32), ("lds_sweep", u32), ("lds_threads", u32), ("graph_substeps", u32),
+                ("streams", u32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

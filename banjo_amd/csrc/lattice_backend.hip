@@ -1420,6 +1420,7 @@ public:
         if (t.lds_threads != 256 && t.lds_threads != 512 && t.lds_threads != 1024)
             throw Invalid("lds_threads is 256, 512 or 1024");
         if (t.graph_substeps > kMaxGraphSubsteps) throw Invalid("graph_substeps is at most 256");
+        if (t.streams > 2) throw Invalid("streams is 0, 1 or 2");
         tuning_ = t;
         maps_dirty_ = true;
         dropGraph();
@@ -1510,10 +1511,11 @@ public:
         const uint32_t graphs[] = {0U, 8U, 32U};
         for (uint32_t g : graphs) {
             if (g != 0 && (g > warm + probe || S_.nc.mode != 0U)) continue;
-            for (uint32_t c = 1; c <= 2; ++c) cand.push_back(blx_tuning{c, 0U, 1024U, g});
+            for (uint32_t c = 1; c <= 2; ++c) cand.push_back(blx_tuning{c, 0U, 1024U, g, tuning_.streams});
             if (any_fits)
                 for (uint32_t th : {256U, 512U, 1024U})
-                    for (uint32_t c = 1; c <= (all_fit ? 1U : 2U); ++c) cand.push_back(blx_tuning{c, 1U, th, g});
+                    for (uint32_t c = 1; c <= (all_fit ? 1U : 2U); ++c)
+                        cand.push_back(blx_tuning{c, 1U, th, g, tuning_.streams});
         }
         if (cand.size() > BLX_TUNE_MAX) throw Logic("too many tuning candidates");
         std::vector<double> best(cand.size(), std::numeric_limits<double>::infinity());
@@ -1924,12 +1926,19 @@ private:
     std::vector<uint32_t> sweep_phase_;  // reference phase id of each group (5 interior, 6 boundary)
     std::vector<uint32_t> sweep_lds_;    // LDS bytes of each group's launch
     std::vector<int64_t> sweep_stage_;   // the stage a group of single entries runs by stage map, or -1
+    // A striker substep splits across streams (striker pass 2 beside the end
+    // sample of what it cannot reach) when a running lattice has at least
+    // kSplitNodes nodes (tuning streams 0; 1 always, 2 never): a smaller
+    // lattice's pass 2 is shorter than the cross-stream waits the split adds,
+    // so its substep stays on one stream.
+    static constexpr uint32_t kSplitNodes = 1U << 17;
+    bool split_nodes_ = true;
     // Launch tuning (blx_tuning). Every setting gives the same bits.
     // A batch's default: colour stages one per launch and no LDS sweep, the
     // faster setting for many small lattices advancing together (a lattice
     // sweeping in LDS holds its launch for the whole sweep); a single-lattice
     // backend starts from kBackendTuning instead.
-    blx_tuning tuning_{1U, 0U, 1024U, 0U};
+    blx_tuning tuning_{1U, 0U, 1024U, 0U, 0U};
     MapRef lds_map_;                     // one entry per lattice that sweeps in LDS (k_sweep_lds)
     uint32_t lds_bytes_ = 0;             // that launch's dynamic LDS: the largest such lattice's nodes
     size_t lds_limit_ = 0;               // LDS one workgroup may hold on this device
@@ -2108,6 +2117,9 @@ private:
             if (ok && st >= 0 && size_t(st) < stage_maps_.size()) sweep_stage_[g] = st;
         }
 
+        split_nodes_ = tuning_.streams == 1U;
+        if (tuning_.streams == 0U)
+            for (uint32_t k : run) split_nodes_ |= slots_[k]->N >= kSplitNodes;
         maps_.alloc(std::max<size_t>(1, all.size()) * sizeof(WgMap));
         if (!all.empty()) HIP_OK(hipMemcpy(maps_.p, all.data(), all.size() * sizeof(WgMap), hipMemcpyHostToDevice));
         maps_dirty_ = false;
@@ -2179,12 +2191,12 @@ private:
     // reference phase id it belongs to (FastLattice.cpp:22-26).
     enum Kernel : uint32_t { KStrainStart, KBondStart, KPredict, KPass1, KSolve, KProject, KVelocity, KDamp,
                              KSupportVelocity, KPass2, KStrain, KBondEnd, KFinish, KRinv, KMark, KStrainLate,
-                             KBondEndLate, KContact, KCount };
+                             KBondEndLate, KContact, KSweepPair, KCount };
     static const char *kernelName(uint32_t k) {
         static const char *names[KCount] = {"k_strain_start", "k_bond_start", "k_predict", "k_pass1", "k_solve",
                                             "k_project", "k_velocity", "k_damp", "k_support_velocity", "k_pass2",
                                             "k_strain", "k_bond_end", "k_finish", "k_rinv", "k_mark_affected",
-                                            "k_strain_late", "k_bond_end_late", "k_node_contact"};
+                                            "k_strain_late", "k_bond_end_late", "k_node_contact", "k_sweep_pair"};
         return k < KCount ? names[k] : "unknown";
     }
     struct ProfRecord { uint32_t kernel, phase; hipEvent_t a, b; };
@@ -2353,8 +2365,10 @@ private:
         const bool plastic = shared_.plastic;
         const bool contact = S.nc.mode != 0U;
         // With a striker, the end sample of what pass 2 cannot reach overlaps
-        // pass 2, unless node contact moves nodes between pass 2 and the sample.
-        const bool split = sphere && !plastic && !contact;
+        // pass 2, unless node contact moves nodes between pass 2 and the sample,
+        // or no running lattice is large enough for the overlap to repay the
+        // cross-stream waits it adds (split_nodes_).
+        const bool split = sphere && !plastic && !contact && split_nodes_;
         // Without damping between the velocity update and the end sample,
         // striker pass 2 need not wait for the velocity update of the rest.
         const bool own = split && !(S.damping > R(0));
@@ -2416,7 +2430,7 @@ private:
                 const MapRef &m = sweep_maps_[g];
                 const uint32_t lds = sweep_lds_[g];
                 const bool last = own && g + 1 == sweep_maps_.size();
-                profBegin(KSolve, sweep_phase_[g]);
+                profBegin(sweep_stage_[g] >= 0 ? KSolve : KSweepPair, sweep_phase_[g]);
                 const dim3 grid(std::max(1U, xcdGrid(m.count / 2)));
                 if (sweep_stage_[g] >= 0) {
                     const MapRef &sm = stage_maps_[size_t(sweep_stage_[g])];
@@ -2549,7 +2563,7 @@ private:
         const bool sphere = S.sphere_enabled != 0;
         const bool plastic = shared_.plastic;
         const bool contact = S.nc.mode != 0U;
-        const bool split = sphere && !plastic && !contact;
+        const bool split = sphere && !plastic && !contact && split_nodes_;
         const Batch<R> nodes = batch(node_tiles_map_), bonds = batch(bond_tiles_map_), list = batch(list_map_),
                        lats = batch(lattice_map_);
         const dim3 ng = tiles(node_tiles_map_), bg = tiles(bond_tiles_map_), lg = entries(list_map_),
@@ -2726,7 +2740,7 @@ uint32_t blx_multiprocessor_count(void) {
 
 // A single lattice's default tuning: stage pairs for a lattice too large for
 // LDS, the whole sweep in one workgroup for one that fits.
-constexpr blx_tuning kBackendTuning{2U, 1U, 1024U, 0U};
+constexpr blx_tuning kBackendTuning{2U, 1U, 1024U, 0U, 0U};
 
 int blx_backend_create(int device, int precision, const blx_schedule_view *sv, blx_backend **out) {
     return guard([&] {

@@ -580,7 +580,7 @@ int blx_backend_kernel_stats(const blx_backend *backend, uint32_t index, char *n
 /* Launch tuning of a backend or a batch. No counterpart in the reference,
  * whose CUDA lane fixed its launch shape when it was built. Every setting
  * gives the same bits; they differ in speed, which depends on the lattices'
- * sizes. Defaults: a backend 2, 1, 1024, 0; a batch 1, 0, 1024, 0.
+ * sizes. Defaults: a backend 2, 1, 1024, 0, 0; a batch 1, 0, 1024, 0, 0.
  *  colours_per_launch  colour stages per sweep launch in a one-iteration run:
  *                      1, or 2 (stage pairs: each workgroup holds a tile of
  *                      nodes and the halo its two stages reach in LDS, solves
@@ -591,12 +591,18 @@ int blx_backend_kernel_stats(const blx_backend *backend, uint32_t index, char *n
  *                      workgroup, the nodes held in LDS;
  *  lds_threads         that workgroup's size: 256, 512 or 1024;
  *  graph_substeps      substeps per captured graph launch, 0..256 (0 enqueues
- *                      every substep; node contact always enqueues). */
+ *                      every substep; node contact always enqueues);
+ *  streams             a striker substep without plasticity or node contact:
+ *                      0 splits across streams (striker pass 2 beside the end
+ *                      sample of what it cannot reach) when a running lattice
+ *                      has at least 131,072 nodes, else runs on one stream;
+ *                      1 always splits; 2 never does. */
 typedef struct blx_tuning {
     uint32_t colours_per_launch;
     uint32_t lds_sweep;
     uint32_t lds_threads;
     uint32_t graph_substeps;
+    uint32_t streams;
 } blx_tuning;
 
 #define BLX_TUNE_MAX 32

@@ -21,7 +21,7 @@ from test_gpu_parity import assert_same
 
 pytestmark = pytest.mark.gpu
 F, D = abi.PRECISION_FLOAT, abi.PRECISION_DOUBLE
-STAGES = {"colours_per_launch": 1, "lds_sweep": 0, "lds_threads": 1024, "graph_substeps": 0}
+STAGES = {"colours_per_launch": 1, "lds_sweep": 0, "lds_threads": 1024, "graph_substeps": 0, "streams": 0}
 VARIANTS = {
     "stages": {},
     "pairs": {"colours_per_launch": 2},
@@ -29,6 +29,9 @@ VARIANTS = {
     "lds1024": {"lds_sweep": 1, "lds_threads": 1024},
     "lds512_graphs8": {"lds_sweep": 1, "lds_threads": 512, "graph_substeps": 8},
     "pairs_graphs32": {"colours_per_launch": 2, "graph_substeps": 32},
+    "split": {"streams": 1},
+    "pairs_split_graphs8": {"colours_per_launch": 2, "streams": 1, "graph_substeps": 8},
+    "one_stream": {"streams": 2},
 }
 
 
@@ -169,9 +172,10 @@ def test_tuning_is_validated(gpu):
     sched, state = prepare(gpu, sc)
     be = gpu.backend(F, sched, 0)
     try:
-        for bad in ({"colours_per_launch": 3}, {"lds_threads": 100}, {"graph_substeps": 1000}):
+        for bad in ({"colours_per_launch": 3}, {"lds_threads": 100}, {"graph_substeps": 1000}, {"streams": 3}):
             with pytest.raises(Exception):
                 be.set_tuning(**bad)
-        assert be.tuning() == {"colours_per_launch": 2, "lds_sweep": 1, "lds_threads": 1024, "graph_substeps": 0}
+        assert be.tuning() == {"colours_per_launch": 2, "lds_sweep": 1, "lds_threads": 1024, "graph_substeps": 0,
+                               "streams": 0}
     finally:
         be.close()
