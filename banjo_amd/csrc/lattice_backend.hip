@@ -2222,6 +2222,20 @@ private:
         HIP_OK(hipEventRecord(r.a, on));
         records_.push_back(r);
     }
+    // A record whose two events a launch carries itself (hipExtLaunchKernelGGL
+    // start and stop events), so that it times the kernel alone, as a kernel
+    // trace does; nullptr when nothing is timed.
+    ProfRecord *profLaunch(uint32_t kernel, uint32_t phase) {
+        if ((!prof_ && !sample_) || capturing_) return nullptr;
+        const size_t need = 2 * (records_.size() + 1);
+        while (pool_.size() < need) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            pool_.push_back(e);
+        }
+        records_.push_back(ProfRecord{kernel, phase, pool_[need - 2], pool_[need - 1]});
+        return &records_.back();
+    }
     void profEnd(hipStream_t on = nullptr) {
         if (!prof_ && !sample_) return;
         HIP_OK(hipEventRecord(records_.back().b, on ? on : stream_));
@@ -2426,32 +2440,38 @@ private:
         }
         if (pairs) {
             // The paired sweep: two stages per launch on LDS tiles (k_sweep_pair).
+            // When timed, each launch carries its own start and stop events.
             for (size_t g = 0; g < sweep_maps_.size(); ++g) {
                 const MapRef &m = sweep_maps_[g];
                 const uint32_t lds = sweep_lds_[g];
                 const bool last = own && g + 1 == sweep_maps_.size();
-                profBegin(sweep_stage_[g] >= 0 ? KSolve : KSweepPair, sweep_phase_[g]);
-                const dim3 grid(std::max(1U, xcdGrid(m.count / 2)));
-                if (sweep_stage_[g] >= 0) {
-                    const MapRef &sm = stage_maps_[size_t(sweep_stage_[g])];
-                    if (last)
-                        BLX_SIGNAL((k_solve<R, DIRECT, false>), tiles(sm), dim3(kWG), stream_, ev_sweep_, batch(sm), S,
-                                   kSolveFirst);
-                    else if (plastic)
-                        hipLaunchKernelGGL((k_solve<R, DIRECT, true>), tiles(sm), dim3(kWG), 0, stream_, batch(sm), S,
-                                           kSolveFirst);
-                    else
-                        hipLaunchKernelGGL((k_solve<R, DIRECT, false>), tiles(sm), dim3(kWG), 0, stream_, batch(sm), S,
-                                           kSolveFirst);
-                } else if (last) {
-                    BLX_SIGNAL_LDS((k_sweep_pair<R, DIRECT, false>), grid, dim3(kPairThreads), lds, stream_, ev_sweep_,
-                                   batch(m), S);
-                } else if (plastic) {
-                    hipLaunchKernelGGL((k_sweep_pair<R, DIRECT, true>), grid, dim3(kPairThreads), lds, stream_, batch(m), S);
-                } else {
-                    hipLaunchKernelGGL((k_sweep_pair<R, DIRECT, false>), grid, dim3(kPairThreads), lds, stream_, batch(m), S);
+                const bool single = sweep_stage_[g] >= 0;
+                ProfRecord *pr = profLaunch(single ? KSolve : KSweepPair, sweep_phase_[g]);
+                hipEvent_t start = pr ? pr->a : nullptr, stop = last ? ev_sweep_ : (pr ? pr->b : nullptr);
+                if (capturing_) {
+                    // A captured launch takes no events; the signal is a graph edge.
+                    start = nullptr;
+                    stop = nullptr;
                 }
-                profEnd();
+                const dim3 grid(std::max(1U, xcdGrid(m.count / 2)));
+                if (single) {
+                    const MapRef &sm = stage_maps_[size_t(sweep_stage_[g])];
+                    if (plastic)
+                        hipExtLaunchKernelGGL((k_solve<R, DIRECT, true>), tiles(sm), dim3(kWG), 0, stream_, start, stop, 0,
+                                              batch(sm), S, kSolveFirst);
+                    else
+                        hipExtLaunchKernelGGL((k_solve<R, DIRECT, false>), tiles(sm), dim3(kWG), 0, stream_, start, stop,
+                                              0, batch(sm), S, kSolveFirst);
+                } else if (plastic) {
+                    hipExtLaunchKernelGGL((k_sweep_pair<R, DIRECT, true>), grid, dim3(kPairThreads), lds, stream_, start,
+                                          stop, 0, batch(m), S);
+                } else {
+                    hipExtLaunchKernelGGL((k_sweep_pair<R, DIRECT, false>), grid, dim3(kPairThreads), lds, stream_, start,
+                                          stop, 0, batch(m), S);
+                }
+                HIP_OK(hipGetLastError());
+                if (capturing_ && last) HIP_OK(hipEventRecord(ev_sweep_, stream_));
+                if (pr && last) HIP_OK(hipEventRecord(pr->b, stream_));
             }
         }
         for (uint32_t it = 0; it < (pairs ? 0U : iterations); ++it) {
