@@ -1,4 +1,4 @@
-"""Float against double on the device, the report north_star asks for.
+"""Float against double on the device: the float lane's drift from an FP64 run.
 
 The float lane is the product; the double lane is the reference's FP64
 answer (the HIP double backend equals the reference's double lane bit for
