@@ -465,6 +465,36 @@ def c1_leg(args):
                         "per substep)"}
 
 
+def slabs_leg(args):
+    """The headline impact with its nodes in four z-slabs instead of one
+    block (the reference's slab schedule, TileImpactScene.hpp:283 defaults to
+    one): each slab lists its own 1,024 striker candidates, so the sphere meets
+    up to four times as many nodes. Same window as the headline run."""
+    from banjo_amd import abi, scenes
+    from banjo_amd.lattice import control, product
+    from banjo_amd.runner import prepare
+
+    hip = product()
+    substeps = args.steps * SUBSTEPS_PER_STEP
+    out = {}
+    for blocks in (1, 4):
+        sc = scenes.impact(128, 64, 128, substeps=substeps, blocks=blocks, name=f"impact_128x64x128_{blocks}")
+        sched, state = prepare(hip, sc)
+        be = tuned(hip.backend(abi.PRECISION_FLOAT, sched, 0), args)
+        be.upload(state, sc.settings, sc.sphere)
+        t0 = time.perf_counter()
+        st = be.run(control(substeps))
+        secs = time.perf_counter() - t0
+        be.close()
+        out[f"blocks_{blocks}"] = {"seconds": secs, "bond_solves_per_s": sc.bond_count * st.total_steps / secs,
+                                   "broken_bonds": st.broken_bonds,
+                                   "striker_contacts": st.contact.impulse_contacts,
+                                   "candidate_overflow": st.contact.candidate_overflow}
+    out["substeps"] = substeps
+    out["workload"] = "the headline impact from t = 0, one block against four z-slabs"
+    return out
+
+
 def cut_leg(args):
     """The cut form: a 256 x 32 x 32 plank on two ledges, plastic, cut at
     mid-span by a 1 mm kinematic blade moving down at 2 m/s. Every step is 12
@@ -530,11 +560,12 @@ def main():
     ap.add_argument("--cpu-warm-substeps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-quick", action="store_true", help="only the config-2 window of the CPU baseline")
-    ap.add_argument("--forms", default="plain,batched,served,cut,contact,c1,precision",
+    ap.add_argument("--forms", default="plain,batched,served,cut,contact,c1,precision,slabs",
                     help="comma list of plain (the headline impact), batched (many uneven lattices), "
                          "served (a stream of trials), cut (a plank cut by a blade, labels every step), "
                          "contact (the cut with node-node contact on), c1 (the 16^3 cantilever in full), "
-                         "precision (float against double on the device)")
+                         "precision (float against double on the device), slabs (the impact in one block "
+                         "and in four z-slabs)")
     ap.add_argument("--cut-steps", type=int, default=600)
     ap.add_argument("--contact-steps", type=int, default=120)
     ap.add_argument("--batch-trials", type=int, default=512)
@@ -577,6 +608,8 @@ def main():
             out["c1"] = c1_leg(args)
         if "precision" in forms:
             out["precision"] = precision_leg(args)
+        if "slabs" in forms:
+            out["slabs"] = slabs_leg(args)
         print(json.dumps(out))
         return
     t_build = time.perf_counter()
@@ -635,6 +668,8 @@ def main():
         out["c1"] = c1_leg(args)
     if "precision" in forms:
         out["precision"] = precision_leg(args)
+    if "slabs" in forms:
+        out["slabs"] = slabs_leg(args)
     print(json.dumps(out))
 
 
