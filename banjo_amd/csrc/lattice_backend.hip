@@ -1057,11 +1057,12 @@ public:
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&early_, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&lds_, hipStreamNonBlocking));
         // Only other streams of this device wait on these events, so their
         // release is device-scope; the default system-scope fence makes every
         // signal also write the caches back for the host (about 1% of the step).
         for (hipEvent_t *e : {&ev_bond_end_, &ev_rinv_, &ev_pass1_, &ev_velocity_, &ev_early_, &ev_predict_, &ev_start_,
-                              &ev_sweep_, &ev_strain_})
+                              &ev_sweep_, &ev_strain_, &ev_lds_})
             HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
         HIP_OK(hipEventCreate(&ev0_));
         HIP_OK(hipEventCreate(&ev1_));
@@ -1080,13 +1081,13 @@ public:
     ~Engine() {
         (void)hipSetDevice(device_);
         dropGraph();
-        for (hipStream_t s : {stream_, side_, early_})
+        for (hipStream_t s : {stream_, side_, early_, lds_})
             if (s) (void)hipStreamSynchronize(s);
         for (hipEvent_t e : {ev_bond_end_, ev_rinv_, ev_pass1_, ev_velocity_, ev_early_, ev_predict_, ev_start_, ev_sweep_,
-                             ev_strain_, ev0_, ev1_})
+                             ev_strain_, ev_lds_, ev0_, ev1_})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
-        for (hipStream_t s : {early_, side_, stream_})
+        for (hipStream_t s : {lds_, early_, side_, stream_})
             if (s) (void)hipStreamDestroy(s);
     }
 
@@ -1903,9 +1904,12 @@ private:
     // overlap the folds; early_ the end sample of the nodes and bonds striker
     // pass 2 cannot reach, which overlaps pass 2.
     hipStream_t stream_ = nullptr, side_ = nullptr, early_ = nullptr;
+    // lds_ runs the LDS sweep of the lattices that fit (k_sweep_lds) beside the
+    // staged sweep of the rest, between pass 1 and the sweep's end.
+    hipStream_t lds_ = nullptr;
     hipEvent_t ev_predict_ = nullptr, ev_start_ = nullptr, ev_sweep_ = nullptr;
     hipEvent_t ev_bond_end_ = nullptr, ev_rinv_ = nullptr, ev_pass1_ = nullptr, ev_velocity_ = nullptr,
-               ev_early_ = nullptr, ev_strain_ = nullptr;
+               ev_early_ = nullptr, ev_strain_ = nullptr, ev_lds_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t cus_ = 256;
     uint64_t launches_ = 0;
@@ -2324,7 +2328,7 @@ private:
         // stream_ by an event (hipStreamEndCapture reports an unjoined fork,
         // and a stream left capturing would capture the next eager launches).
         bool left = false;
-        for (hipStream_t s : {side_, early_}) {
+        for (hipStream_t s : {side_, early_, lds_}) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) left = true;
         }
@@ -2411,8 +2415,14 @@ private:
         profBegin(KBondStart, 2, side_);
         BLX_SIGNAL((k_bond_start<R, DIRECT>), lg, dim3(kWG), side_, ev_start_, list);
         profEnd(side_);
+        const uint32_t iterations = shared_.iterations;
+        const bool pairs = iterations == 1 && tuning_.colours_per_launch == 2;
+        // The lattices that sweep in LDS do so on lds_, beside the staged sweep
+        // of the rest, once pass 1 (or, without a striker, k_predict) is done.
+        const bool fork = lds_map_.count > 0 && (pairs ? !sweep_maps_.empty() : !stage_maps_.empty());
         if (sphere)
-            BLX_LAUNCH(KPass1, 4, BLX_SIGNAL((k_pass<R, 1>), fg, dim3(kFinishThreads), stream_, split && !own ? ev_pass1_ : nullptr, lats, S));
+            BLX_LAUNCH(KPass1, 4, BLX_SIGNAL((k_pass<R, 1>), fg, dim3(kFinishThreads), stream_,
+                                             (split && !own) || fork ? ev_pass1_ : nullptr, lats, S));
         // Without damping the velocity update on the early stream marks the
         // nodes pass 2 can reach; otherwise k_mark_affected does it there
         // during the sweep, after pass 1.
@@ -2422,21 +2432,27 @@ private:
             hipLaunchKernelGGL((k_mark_affected<R>), entries(mark_map_), dim3(kWG), 0, early_, batch(mark_map_));
             profEnd(early_);
         }
-        const uint32_t iterations = shared_.iterations;
-        const bool pairs = iterations == 1 && tuning_.colours_per_launch == 2;
         if (lds_map_.count > 0) {
-            // Every lattice that fits sweeps in LDS, one workgroup each.
-            const bool last = own && (pairs ? sweep_maps_.empty() : stage_maps_.empty());
+            // Every lattice that fits sweeps in LDS, one workgroup each: on lds_
+            // when other lattices sweep by stages (joined after their sweep),
+            // else on stream_, where it is the whole sweep.
+            const bool last = own && !fork;
             const Batch<R> lb = batch(lds_map_);
             const dim3 g(lds_map_.count), t(tuning_.lds_threads);
-            profBegin(KSolve, 5);
-            if (last)
-                BLX_SIGNAL_LDS((k_sweep_lds<R, DIRECT, false>), g, t, lds_bytes_, stream_, ev_sweep_, lb, S);
+            hipStream_t on = fork ? lds_ : stream_;
+            if (fork) HIP_OK(hipStreamWaitEvent(lds_, sphere ? ev_pass1_ : ev_predict_, 0));
+            profBegin(KSolve, 5, on);
+            if (fork && plastic)
+                BLX_SIGNAL_LDS((k_sweep_lds<R, DIRECT, true>), g, t, lds_bytes_, on, ev_lds_, lb, S);
+            else if (fork)
+                BLX_SIGNAL_LDS((k_sweep_lds<R, DIRECT, false>), g, t, lds_bytes_, on, ev_lds_, lb, S);
+            else if (last)
+                BLX_SIGNAL_LDS((k_sweep_lds<R, DIRECT, false>), g, t, lds_bytes_, on, ev_sweep_, lb, S);
             else if (plastic)
-                hipLaunchKernelGGL((k_sweep_lds<R, DIRECT, true>), g, t, lds_bytes_, stream_, lb, S);
+                hipLaunchKernelGGL((k_sweep_lds<R, DIRECT, true>), g, t, lds_bytes_, on, lb, S);
             else
-                hipLaunchKernelGGL((k_sweep_lds<R, DIRECT, false>), g, t, lds_bytes_, stream_, lb, S);
-            profEnd();
+                hipLaunchKernelGGL((k_sweep_lds<R, DIRECT, false>), g, t, lds_bytes_, on, lb, S);
+            profEnd(on);
         }
         if (pairs) {
             // The paired sweep: two stages per launch on LDS tiles (k_sweep_pair).
@@ -2502,7 +2518,11 @@ private:
             if ((S.plane_count > 0 || S.blade.enabled) && it + 1 < iterations)
                 BLX_LAUNCH(KProject, 7, hipLaunchKernelGGL((k_project<R>), ng, dim3(kWG), 0, stream_, nodes, S));
         }
+        // The LDS sweep on lds_ joins here: what follows on stream_ reads every
+        // lattice's swept nodes, and so does striker pass 2 on the early stream.
+        if (fork) HIP_OK(hipStreamWaitEvent(stream_, ev_lds_, 0));
         if (own) {
+            if (fork) HIP_OK(hipStreamWaitEvent(early_, ev_lds_, 0));
             // Striker pass 2 forks onto the early stream after the sweep and
             // takes its listed candidates' velocity update itself; the strain
             // of the nodes it can reach follows it there. The velocity update
