@@ -1753,6 +1753,9 @@ __device__ double fixedOrderSum(const double *part, u32 count, u32 stride, u32 o
 constexpr u32 kEventBatch = 4096;
 constexpr u32 kGroupsPerThread = 4;  // summary words a finish thread reads at once
 constexpr u32 kFlagsInLds = 2048;    // flagged event words kept in LDS
+constexpr u32 kEnergyAhead = 24;     // removed energies the finish's sum reads ahead
+constexpr u32 kWordsPerThread = 2;    // flagged event words a finish thread stages per batch
+constexpr u32 kBatchWords = kFinishThreads * kWordsPerThread;
 constexpr unsigned kBudgetSpent = 3U;  // the reference's max-steps exit reason
 
 // Position of the k-th (from 0) set bit of m; k < popcount(m).
@@ -1776,11 +1779,17 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     if (ctl->stop) return;
     __shared__ double s_red[kFinishThreads];
     __shared__ u32 s_waves[kFinishThreads / 64];
-    __shared__ u32 s_word[kFinishThreads], s_incl[kFinishThreads];
-    __shared__ u64 s_evw[kFinishThreads], s_bkw[kFinishThreads];
-    __shared__ u32 s_j[kEventBatch];
-    __shared__ float s_d[kEventBatch];
-    __shared__ R s_e[kEventBatch];
+    __shared__ u32 s_word[kBatchWords], s_incl[kBatchWords];
+    __shared__ u64 s_evw[kBatchWords], s_bkw[kBatchWords];
+    // A staged batch's bond ids and damages; once wave 0 has read them, the
+    // same bytes hold the running removed-energy sums of its broken bonds.
+    __shared__ double s_jd[kEventBatch];
+    u32 *const s_j = reinterpret_cast<u32 *>(s_jd);
+    float *const s_d = reinterpret_cast<float *>(s_jd) + kEventBatch;
+    double *const s_sum = s_jd;
+    // Removed energies, widened as the ledger sums them; the tail pads the
+    // energy sum's reads ahead (kEnergyAhead).
+    __shared__ double s_e[kEventBatch + kEnergyAhead];
     __shared__ u32 s_take;
     __shared__ u32 s_flags[kFlagsInLds];
     const u32 tid = threadIdx.x, lane = tid & 63U;
@@ -1801,7 +1810,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     }
     const u64 step = ctl->step;
     // Wave 0 carries the folds: its lanes all hold the same maximum damage and
-    // counts; lane 0 holds the removed-energy sum and its gain flag.
+    // counts; lane 0 holds the removed-energy sum.
     double m = ctl->max_damage, removed = ctl->removed_energy_j;
     bool damage_gain = false, energy_gain = false;
     u32 broken_now = 0;
@@ -1838,28 +1847,47 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
         __syncthreads();
         for (u32 fb = 0; fb < nflag;) {
             // As many whole event words as fit in the event batch; a word holds
-            // at most 64 events, so at least one always fits.
-            const u32 mcount = min(u32(kFinishThreads), nflag - fb);
-            const u32 wd = tid < mcount ? flags[fb + tid] : 0U;
-            u64 evw = 0, bkw = 0;
-            if (tid < mcount) {
-                evw = L.ev_bits[wd];
-                bkw = L.brk_bits[wd];
+            // at most 64 events, so at least one always fits. A thread takes
+            // kWordsPerThread consecutive flagged words.
+            const u32 mcount = min(kBatchWords, nflag - fb);
+            u32 wd[kWordsPerThread], incl[kWordsPerThread], c = 0;
+            u64 evw[kWordsPerThread], bkw[kWordsPerThread];
+#pragma unroll
+            for (u32 h = 0; h < kWordsPerThread; ++h) {
+                const u32 k = tid * kWordsPerThread + h;
+                wd[h] = k < mcount ? flags[fb + k] : 0U;
+                evw[h] = k < mcount ? L.ev_bits[wd[h]] : 0ULL;
+                bkw[h] = k < mcount ? L.brk_bits[wd[h]] : 0ULL;
             }
-            const u32 c = u32(__popcll(evw));
+#pragma unroll
+            for (u32 h = 0; h < kWordsPerThread; ++h) c += u32(__popcll(evw[h]));
             u32 all = 0;
-            const u32 incl = blockScan1024(c, s_waves, &all) + c;
-            s_word[tid] = wd;
-            s_incl[tid] = incl;
-            s_evw[tid] = evw;
-            s_bkw[tid] = bkw;
+            u32 run = blockScan1024(c, s_waves, &all);
+#pragma unroll
+            for (u32 h = 0; h < kWordsPerThread; ++h) {
+                const u32 k = tid * kWordsPerThread + h;
+                run += u32(__popcll(evw[h]));
+                incl[h] = run;
+                s_word[k] = wd[h];
+                s_incl[k] = run;
+                s_evw[k] = evw[h];
+                s_bkw[k] = bkw[h];
+            }
             __syncthreads();
-            if (tid < mcount && incl <= kEventBatch && (tid + 1 == mcount || s_incl[tid + 1] > kEventBatch)) s_take = tid + 1;
+#pragma unroll
+            for (u32 h = 0; h < kWordsPerThread; ++h) {
+                const u32 k = tid * kWordsPerThread + h;
+                if (k < mcount && incl[h] <= kEventBatch && (k + 1 == mcount || s_incl[k + 1] > kEventBatch))
+                    s_take = k + 1;
+            }
             __syncthreads();
             const u32 take = s_take;
-            if (tid < take) {
-                L.ev_bits[wd] = 0;
-                L.brk_bits[wd] = 0;
+#pragma unroll
+            for (u32 h = 0; h < kWordsPerThread; ++h) {
+                if (tid * kWordsPerThread + h < take) {
+                    L.ev_bits[wd[h]] = 0;
+                    L.brk_bits[wd[h]] = 0;
+                }
             }
             const u32 nev = s_incl[take - 1];
             for (u32 e = tid; e < nev; e += kFinishThreads) {
@@ -1873,7 +1901,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
                 const u32 j = s_word[lo] * 64U + pos;
                 s_j[e] = j | (((s_bkw[lo] >> pos) & 1ULL) ? 0x80000000U : 0U);
                 s_d[e] = L.ev_d[j];
-                s_e[e] = L.ev_e[j];
+                s_e[e] = double(L.ev_e[j]);
             }
             __syncthreads();
             // Wave 0 folds the staged batch in bond order, 64 events at a time;
@@ -1885,7 +1913,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
                     const bool ok = e < nev;
                     const u32 jr = ok ? s_j[e] : 0U;
                     const float dmg = ok ? s_d[e] : 0.f;
-                    const R en = ok ? s_e[e] : R(0);
+                    const double en = ok ? s_e[e] : 0.0;
                     // max_damage rises only by more than 1e-6 (CpuLatticeBackend.cpp:462-465).
                     u64 cand = __ballot(ok && double(dmg) > m + 1.0e-6);
                     while (cand) {
@@ -1907,20 +1935,51 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
                     nb += u32(__popcll(bmask));
                 }
                 // Removed energy summed in bond order by one lane: the only
-                // dependence from one bond to the next is the addition.
+                // dependence from one bond to the next is the addition, so the
+                // lane keeps only the additions and stores each running sum
+                // (over the ids and damages, which every lane has read); the
+                // energy-gain test on each sum then runs on all 64 lanes.
+                // The lane works on three register groups of energies in turn:
+                // it adds one group and stores its sums while the reads of the
+                // next two are in flight, so no addition waits on LDS (the
+                // empty asm statements keep each group's reads ahead of the
+                // next group's additions). Reads past nb land in the padding
+                // and are not used.
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 if (lane == 0) {
-                    double sum = removed;
-                    bool gain = false;
-#pragma unroll 8
-                    for (u32 k = 0; k < nb; ++k) {
-                        const double el = double(s_e[k]);
-                        sum += el;
-                        gain |= el > flat * (sum > 1.0e-12 ? sum : 1.0e-12);
+                    constexpr u32 G = kEnergyAhead / 3;
+                    double sum = removed, g[3][G];
+#pragma unroll
+                    for (u32 h = 0; h < 3; ++h)
+#pragma unroll
+                        for (u32 q = 0; q < G; ++q) g[h][q] = s_e[h * G + q];
+                    u32 k = 0;
+                    for (; k + 3 * G <= nb; k += 3 * G) {
+#pragma unroll
+                        for (u32 h = 0; h < 3; ++h) {
+#pragma unroll
+                            for (u32 q = 0; q < G; ++q) {
+                                sum += g[h][q];
+                                s_sum[k + h * G + q] = sum;
+                            }
+#pragma unroll
+                            for (u32 q = 0; q < G; ++q) g[h][q] = s_e[k + (h + 3) * G + q];
+                            asm volatile("" ::: "memory");
+                        }
+                    }
+                    for (; k < nb; ++k) {
+                        sum += s_e[k];
+                        s_sum[k] = sum;
                     }
                     removed = sum;
-                    energy_gain |= gain;
                 }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                bool gain = false;
+                for (u32 k = lane; k < nb; k += 64) {
+                    const double el = s_e[k], sum = s_sum[k];
+                    gain |= el > flat * (sum > 1.0e-12 ? sum : 1.0e-12);
+                }
+                energy_gain |= __ballot(gain) != 0ULL;
             }
             __syncthreads();
             fb += take;
