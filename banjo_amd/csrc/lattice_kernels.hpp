@@ -595,16 +595,24 @@ template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_prev_sa
 // ---------------------------------------------------------------------------
 // Kick, predict, classify.
 
-template <class R> struct Ledger13 { double v[13]; };
 
-template <class R> __device__ void blockSum13(double (*sh)[kWG], double v[13]) {
-    for (int k = 0; k < 13; ++k) sh[k][threadIdx.x] = v[k];
+// Sums of 13 per-thread values over a workgroup of WAVES waves, in a fixed
+// order: each wave folds its lanes by a shuffle tree, then thread k < 13 adds
+// the waves' sums in wave order into sh[WAVES][k], which every thread can read
+// after the call. Every thread of the workgroup calls it.
+template <u32 WAVES> __device__ void groupSum13(double (*sh)[13], double v[13]) {
+    const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
+    for (u32 off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 13; ++k) v[k] += __shfl_down(v[k], off);
+    if (lane == 0)
+        for (int k = 0; k < 13; ++k) sh[wave][k] = v[k];
     __syncthreads();
-    for (int w = kWG / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < unsigned(w))
-            for (int k = 0; k < 13; ++k) sh[k][threadIdx.x] = sh[k][threadIdx.x] + sh[k][threadIdx.x + w];
-        __syncthreads();
+    if (threadIdx.x < 13) {
+        double t = sh[0][threadIdx.x];
+        for (u32 w = 1; w < WAVES; ++w) t += sh[w][threadIdx.x];
+        sh[WAVES][threadIdx.x] = t;
     }
+    __syncthreads();
 }
 
 template <class R, bool SPHERE, bool LOADS, bool DIRECT>
@@ -696,10 +704,10 @@ __global__ __launch_bounds__(kWG) void k_predict(Batch<R> B, Step<R> S) {
         if (threadIdx.x == 0) L.tile_cand[tile] = total;
     }
     if (LOADS && loads) {
-        __shared__ double sh[13][kWG];
-        blockSum13<R>(sh, led);
+        __shared__ double sh[kWG / 64 + 1][13];
+        groupSum13<kWG / 64>(sh, led);
         if (threadIdx.x == 0)
-            for (int k = 0; k < 13; ++k) L.load_part[size_t(tile) * 13 + k] = sh[k][0];
+            for (int k = 0; k < 13; ++k) L.load_part[size_t(tile) * 13 + k] = sh[kWG / 64][k];
     }
 }
 
@@ -1987,8 +1995,16 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     }
     double ledger[13];
     const bool loads = LOADS && ctl->load_remaining > 0;
-    if (LOADS && loads)
-        for (int k = 0; k < 13; ++k) ledger[k] = fixedOrderSum<R>(L.load_part, L.node_tiles, 13, k, s_red);
+    if (LOADS && loads) {
+        // The 13 sums over the node tiles in one pass: each thread adds its
+        // tiles in order, then groupSum13's fixed tree.
+        __shared__ double s_led[kFinishThreads / 64 + 1][13];
+        double v[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (u32 t = tid; t < L.node_tiles; t += kFinishThreads)
+            for (int k = 0; k < 13; ++k) v[k] += L.load_part[size_t(t) * 13 + k];
+        groupSum13<kFinishThreads / 64>(s_led, v);
+        for (int k = 0; k < 13; ++k) ledger[k] = s_led[kFinishThreads / 64][k];
+    }
     double plastic = 0.0;
     if (PLASTIC) plastic = fixedOrderSum<R>(L.tile_plastic, L.bond_tiles, 1, 0, s_red);
     // The serial tail, on the LDS copy; then every thread writes the copy back.
