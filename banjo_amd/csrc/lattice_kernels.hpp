@@ -537,7 +537,7 @@ template <class R> __global__ void k_edge_palette(EdgePal<R> *pal, u32 *no_dir, 
 }
 
 __device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, float ms, float mp,
-                                              float (*s_max)[kWG / 64]);
+                                              u32 (*s_max)[kWG / 64]);
 
 // The start sample of the live bonds at dirty nodes (LatticePhysics.hpp:1118-1126).
 // A bond with no dirty end samples exactly what its end sample stored, so
@@ -553,7 +553,7 @@ template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_st
     const DevLattice<R> L = B.L[m.lat];
     const u32 half = u32(L.ctl->step & 1U) ^ 1U;  // the previous substep's half
     if (L.ctl->stop || !L.ctl->dirty_start[half]) return;
-    __shared__ float s_max[4][kWG / 64];
+    __shared__ u32 s_max[4][kWG / 64];
     float mc = 0.f, ms = 0.f;
     // Node slots D * N are below 2^32 (refused at upload otherwise).
     const u32 pairs = L.ctl->dirty_count[half] * L.D;
@@ -1584,33 +1584,41 @@ __device__ __forceinline__ void appendDirty(u32 *list, u32 *count, u32 claim_a, 
     if (claim_b != kNone) list[at] = claim_b;
 }
 
-// Running maxima of non-negative floats: integer atomicMax on the bits.
-__device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, float ms, float mp, float (*s_max)[kWG / 64]) {
+// The largest positive value of a wave, as its bits (0 when there is none):
+// positive floats order as their bit patterns, so the wave folds unsigned
+// maxima by DPP row shifts and row broadcasts into lane 63 (every lane active).
+__device__ __forceinline__ u32 wavePeakBits(float x) {
+    u32 v = x > 0.f ? __float_as_uint(x) : 0U;
+    v = max(v, u32(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xf, 0xf, false)));  // row_shr:1
+    v = max(v, u32(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xf, 0xf, false)));  // row_shr:2
+    v = max(v, u32(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xf, 0xf, false)));  // row_shr:4
+    v = max(v, u32(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xf, 0xf, false)));  // row_shr:8
+    v = max(v, u32(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xa, 0xf, false)));  // row_bcast:15
+    v = max(v, u32(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xc, 0xf, false)));  // row_bcast:31
+    return u32(__builtin_amdgcn_readlane(int(v), 63));
+}
+
+// Running maxima of non-negative floats: integer atomicMax on the bits of
+// the workgroup's largest positive value. Every thread calls it.
+__device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, float ms, float mp, u32 (*s_max)[kWG / 64]) {
     const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
-    for (int q = 32; q > 0; q >>= 1) {
-        mt = fmaxf(mt, __shfl_xor(mt, q));
-        mc = fmaxf(mc, __shfl_xor(mc, q));
-        ms = fmaxf(ms, __shfl_xor(ms, q));
-        mp = fmaxf(mp, __shfl_xor(mp, q));
-    }
+    const u32 bt = wavePeakBits(mt), bc = wavePeakBits(mc), bs = wavePeakBits(ms), bp = wavePeakBits(mp);
     if (lane == 0) {
-        s_max[0][wave] = mt;
-        s_max[1][wave] = mc;
-        s_max[2][wave] = ms;
-        s_max[3][wave] = mp;
+        s_max[0][wave] = bt;
+        s_max[1][wave] = bc;
+        s_max[2][wave] = bs;
+        s_max[3][wave] = bp;
     }
     __syncthreads();
     if (threadIdx.x < 4) {
-        float m = 0.f;
-        for (int w = 0; w < kWG / 64; ++w) m = fmaxf(m, s_max[threadIdx.x][w]);
+        u32 m = 0;
+        for (int w = 0; w < kWG / 64; ++w) m = max(m, s_max[threadIdx.x][w]);
         u32 *dst = threadIdx.x < 3 ? &ctl->peak_bits[threadIdx.x] : &ctl->pstretch_bits;
         // The running maxima rarely move: read first, so thousands of
         // workgroups do not queue atomics on the same four words. The read goes
         // through the caches: a maximum only grows, so a stale value is never
         // above the current one and never skips a needed update.
-        if (m > 0.f && __float_as_uint(m) > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT)) {
-            atomicMax(dst, __float_as_uint(m));
-        }
+        if (m > 0U && m > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT)) atomicMax(dst, m);
     }
 }
 
@@ -1631,7 +1639,7 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
         }
     }
     if (ctl->stop) return;
-    __shared__ float s_max[4][kWG / 64];
+    __shared__ u32 s_max[4][kWG / 64];
     __shared__ double s_pw[kWG];
     const double m0 = ctl->max_damage;
     const u32 stamp = u32(ctl->step) + 1U;
