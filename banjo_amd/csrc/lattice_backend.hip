@@ -328,7 +328,7 @@ public:
     // Offsets of the device arrays in the arena.
     struct Off {
         size_t x04, u4, u4b, pnodes, pbonds, stage_ranges, stage_end, up4, v4, rinv, unm, sa, sb, valid, approach, engaged, cand;
-        size_t nbr_key, spal, sdelta, nbr_bond, epal, nodir, rec, alive, lambda, pext;
+        size_t nbr_key, spal, sdelta, nbr_bond, epal, nodir, rec, cmpl, alive, lambda, pext;
         size_t bnd, alive_n, thr, thr2, pv, pyz, mode, pstr, slots;
         size_t ev_bits, brk_bits, grp_bits, ev_d, ev_e, flag_list, tile_cand, dirty_list, dirty_stamp, rank_stamp;
         size_t affected, affected_list, tile_plastic, load_part, fext, freq, cand_all, cand_tile, cand_lo, cand_cnt;
@@ -420,6 +420,7 @@ public:
         l.nbr_bond = arena.at<uint32_t>(o.nbr_bond);
         l.epal = arena.at<EdgePal<R>>(o.epal);
         l.rec = arena.at<BondRec<R>>(o.rec);
+        l.cmpl = arena.at<R>(o.cmpl);
         l.alive = arena.at<uint8_t>(o.alive);
         l.lambda = arena.at<R>(o.lambda);
         l.pext = arena.at<R>(o.pext);
@@ -856,6 +857,7 @@ public:
         o.epal = arena.reserve<EdgePal<R>>(np);
         o.nodir = arena.reserve<uint32_t>(np);
         o.rec = arena.reserve<BondRec<R>>(B);
+        o.cmpl = arena.reserve<R>(B);
         o.alive = arena.reserve<uint8_t>(B);
         o.lambda = arena.reserve<R>(B);
         o.pext = arena.reserve<R>(B);
@@ -967,6 +969,7 @@ public:
         for (uint32_t k = 0; k < B; ++k) pal_of[k] |= no_dir[pal_of[k]] ? kNoDir : 0U;
         {
             std::vector<BondRec<R>> rec(B);
+            std::vector<R> cmpl(B);
             std::vector<BondNode> bnd(B);
             std::vector<R2<R>> pv(B), pyz(B);
             std::vector<R> pext(B), pstr(B);
@@ -979,7 +982,11 @@ public:
                 r.a = st.bond_a[k];
                 r.b = st.bond_b[k];
                 r.p = pal_of[k];
-                r.c = R(st.compliance[k]);
+                // The solve's compliance term alpha = compliance / dt^2
+                // (LatticePhysics.hpp:1132-1168), the same float operations
+                // the solve would repeat each substep.
+                cmpl[k] = R(st.compliance[k]);
+                r.c = cmpl[k] / (S.dt * S.dt);
                 rec[k] = r;
                 pext[k] = R(st.plastic_extension[k]);
             }
@@ -997,6 +1004,7 @@ public:
                 mode[q] = st.failure_mode[k];
             }
             putv(o.rec, rec);
+            putv(o.cmpl, cmpl);
             for (PairBond<R> &e : sweep_bonds) {
                 e.p = pal_of[e.j];
                 e.c = rec[e.j].c;

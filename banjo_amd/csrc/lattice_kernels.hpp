@@ -131,7 +131,8 @@ template <class R> struct DevLattice {
     const u32 *nbr_bond; // D planes of N: node-major index of the slot's bond (start sample)
     const EdgePal<R> *epal;
     // Bonds in schedule order (the colour sweep).
-    const BondRec<R> *rec;
+    const BondRec<R> *rec;  // c holds compliance / dt^2
+    const R *cmpl;          // compliance per bond, schedule order
     u8 *alive;
     R *lambda, *pext;
     // Bonds in node-major order (the end sample).
@@ -1081,10 +1082,11 @@ __global__ __launch_bounds__(PASS == 1 ? kFinishThreads : kPass2Threads) void k_
 // the bond's edge palette entry, lam the multiplier it reads (0 in the first
 // iteration); a solve stores lam + dl in lam_out and returns true, a bond
 // shorter than 1e-12 returns false. r.a and r.b (the global node indices) are
-// read only with DIRECT, c always and p only through g.
+// read only with DIRECT, c always and p only through g. alpha is the record's
+// compliance / dt^2, computed once at upload by the same float operations.
 template <class R, bool DIRECT, bool PLASTIC>
 __device__ __forceinline__ bool solveBondWith(const DevLattice<R> &L, const Step<R> &S, const EdgePal<R> &g, u32 a,
-                                              u32 b, R compliance, u32 j, R lam, R4<R> &ua, R4<R> &ub, R &lam_out) {
+                                              u32 b, R alpha, u32 j, R lam, R4<R> &ua, R4<R> &ub, R &lam_out) {
     const R4<R> ge = g.e;
     const V3<R> du = xyz(ub) - xyz(ua);
     const V3<R> e = xyz(ge);
@@ -1103,7 +1105,6 @@ __device__ __forceinline__ bool solveBondWith(const DevLattice<R> &L, const Step
     }
     const V3<R> dir = delta / len;
     const R wa = ua.w, wb = ub.w;
-    const R alpha = compliance / (S.dt * S.dt);
     const R dl = (-C - alpha * lam) / (wa + wb + alpha);
     lam_out = lam + dl;
     ua = with(xyz(ua) - wa * dl * dir, ua.w);
@@ -1496,7 +1497,7 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
             ps = ps + inc;
             L.pext[j] = plastic;
             L.pstr[o] = ps;
-            const R c = L.rec[j].c;
+            const R c = L.cmpl[j];
             if (c > R(0)) {
                 const R yend = ye + h * inc;
                 pw = double(R(0.5) * (ye + yend) * inc / c);
@@ -1544,7 +1545,7 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
             L.pyz[o] = R2<R>{maxR(R(0), c2), maxR(R(0), s2)};
         }
         const R ext = length(bondVec<R, DIRECT>(L, r.a, r.b, g.e, ua, ub)) - restL - plastic;
-        const R cc = L.rec[j].c;
+        const R cc = L.cmpl[j];
         event_energy = cc <= R(0) ? R(0) : R(0.5) * ext * ext / cc;
         L.alive[j] = 0;
         L.alive_n[o] = 0;
