@@ -597,15 +597,34 @@ template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_prev_sa
 // Kick, predict, classify.
 
 
+// Sum of a double over the 64 lanes of a wave in a fixed order, in lane 63:
+// DPP row shifts by 1, 2, 4 and 8 leave each row's sum in its last lane, row
+// broadcasts add row 0 into row 1 and row 2 into row 3, then lane 31 into the
+// upper half. No LDS and no barrier; every lane of the wave must be active.
+template <int CTRL, int ROWS> __device__ __forceinline__ double dppShift(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ double waveSumLast(double v) {
+    v += dppShift<0x111, 0xf>(v);  // row_shr:1
+    v += dppShift<0x112, 0xf>(v);  // row_shr:2
+    v += dppShift<0x114, 0xf>(v);  // row_shr:4
+    v += dppShift<0x118, 0xf>(v);  // row_shr:8
+    v += dppShift<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dppShift<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // Sums of 13 per-thread values over a workgroup of WAVES waves, in a fixed
-// order: each wave folds its lanes by a shuffle tree, then thread k < 13 adds
-// the waves' sums in wave order into sh[WAVES][k], which every thread can read
+// order: each wave folds its lanes (waveSumLast), then thread k < 13 adds the
+// waves' sums in wave order into sh[WAVES][k], which every thread can read
 // after the call. Every thread of the workgroup calls it.
 template <u32 WAVES> __device__ void groupSum13(double (*sh)[13], double v[13]) {
     const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
-    for (u32 off = 32; off > 0; off >>= 1)
-        for (int k = 0; k < 13; ++k) v[k] += __shfl_down(v[k], off);
-    if (lane == 0)
+    for (int k = 0; k < 13; ++k) v[k] = waveSumLast(v[k]);
+    if (lane == 63)
         for (int k = 0; k < 13; ++k) sh[wave][k] = v[k];
     __syncthreads();
     if (threadIdx.x < 13) {
@@ -2005,14 +2024,19 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(Batch<R> B, Step<R> S
     double ledger[13];
     const bool loads = LOADS && ctl->load_remaining > 0;
     if (LOADS && loads) {
-        // The 13 sums over the node tiles in one pass: each thread adds its
-        // tiles in order, then groupSum13's fixed tree.
-        __shared__ double s_led[kFinishThreads / 64 + 1][13];
-        double v[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (u32 t = tid; t < L.node_tiles; t += kFinishThreads)
-            for (int k = 0; k < 13; ++k) v[k] += L.load_part[size_t(t) * 13 + k];
-        groupSum13<kFinishThreads / 64>(s_led, v);
-        for (int k = 0; k < 13; ++k) ledger[k] = s_led[kFinishThreads / 64][k];
+        // The 13 sums over the node tiles in one pass on wave 0: each lane adds
+        // its tiles in order, then waveSumLast's fixed order.
+        __shared__ double s_led[13];
+        if (tid < 64) {
+            double v[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (u32 t = tid; t < L.node_tiles; t += 64)
+                for (int k = 0; k < 13; ++k) v[k] += L.load_part[size_t(t) * 13 + k];
+            for (int k = 0; k < 13; ++k) v[k] = waveSumLast(v[k]);
+            if (tid == 63)
+                for (int k = 0; k < 13; ++k) s_led[k] = v[k];
+        }
+        __syncthreads();
+        for (int k = 0; k < 13; ++k) ledger[k] = s_led[k];
     }
     double plastic = 0.0;
     if (PLASTIC) plastic = fixedOrderSum<R>(L.tile_plastic, L.bond_tiles, 1, 0, s_red);
