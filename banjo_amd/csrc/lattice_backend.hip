@@ -395,12 +395,28 @@ public:
         l.node_tiles = node_tiles;
         l.bond_tiles = bond_tiles;
         l.group_words = group_words;
+        l.npal = edge_palette;
         l.tensile_only = tensile_only ? 1U : 0U;
         l.u4 = arena.at<R4<R>>(o.u4);
         l.u4b = arena.at<R4<R>>(o.u4b);
         l.stage_ranges = arena.at<uint2>(o.stage_ranges);
         l.stage_end = arena.at<uint32_t>(o.stage_end);
         l.stage_count = uint32_t(stage_end.size());
+        {
+            // The LDS sweep's prefetching path: one bond range per stage and
+            // a palette that fits its LDS slots.
+            bool one = true;
+            uint32_t widest = 0;
+            for (size_t st = 0; st < stage_end.size(); ++st) {
+                const uint32_t q0 = st == 0 ? 0U : stage_end[st - 1], q1 = stage_end[st];
+                one = one && q1 == q0 + 1;
+                uint32_t n = 0;
+                for (uint32_t q = q0; q < q1; ++q) n += sweep_ranges[q].y - sweep_ranges[q].x;
+                widest = std::max(widest, n);
+            }
+            l.lds_fast = one && edge_palette <= kLdsPal ? 1U : 0U;
+            l.stage_max = widest;
+        }
         l.pnodes = arena.at<uint32_t>(o.pnodes);
         l.pbonds = arena.at<PairBond<R>>(o.pbonds);
         l.up4 = arena.at<R4<R>>(o.up4);
@@ -2038,7 +2054,7 @@ private:
         lds_bytes_ = 0;
         std::vector<uint32_t> staged;
         for (uint32_t k : run) {
-            const size_t bytes = size_t(slots_[k]->N) * sizeof(R4<R>);
+            const size_t bytes = kLdsPal * sizeof(EdgePal<R>) + size_t(slots_[k]->N) * sizeof(R4<R>);
             if (ldsSweeps(*slots_[k])) {
                 all.push_back(WgMap{k, 0, 0, 0});
                 lds_bytes_ = std::max(lds_bytes_, uint32_t(bytes));
@@ -2141,7 +2157,7 @@ private:
     // one-iteration run, with its nodes within one workgroup's LDS.
     bool ldsSweeps(const Slot<R> &sl) const {
         return tuning_.lds_sweep != 0 && shared_.iterations == 1 && sl.N > 0 &&
-               size_t(sl.N) * sizeof(R4<R>) <= lds_limit_;
+               kLdsPal * sizeof(EdgePal<R>) + size_t(sl.N) * sizeof(R4<R>) <= lds_limit_;
     }
 
     Batch<R> batch(const MapRef &r) const {

@@ -104,6 +104,7 @@ __device__ __forceinline__ u32 xcdTile(u32 w, u32 tiles) {
 
 template <class R> struct DevLattice {
     u32 N, B, D, blocks, node_tiles, bond_tiles, group_words;
+    u32 npal;         // edge palette entries
     u32 tensile_only; // every bond's compression and shear ramps are off: thr2 replaces thr
     R4<R> *u4;        // displacement, w = inv_mass
     R4<R> *u4b;       // the paired sweep's second node buffer
@@ -114,6 +115,8 @@ template <class R> struct DevLattice {
     const uint2 *stage_ranges;  // the colour stages as bond ranges [x, y), in sweep order
     const u32 *stage_end;       // stage s is stage_ranges[stage_end[s - 1] .. stage_end[s])
     u32 stage_count;
+    u32 lds_fast;               // every stage one range and the edge palette within kLdsPal
+    u32 stage_max;              // bonds in the largest stage
     R4<R> *up4;       // displacement at substep start
     R4<R> *v4;        // velocity
     const R4<R> *x04; // reference position, w = mass
@@ -1294,38 +1297,91 @@ __global__ __launch_bounds__(kPairThreads, sizeof(R) == 4 ? 8 : 4) void k_sweep_
 // back. A stage's bonds share no node, so the threads of a stage never touch
 // the same LDS entry, and each solve is solveBond on the same values the
 // stage-by-stage launches read from memory: the same bits. Launch one
-// workgroup per map entry (lat = the lattice) with N * sizeof(R4<R>) bytes of
-// dynamic LDS and 256, 512 or 1024 threads.
+// workgroup per map entry (lat = the lattice) with kLdsPal * sizeof(EdgePal)
+// + N * sizeof(R4<R>) bytes of dynamic LDS and 256, 512 or 1024 threads.
+//
+// When every stage is one bond range of at most kLdsBondRegs bonds per thread
+// and the edge palette fits its LDS slots (lds_fast), the palette is copied
+// into LDS and each stage's bond records and alive flags are loaded while the
+// stage before it is solved, so a stage waits on no memory round trip.
 constexpr int kSweepLdsThreads = 1024;
+constexpr u32 kLdsPal = 128;       // edge palette entries held in the LDS sweep's LDS
+constexpr u32 kLdsBondRegs = 4;    // bonds per thread per stage on the prefetching path
 template <class R, bool DIRECT, bool PLASTIC>
 __global__ __launch_bounds__(kSweepLdsThreads) void k_sweep_lds(Batch<R> B, Step<R> S) {
     extern __shared__ __align__(16) unsigned char sweep_lds[];
-    R4<R> *u = reinterpret_cast<R4<R> *>(sweep_lds);
+    EdgePal<R> *pal = reinterpret_cast<EdgePal<R> *>(sweep_lds);
+    R4<R> *u = reinterpret_cast<R4<R> *>(sweep_lds + kLdsPal * sizeof(EdgePal<R>));
     const DevLattice<R> L = B.L[B.map[blockIdx.x].lat];
     // Only the end of a substep (k_finish) sets stop, so it holds for the sweep.
     if (ldg(&L.ctl->stop)) return;
-    const u32 N = L.N, T = blockDim.x;
-    for (u32 i = threadIdx.x; i < N; i += T) u[i] = ldg(&L.u4[i]);
-    __syncthreads();
-    u32 q0 = 0;
-    for (u32 st = 0; st < L.stage_count; ++st) {
-        const u32 q1 = ldg(&L.stage_end[st]);
-        for (u32 q = q0; q < q1; ++q) {
-            const uint2 rg = ldg(&L.stage_ranges[q]);
-            for (u32 j = rg.x + threadIdx.x; j < rg.y; j += T) {
-                const BondRec<R> r = ldg(&L.rec[j]);
-                if (!ldg(&L.alive[j])) continue;
+    const u32 N = L.N, T = blockDim.x, t = threadIdx.x;
+    for (u32 i = t; i < N; i += T) u[i] = ldg(&L.u4[i]);
+    if (L.lds_fast && L.stage_max <= kLdsBondRegs * T) {
+        for (u32 k = t; k < L.npal; k += T) pal[k] = ldg(&L.epal[k]);
+        BondRec<R> ra[kLdsBondRegs], rb[kLdsBondRegs];
+        u32 ja[kLdsBondRegs], jb[kLdsBondRegs];
+        u8 la[kLdsBondRegs], lb[kLdsBondRegs];
+        // Stage st's records: bond j = lo + t + k * T for slot k.
+        auto fetch = [&](u32 st, BondRec<R> (&r)[kLdsBondRegs], u32 (&jj)[kLdsBondRegs], u8 (&l)[kLdsBondRegs]) {
+            const uint2 rg = ldg(&L.stage_ranges[st]);
+#pragma unroll
+            for (u32 k = 0; k < kLdsBondRegs; ++k) {
+                const u32 j = rg.x + t + k * T;
+                jj[k] = j;
+                l[k] = 0;
+                if (j < rg.y) {
+                    r[k] = ldg(&L.rec[j]);
+                    l[k] = ldg(&L.alive[j]);
+                }
+            }
+        };
+        if (L.stage_count > 0) fetch(0, ra, ja, la);
+        __syncthreads();
+        for (u32 st = 0; st < L.stage_count; ++st) {
+            if (st + 1 < L.stage_count) fetch(st + 1, rb, jb, lb);
+#pragma unroll
+            for (u32 k = 0; k < kLdsBondRegs; ++k) {
+                if (!la[k]) continue;
+                const BondRec<R> &r = ra[k];
                 R4<R> ua = u[r.a], ub = u[r.b];
                 R lam_out;
-                if (!solveBond<R, DIRECT, PLASTIC>(L, S, r, j, R(0), ua, ub, lam_out)) continue;
+                if (!solveBondWith<R, DIRECT, PLASTIC>(L, S, pal[r.p & ~kNoDir], r.a, r.b, r.c, ja[k], R(0), ua, ub,
+                                                       lam_out))
+                    continue;
                 u[r.a] = ua;
                 u[r.b] = ub;
             }
+            __syncthreads();
+#pragma unroll
+            for (u32 k = 0; k < kLdsBondRegs; ++k) {
+                ra[k] = rb[k];
+                ja[k] = jb[k];
+                la[k] = lb[k];
+            }
         }
-        q0 = q1;
+    } else {
         __syncthreads();
+        u32 q0 = 0;
+        for (u32 st = 0; st < L.stage_count; ++st) {
+            const u32 q1 = ldg(&L.stage_end[st]);
+            for (u32 q = q0; q < q1; ++q) {
+                const uint2 rg = ldg(&L.stage_ranges[q]);
+                for (u32 j = rg.x + t; j < rg.y; j += T) {
+                    const BondRec<R> r = ldg(&L.rec[j]);
+                    if (!ldg(&L.alive[j])) continue;
+                    R4<R> ua = u[r.a], ub = u[r.b];
+                    R lam_out;
+                    if (!solveBond<R, DIRECT, PLASTIC>(L, S, r, j, R(0), ua, ub, lam_out)) continue;
+                    u[r.a] = ua;
+                    u[r.b] = ub;
+                }
+            }
+            q0 = q1;
+            __syncthreads();
+        }
     }
-    for (u32 i = threadIdx.x; i < N; i += T) stg(&L.u4[i], u[i]);
+    for (u32 i = t; i < N; i += T) stg(&L.u4[i], u[i]);
 }
 
 // Radial pair damping, one colour stage (LatticePhysics.hpp:1171-1184).
