@@ -547,18 +547,24 @@ public:
         armed = false;
     }
 
-    // The paired sweep's plan (k_sweep_pair) of a one-iteration run. Nodes are
-    // grouped in tiles of kPairTile (kPairTileSmall below kPairWideNodes
-    // nodes) cells of the reference grid (the cell is
-    // the shortest rest length), ordered z, y, x, so that a workgroup's owned
-    // nodes fill whole cache lines and the workgroups next to it in the launch
-    // (which run on the same XCD) share its halo. Per stage pair, the
-    // workgroup of each tile lists its owned nodes, the halo they depend on and
-    // the bonds to solve; a tile whose lists exceed one workgroup is split in
-    // halves by node order until they fit (one owned node needs at most four
-    // nodes and two bonds). The stages are grouped as the stages left over
-    // from an even number of pairs, each in place, then the pairs, so that the
-    // pairs' ping-pong ends in u4. Bond entries get their palette index and
+    // The grouped sweep's plan (k_sweep_pair, k_sweep_triple) of a
+    // one-iteration run. Nodes are grouped in tiles of kPairTile
+    // (kPairTileSmall below kPairWideNodes nodes) cells of the reference grid
+    // (the cell is the shortest rest length), ordered z, y, x, so that a
+    // workgroup's owned nodes fill whole cache lines and the workgroups next
+    // to it in the launch (which run on the same XCD) share its halo. Per
+    // group of stages, the workgroup of each tile lists its owned nodes, the
+    // halo they depend on and each stage's bonds to solve; a tile whose lists
+    // exceed one workgroup is split in halves by node order until they fit
+    // (one owned node needs at most 2^K nodes and K bonds). The stages are
+    // grouped as a run of single stages, each in place, then an even number
+    // of pairs and triples, so that the groups' ping-pong ends in u4. A
+    // triple is used, on lattices of at least kPairWideNodes nodes, only where
+    // no tile of it splits: a split adds workgroups past the 1,024 a launch
+    // holds resident at once, and the launch then runs a second round. Among
+    // those choices the plan takes the one of least launch cost (single 11,
+    // pair 16, triple 22: the measured microseconds of each kind on the
+    // 128 x 64 x 128 impact). Bond entries get their palette index and
     // compliance when the record is built.
     void buildSweep(const blx_state_view &st, std::vector<uint32_t> &nodes, std::vector<PairBond<R>> &bonds) {
         sweep_groups.clear();
@@ -567,8 +573,6 @@ public:
         nodes.clear();
         bonds.clear();
         const size_t S = stages.size();
-        const size_t pairs = (S / 2) & ~size_t(1);
-        const size_t singles = S - 2 * pairs;
         const uint32_t *end_a = st.bond_a, *end_b = st.bond_b;
         auto single = [&](size_t s) {
             std::vector<uint2> runs;
@@ -586,128 +590,165 @@ public:
             sweep_first_stage.push_back(uint32_t(s));
             sweep_lds_nodes.push_back(0);
         };
+        if (S < 4) {
+            for (size_t s = 0; s < S; ++s) single(s);
+            return;
+        }
+        // Nodes in tile order (ties in node order).
+        double cell = std::numeric_limits<double>::infinity();
+        for (uint32_t k = 0; k < B; ++k)
+            if (st.rest_length[k] > 0.0) cell = std::min(cell, st.rest_length[k]);
+        if (!(cell < std::numeric_limits<double>::infinity())) cell = 1.0;
+        double lo[3] = {0.0, 0.0, 0.0};
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::numeric_limits<double>::infinity();
+            for (uint32_t i = 0; i < N; ++i) lo[c] = std::min(lo[c], st.x0[3 * size_t(i) + c]);
+        }
+        std::vector<uint64_t> key(N);
+        const uint32_t *tile = N >= kPairWideNodes ? kPairTile : kPairTileSmall;
+        for (uint32_t i = 0; i < N; ++i) {
+            uint64_t k = 0;
+            for (int c = 2; c >= 0; --c) {
+                const double g = std::floor((st.x0[3 * size_t(i) + c] - lo[c]) / cell + 0.5);
+                const uint64_t q = g > 0.0 ? std::min<uint64_t>(uint64_t(g) / tile[c], (1ULL << 21) - 1) : 0;
+                k = (k << 21) | q;
+            }
+            key[i] = k;
+        }
+        std::vector<uint32_t> order(N);
+        for (uint32_t i = 0; i < N; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
+        std::vector<uint32_t> tile_begin;
+        for (uint32_t q = 0; q < N; ++q)
+            if (q == 0 || key[order[q]] != key[order[q - 1]]) tile_begin.push_back(q);
+        tile_begin.push_back(N);
+
+        // partner[k][v]: the bond of stage s + k at node v, if any.
+        std::vector<uint32_t> partner[3], nstamp(N, 0), nslot(N, 0), bstamp(B, 0);
+        for (auto &p : partner) p.assign(N, kNone);
+        uint32_t stamp = 0;
+        std::vector<uint32_t> halo, level, list[3];
+        auto other = [&](uint32_t j, uint32_t v) { return end_a[j] == v ? end_b[j] : end_a[j]; };
+        auto setPartners = [&](size_t s, size_t K, bool on) {
+            for (size_t k = 0; k < K; ++k)
+                for (const uint2 &r : stages[s + k])
+                    for (uint32_t j = r.x; j < r.y; ++j) partner[k][end_a[j]] = partner[k][end_b[j]] = on ? j : kNone;
+        };
+        // The lists of one workgroup owning own[0 .. n) for a group of K
+        // stages; whether they fit one workgroup.
+        auto gather = [&](const uint32_t *own, uint32_t n, size_t K) {
+            if (++stamp == 0) throw Logic("grouped sweep plan stamp wrapped");
+            halo.clear();
+            for (size_t k = 0; k < K; ++k) list[k].clear();
+            for (uint32_t q = 0; q < n; ++q) {
+                nstamp[own[q]] = stamp;
+                nslot[own[q]] = q;
+            }
+            // level: the nodes whose values must be right after stage s + k,
+            // from the owned ones at the last stage back to stage s; every
+            // node a listed bond reaches joins it (and the halo).
+            level.assign(own, own + n);
+            for (size_t k = K; k-- > 0;) {
+                const size_t reach = level.size();
+                for (size_t q = 0; q < reach; ++q) {
+                    const uint32_t v = level[q], j = partner[k][v];
+                    if (j == kNone || bstamp[j] == stamp) continue;
+                    bstamp[j] = stamp;
+                    list[k].push_back(j);
+                    const uint32_t y = other(j, v);
+                    if (nstamp[y] != stamp) {
+                        nstamp[y] = stamp;
+                        halo.push_back(y);
+                        level.push_back(y);
+                    }
+                }
+            }
+            bool fits = n + halo.size() <= kPairMaxNodes;
+            for (size_t k = 0; k < K; ++k) fits = fits && list[k].size() <= kPairMaxBonds;
+            return fits;
+        };
+        uint32_t dir = 0, widest = 0;
+        std::vector<WgMap> g;
+        // One workgroup's lists, or two halves' when they do not fit.
+        std::function<void(const uint32_t *, uint32_t, size_t)> emit = [&](const uint32_t *own, uint32_t n, size_t K) {
+            if (!gather(own, n, K)) {
+                if (n < 2) throw Logic("a grouped sweep tile of one node does not fit a workgroup");
+                const uint32_t h = n / 2;
+                emit(own, h, K);
+                emit(own + h, n - h, K);
+                return;
+            }
+            std::sort(halo.begin(), halo.end());
+            for (size_t q = 0; q < halo.size(); ++q) nslot[halo[q]] = uint32_t(n + q);
+            const uint32_t all = uint32_t(n + halo.size());
+            const uint32_t kind = K == 3 ? kSweepTriple : kSweepPair;
+            g.push_back(WgMap{0, uint32_t(nodes.size()), uint32_t(bonds.size()), (kind << 16) | dir});
+            g.push_back(WgMap{n | (all << 16), uint32_t(list[0].size()) | (uint32_t(list[1].size()) << 16),
+                              K == 3 ? uint32_t(list[2].size()) : 0U, 0});
+            nodes.insert(nodes.end(), own, own + n);
+            nodes.insert(nodes.end(), halo.begin(), halo.end());
+            for (size_t k = 0; k < K; ++k) {
+                std::sort(list[k].begin(), list[k].end());
+                for (uint32_t j : list[k]) {
+                    PairBond<R> e{};
+                    e.j = j;
+                    e.slots = nslot[end_a[j]] | (nslot[end_b[j]] << 16);
+                    bonds.push_back(e);
+                }
+            }
+            if (nodes.size() >= 0xffffffffULL || bonds.size() >= 0xffffffffULL)
+                throw Invalid("the grouped sweep holds at most 2^32 - 1 node and bond entries");
+            widest = std::max(widest, all);
+        };
+
+        // Where a triple fits without a split.
+        std::vector<char> triple(S, 0);
+        if (N >= kPairWideNodes)
+            for (size_t s = 0; s + 3 <= S; ++s) {
+                setPartners(s, 3, true);
+                bool ok = true;
+                for (size_t t = 0; ok && t + 1 < tile_begin.size(); ++t)
+                    ok = gather(order.data() + tile_begin[t], tile_begin[t + 1] - tile_begin[t], 3);
+                setPartners(s, 3, false);
+                triple[s] = ok;
+            }
+        // Least cost of the groups covering stages s .. S with `odd` groups
+        // before them, ending on an even count; then the run of singles.
+        constexpr double kSingle = 11.0, kPair = 16.0, kTriple = 22.0, kNever = 1e30;
+        std::vector<double> cost(2 * (S + 1), kNever);
+        std::vector<uint8_t> take(2 * (S + 1), 0);
+        cost[2 * S] = 0.0;
+        for (size_t s = S; s-- > 0;)
+            for (int odd = 0; odd < 2; ++odd) {
+                double &c = cost[2 * s + odd];
+                if (s + 2 <= S && kPair + cost[2 * (s + 2) + (odd ^ 1)] < c) {
+                    c = kPair + cost[2 * (s + 2) + (odd ^ 1)];
+                    take[2 * s + odd] = 2;
+                }
+                if (s + 3 <= S && triple[s] && kTriple + cost[2 * (s + 3) + (odd ^ 1)] < c) {
+                    c = kTriple + cost[2 * (s + 3) + (odd ^ 1)];
+                    take[2 * s + odd] = 3;
+                }
+            }
+        size_t singles = S;
+        for (size_t f = 0; f <= S; ++f)
+            if (kSingle * f + cost[2 * f] < kSingle * singles + cost[2 * singles]) singles = f;
         size_t s = 0;
         while (s < singles) single(s++);
-        if (pairs > 0) {
-            // Nodes in tile order (ties in node order).
-            double cell = std::numeric_limits<double>::infinity();
-            for (uint32_t k = 0; k < B; ++k)
-                if (st.rest_length[k] > 0.0) cell = std::min(cell, st.rest_length[k]);
-            if (!(cell < std::numeric_limits<double>::infinity())) cell = 1.0;
-            double lo[3] = {0.0, 0.0, 0.0};
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = std::numeric_limits<double>::infinity();
-                for (uint32_t i = 0; i < N; ++i) lo[c] = std::min(lo[c], st.x0[3 * size_t(i) + c]);
-            }
-            std::vector<uint64_t> key(N);
-            const uint32_t *tile = N >= kPairWideNodes ? kPairTile : kPairTileSmall;
-            for (uint32_t i = 0; i < N; ++i) {
-                uint64_t k = 0;
-                for (int c = 2; c >= 0; --c) {
-                    const double g = std::floor((st.x0[3 * size_t(i) + c] - lo[c]) / cell + 0.5);
-                    const uint64_t q = g > 0.0 ? std::min<uint64_t>(uint64_t(g) / tile[c], (1ULL << 21) - 1) : 0;
-                    k = (k << 21) | q;
-                }
-                key[i] = k;
-            }
-            std::vector<uint32_t> order(N);
-            for (uint32_t i = 0; i < N; ++i) order[i] = i;
-            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
-            std::vector<uint32_t> tile_begin;
-            for (uint32_t q = 0; q < N; ++q)
-                if (q == 0 || key[order[q]] != key[order[q - 1]]) tile_begin.push_back(q);
-            tile_begin.push_back(N);
-
-            std::vector<uint32_t> p1(N, kNone), p2(N, kNone), nstamp(N, 0), nslot(N, 0), bstamp(B, 0);
-            uint32_t stamp = 0;
-            std::vector<uint32_t> halo, level1, b1, b2;
-            auto other = [&](uint32_t j, uint32_t v) { return end_a[j] == v ? end_b[j] : end_a[j]; };
-            uint32_t dir = 0, widest = 0;
-            std::vector<WgMap> g;
-            // One workgroup's lists for the owned nodes own[0 .. n), or two
-            // halves' when they do not fit.
-            std::function<void(const uint32_t *, uint32_t)> emit = [&](const uint32_t *own, uint32_t n) {
-                if (++stamp == 0) throw Logic("paired sweep plan stamp wrapped");
-                halo.clear();
-                level1.clear();
-                b1.clear();
-                b2.clear();
-                for (uint32_t q = 0; q < n; ++q) {
-                    nstamp[own[q]] = stamp;
-                    nslot[own[q]] = q;
-                }
-                auto need = [&](uint32_t v) {
-                    if (nstamp[v] != stamp) {
-                        nstamp[v] = stamp;
-                        halo.push_back(v);
-                    }
-                };
-                for (uint32_t q = 0; q < n; ++q) {
-                    const uint32_t j = p2[own[q]];
-                    if (j == kNone) continue;
-                    const uint32_t y = other(j, own[q]);
-                    if (nstamp[y] != stamp) level1.push_back(y);
-                    need(y);
-                    if (bstamp[j] != stamp) {
-                        bstamp[j] = stamp;
-                        b2.push_back(j);
-                    }
-                }
-                auto stage_s = [&](uint32_t v) {
-                    const uint32_t j = p1[v];
-                    if (j == kNone) return;
-                    need(other(j, v));
-                    if (bstamp[j] != stamp) {
-                        bstamp[j] = stamp;
-                        b1.push_back(j);
-                    }
-                };
-                for (uint32_t q = 0; q < n; ++q) stage_s(own[q]);
-                for (uint32_t v : level1) stage_s(v);
-                if (n + halo.size() > kPairMaxNodes || b1.size() > kPairMaxBonds || b2.size() > kPairMaxBonds) {
-                    const uint32_t h = n / 2;
-                    emit(own, h);
-                    emit(own + h, n - h);
-                    return;
-                }
-                std::sort(halo.begin(), halo.end());
-                for (size_t q = 0; q < halo.size(); ++q) nslot[halo[q]] = uint32_t(n + q);
-                std::sort(b1.begin(), b1.end());
-                std::sort(b2.begin(), b2.end());
-                const uint32_t all = uint32_t(n + halo.size());
-                g.push_back(WgMap{0, uint32_t(nodes.size()), uint32_t(bonds.size()), (kSweepPair << 16) | dir});
-                g.push_back(WgMap{n | (all << 16), uint32_t(b1.size()) | (uint32_t(b2.size()) << 16), 0, 0});
-                nodes.insert(nodes.end(), own, own + n);
-                nodes.insert(nodes.end(), halo.begin(), halo.end());
-                for (const std::vector<uint32_t> *list : {&b1, &b2})
-                    for (uint32_t j : *list) {
-                        PairBond<R> e{};
-                        e.j = j;
-                        e.slots = nslot[end_a[j]] | (nslot[end_b[j]] << 16);
-                        bonds.push_back(e);
-                    }
-                if (nodes.size() >= 0xffffffffULL || bonds.size() >= 0xffffffffULL)
-                    throw Invalid("the paired sweep holds at most 2^32 - 1 node and bond entries");
-                widest = std::max(widest, all);
-            };
-            for (size_t p = 0; p < pairs; ++p, s += 2) {
-                for (const uint2 &r : stages[s])
-                    for (uint32_t j = r.x; j < r.y; ++j) p1[end_a[j]] = p1[end_b[j]] = j;
-                for (const uint2 &r : stages[s + 1])
-                    for (uint32_t j = r.x; j < r.y; ++j) p2[end_a[j]] = p2[end_b[j]] = j;
-                dir = (p & 1) ? kSweepReverse : 0U;
-                widest = 0;
-                g.clear();
-                for (size_t t = 0; t + 1 < tile_begin.size(); ++t)
-                    emit(order.data() + tile_begin[t], tile_begin[t + 1] - tile_begin[t]);
-                for (const uint2 &r : stages[s])
-                    for (uint32_t j = r.x; j < r.y; ++j) p1[end_a[j]] = p1[end_b[j]] = kNone;
-                for (const uint2 &r : stages[s + 1])
-                    for (uint32_t j = r.x; j < r.y; ++j) p2[end_a[j]] = p2[end_b[j]] = kNone;
-                sweep_groups.push_back(g);
-                sweep_first_stage.push_back(uint32_t(s));
-                sweep_lds_nodes.push_back(widest);
-            }
+        for (size_t p = 0; s < S; ++p) {
+            const size_t K = take[2 * s + (p & 1)];
+            if (K != 2 && K != 3) throw Logic("grouped sweep plan without a cover");
+            setPartners(s, K, true);
+            dir = (p & 1) ? kSweepReverse : 0U;
+            widest = 0;
+            g.clear();
+            for (size_t t = 0; t + 1 < tile_begin.size(); ++t)
+                emit(order.data() + tile_begin[t], tile_begin[t + 1] - tile_begin[t], K);
+            setPartners(s, K, false);
+            sweep_groups.push_back(g);
+            sweep_first_stage.push_back(uint32_t(s));
+            sweep_lds_nodes.push_back(widest);
+            s += K;
         }
     }
 
@@ -1953,6 +1994,7 @@ private:
     std::vector<MapRef> sweep_maps_;     // the paired sweep's launch groups
     std::vector<uint32_t> sweep_phase_;  // reference phase id of each group (5 interior, 6 boundary)
     std::vector<uint32_t> sweep_lds_;    // LDS bytes of each group's launch
+    std::vector<uint32_t> sweep_k_;      // 3 where a group holds a triple (k_sweep_triple), else 2
     std::vector<int64_t> sweep_stage_;   // the stage a group of single entries runs by stage map, or -1
     // A striker substep splits across streams (striker pass 2 beside the end
     // sample of what it cannot reach) when a running lattice has at least
@@ -2104,6 +2146,7 @@ private:
         sweep_maps_.assign(groups, MapRef{});
         sweep_phase_.assign(groups, 5U);
         sweep_lds_.assign(groups, 0U);
+        sweep_k_.assign(groups, 2U);
         for (size_t g = 0; g < groups; ++g) {
             begin(sweep_maps_[g]);
             for (uint32_t k : staged) {
@@ -2111,6 +2154,7 @@ private:
                 if (g >= sl.sweep_groups.size()) continue;
                 if (sl.sweep_first_stage[g] >= sl.interior_stages) sweep_phase_[g] = 6U;
                 sweep_lds_[g] = std::max(sweep_lds_[g], uint32_t(sl.sweep_lds_nodes[g] * sizeof(R4<R>)));
+                if (((sl.sweep_groups[g][0].pad >> 16) & 0xfU) == kSweepTriple) sweep_k_[g] = 3U;
                 // Two entries per workgroup: the second holds counts, not a lattice.
                 const std::vector<WgMap> &es = sl.sweep_groups[g];
                 for (size_t q = 0; q < es.size(); q += 2) {
@@ -2219,12 +2263,13 @@ private:
     // reference phase id it belongs to (FastLattice.cpp:22-26).
     enum Kernel : uint32_t { KStrainStart, KBondStart, KPredict, KPass1, KSolve, KProject, KVelocity, KDamp,
                              KSupportVelocity, KPass2, KStrain, KBondEnd, KFinish, KRinv, KMark, KStrainLate,
-                             KBondEndLate, KContact, KSweepPair, KCount };
+                             KBondEndLate, KContact, KSweepPair, KSweepTriple, KCount };
     static const char *kernelName(uint32_t k) {
         static const char *names[KCount] = {"k_strain_start", "k_bond_start", "k_predict", "k_pass1", "k_solve",
                                             "k_project", "k_velocity", "k_damp", "k_support_velocity", "k_pass2",
                                             "k_strain", "k_bond_end", "k_finish", "k_rinv", "k_mark_affected",
-                                            "k_strain_late", "k_bond_end_late", "k_node_contact", "k_sweep_pair"};
+                                            "k_strain_late", "k_bond_end_late", "k_node_contact", "k_sweep_pair",
+                                            "k_sweep_triple"};
         return k < KCount ? names[k] : "unknown";
     }
     struct ProfRecord { uint32_t kernel, phase; hipEvent_t a, b; };
@@ -2486,7 +2531,8 @@ private:
                 const uint32_t lds = sweep_lds_[g];
                 const bool last = own && g + 1 == sweep_maps_.size();
                 const bool single = sweep_stage_[g] >= 0;
-                ProfRecord *pr = profLaunch(single ? KSolve : KSweepPair, sweep_phase_[g]);
+                const bool triple = sweep_k_[g] == 3U;
+                ProfRecord *pr = profLaunch(single ? KSolve : (triple ? KSweepTriple : KSweepPair), sweep_phase_[g]);
                 hipEvent_t start = pr ? pr->a : nullptr, stop = last ? ev_sweep_ : (pr ? pr->b : nullptr);
                 if (capturing_) {
                     // A captured launch takes no events; the signal is a graph edge.
@@ -2502,6 +2548,12 @@ private:
                     else
                         hipExtLaunchKernelGGL((k_solve<R, DIRECT, false>), tiles(sm), dim3(kWG), 0, stream_, start, stop,
                                               0, batch(sm), S, kSolveFirst);
+                } else if (triple && plastic) {
+                    hipExtLaunchKernelGGL((k_sweep_triple<R, DIRECT, true>), grid, dim3(kPairThreads), lds, stream_,
+                                          start, stop, 0, batch(m), S);
+                } else if (triple) {
+                    hipExtLaunchKernelGGL((k_sweep_triple<R, DIRECT, false>), grid, dim3(kPairThreads), lds, stream_,
+                                          start, stop, 0, batch(m), S);
                 } else if (plastic) {
                     hipExtLaunchKernelGGL((k_sweep_pair<R, DIRECT, true>), grid, dim3(kPairThreads), lds, stream_, start,
                                           stop, 0, batch(m), S);

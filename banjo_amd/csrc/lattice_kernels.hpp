@@ -1165,28 +1165,35 @@ __global__ __launch_bounds__(kWG) void k_solve(Batch<R> B, Step<R> S, u32 flags)
     stStream(&L.u4[r.b], ub);
 }
 
-// The paired colour sweep of a one-iteration run: two consecutive colour
-// stages s, s + 1 in one launch, each workgroup on a tile of nodes staged in
-// LDS. A workgroup owns a set of nodes and loads, besides them, every node the
-// owned nodes' final values depend on: for an owned node x with stage-s bond
-// partner p1(x) and stage-(s + 1) partner p2(x), those are p1(x), p2(x) and
-// p1(p2(x)). It solves every stage-s bond at x or at p2(x), then every
-// stage-(s + 1) bond at x, on its LDS copies (a stage shares no node, so no
-// two threads touch one slot), and writes back only the owned nodes. A bond
-// solved by two workgroups is solved from the same inputs in both, so each
-// owned node gets the bits the stage-by-stage sweep gives. Workgroups read
-// one node buffer and write the other (u4 <-> u4b), so none reads a node
-// another has already written; every node is owned by one workgroup, so the
-// written buffer is whole. Pairs alternate direction and a sweep has an even
-// number of them, so it ends in u4; the stages left over run in place on u4
-// as single entries. Each workgroup has two map entries, read together:
+// The grouped colour sweep of a one-iteration run: K = 2 (a pair) or 3 (a
+// triple) consecutive colour stages s .. s + K - 1 in one launch, each
+// workgroup on a tile of nodes staged in LDS. A workgroup owns a set of nodes
+// and loads, besides them, every node the owned nodes' final values depend
+// on. Going back from the last stage: the owned nodes need their stage
+// s + K - 1 bonds solved, those bonds' other ends need their values after
+// stage s + K - 2, so their stage s + K - 2 bonds are solved too, and so on
+// down to stage s, whose bonds' ends are loaded as they stand before the
+// launch. For a pair and an owned node x with stage-s partner p1(x) and
+// stage-(s + 1) partner p2(x), that is p1(x), p2(x) and p1(p2(x)). The
+// workgroup solves its stage lists in order on its LDS copies (a stage shares
+// no node, so no two threads touch one slot) and writes back only the owned
+// nodes. A bond solved by two workgroups is solved from the same inputs in
+// both, so each owned node gets the bits the stage-by-stage sweep gives.
+// Workgroups read one node buffer and write the other (u4 <-> u4b), so none
+// reads a node another has already written; every node is owned by one
+// workgroup, so the written buffer is whole. Groups alternate direction and
+// a sweep has an even number of them, so it ends in u4; the stages left over
+// run in place on u4 as single entries. Each workgroup has two map entries,
+// read together:
 //   single: {lat, a, b, pad}: a, b = bond range of one stage (at most
 //           kPairThreads bonds); the second entry unused;
-//   pair:   {lat, node list offset, bond list offset, pad},
-//           {owned | all nodes << 16, stage-s | stage-(s + 1) bonds << 16, 0, 0};
+//   group:  {lat, node list offset, bond list offset, pad},
+//           {owned | all nodes << 16, stage-s | stage-(s + 1) bonds << 16,
+//            stage-(s + 2) bonds (0 in a pair), 0};
 // pad = kind (bits 16-19) | direction (bit 20: u4b -> u4). A launch has
-// xcdGrid(count / 2) workgroups.
-constexpr u32 kSweepSingle = 0U, kSweepPair = 1U;
+// xcdGrid(count / 2) workgroups; a triple launch also runs pair entries (a
+// batch merges its lattices' groups by position).
+constexpr u32 kSweepSingle = 0U, kSweepPair = 1U, kSweepTriple = 2U;
 constexpr u32 kSweepReverse = 1U << 20;
 constexpr u32 kPairThreads = 512;
 constexpr u32 kPairNodeRegs = 4;  // node slots per thread: a workgroup holds at most 2048 nodes
@@ -1219,10 +1226,8 @@ __device__ __forceinline__ void pairStage(const DevLattice<R> &L, const Step<R> 
     }
 }
 
-// Eight waves per SIMD in float: the launch's workgroups of a full lattice
-// are all resident at once, so none waits for a second round.
-template <class R, bool DIRECT, bool PLASTIC>
-__global__ __launch_bounds__(kPairThreads, sizeof(R) == 4 ? 8 : 4) void k_sweep_pair(Batch<R> B, Step<R> S) {
+template <class R, bool DIRECT, bool PLASTIC, int K>
+__device__ __forceinline__ void sweepGroup(const Batch<R> &B, const Step<R> &S) {
     extern __shared__ __align__(16) unsigned char pair_lds[];
     R4<R> *u = reinterpret_cast<R4<R> *>(pair_lds);
     const u32 wgs = B.count / 2, w = xcdTile(blockIdx.x, wgs);
@@ -1248,7 +1253,8 @@ __global__ __launch_bounds__(kPairThreads, sizeof(R) == 4 ? 8 : 4) void k_sweep_
     const R4<R> *src = rev ? L.u4b : L.u4;
     R4<R> *dst = rev ? L.u4 : L.u4b;
     // The node ids and bond entries go out with the stop flag.
-    const u32 n_own = m1.lat & 0xffffU, n_all = m1.lat >> 16, n1 = m1.a & 0xffffU, n2 = m1.a >> 16;
+    const u32 n_own = m1.lat & 0xffffU, n_all = m1.lat >> 16;
+    const u32 nb[3] = {m1.a & 0xffffU, m1.a >> 16, K > 2 ? m1.b : 0U};
     const u32 *nodes = L.pnodes + m->a;
     const PairBond<R> *bonds = L.pbonds + m->b;
     u32 id[kPairNodeRegs];
@@ -1257,14 +1263,15 @@ __global__ __launch_bounds__(kPairThreads, sizeof(R) == 4 ? 8 : 4) void k_sweep_
         const u32 k = t + q * kPairThreads;
         id[q] = k < n_all ? ldStream(&nodes[k]) : 0U;
     }
-    PairBond<R> e1[kPairBondRegs] = {}, e2[kPairBondRegs] = {};
-    u8 l1[kPairBondRegs] = {}, l2[kPairBondRegs] = {};
+    PairBond<R> e[K][kPairBondRegs] = {};
+    u8 l[K][kPairBondRegs] = {};
 #pragma unroll
-    for (u32 q = 0; q < kPairBondRegs; ++q) {
-        const u32 k = t + q * kPairThreads;
-        if (k < n1) e1[q] = ldStream(&bonds[k]);
-        if (k < n2) e2[q] = ldStream(&bonds[n1 + k]);
-    }
+    for (int g = 0, at = 0; g < K; at += nb[g], ++g)
+#pragma unroll
+        for (u32 q = 0; q < kPairBondRegs; ++q) {
+            const u32 k = t + q * kPairThreads;
+            if (k < nb[g]) e[g][q] = ldStream(&bonds[at + k]);
+        }
     // Only the end of a substep (k_finish) sets stop, so it holds for the launch.
     if (L.ctl->stop) return;
 #pragma unroll
@@ -1273,21 +1280,34 @@ __global__ __launch_bounds__(kPairThreads, sizeof(R) == 4 ? 8 : 4) void k_sweep_
         if (k < n_all) u[k] = ldg(&src[id[q]]);
     }
 #pragma unroll
-    for (u32 q = 0; q < kPairBondRegs; ++q) {
-        const u32 k = t + q * kPairThreads;
-        if (k < n1) l1[q] = ldg(&L.alive[e1[q].j]);
-        if (k < n2) l2[q] = ldg(&L.alive[e2[q].j]);
+    for (int g = 0; g < K; ++g)
+#pragma unroll
+        for (u32 q = 0; q < kPairBondRegs; ++q) {
+            const u32 k = t + q * kPairThreads;
+            if (k < nb[g]) l[g][q] = ldg(&L.alive[e[g][q].j]);
+        }
+#pragma unroll
+    for (int g = 0; g < K; ++g) {
+        __syncthreads();
+        pairStage<R, DIRECT, PLASTIC>(L, S, nodes, u, e[g], l[g], nb[g]);
     }
-    __syncthreads();
-    pairStage<R, DIRECT, PLASTIC>(L, S, nodes, u, e1, l1, n1);
-    __syncthreads();
-    pairStage<R, DIRECT, PLASTIC>(L, S, nodes, u, e2, l2, n2);
     __syncthreads();
 #pragma unroll
     for (u32 q = 0; q < kPairNodeRegs; ++q) {
         const u32 k = t + q * kPairThreads;
         if (k < n_own) stStream(&dst[id[q]], u[k]);
     }
+}
+
+// Eight waves per SIMD in float: the launch's workgroups of a full lattice
+// are all resident at once, so none waits for a second round.
+template <class R, bool DIRECT, bool PLASTIC>
+__global__ __launch_bounds__(kPairThreads, sizeof(R) == 4 ? 8 : 4) void k_sweep_pair(Batch<R> B, Step<R> S) {
+    sweepGroup<R, DIRECT, PLASTIC, 2>(B, S);
+}
+template <class R, bool DIRECT, bool PLASTIC>
+__global__ __launch_bounds__(kPairThreads, sizeof(R) == 4 ? 8 : 4) void k_sweep_triple(Batch<R> B, Step<R> S) {
+    sweepGroup<R, DIRECT, PLASTIC, 3>(B, S);
 }
 
 // The whole colour sweep of a one-iteration run for a lattice whose nodes fit

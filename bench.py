@@ -67,6 +67,7 @@ def algorithmic_bytes(N, B, stages, sphere=True, plastic=False, iterations=1):
     sweep_bond = 37 + (4 if iterations > 1 else 0) + (4 if plastic else 0)
     per["k_solve"] = (B * sweep_bond + N * 28) / max(stages, 1)
     per["k_sweep_pair"] = 2 * per["k_solve"]  # two colour stages per launch
+    per["k_sweep_triple"] = 3 * per["k_solve"]  # three colour stages per launch
     per["k_velocity"] = N * 38
     per["k_strain"] = B * 42 + N * 84
     per["k_bond_end"] = B * (88 + (16 if plastic else 0)) + N * 37

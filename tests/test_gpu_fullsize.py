@@ -5,7 +5,8 @@ config-5 plank cut (256x32x32: 262,144 nodes, 3,253,116 bonds) are checked
 through what holds at any size:
 - the first 14 substeps equal the oracle bit for bit (the striker's first
   contacts, the start sample of every node and the first substeps whose
-  candidates overflow the block's list included);
+  candidates overflow the block's list included; at this size the colour
+  sweep runs as stage pairs and stage triples on LDS tiles);
 - a run repeats bit for bit;
 - the device's fragment labels equal the host union-find on the same bonds;
 - a batch slot equals the plain backend;
