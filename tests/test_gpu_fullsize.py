@@ -7,6 +7,8 @@ through what holds at any size:
   contacts, the start sample of every node and the first substeps whose
   candidates overflow the block's list included; at this size the colour
   sweep runs as stage pairs and stage triples on LDS tiles);
+- the first 4 substeps of the impact in double, and of a 96x48x128 block
+  (whose stages group differently), equal the oracle bit for bit;
 - a run repeats bit for bit;
 - the device's fragment labels equal the host union-find on the same bonds;
 - a batch slot equals the plain backend;
@@ -40,6 +42,21 @@ def test_config2_first_substeps_equal_the_oracle(gpu, oracle):
     o = run_scene(oracle, sc, F)
     assert h["status"].contact.impulse_contacts > 0
     assert h["status"].contact.candidate_overflow > 0
+    assert_same(h, o)
+
+
+@pytest.mark.parametrize("dims,precision", [((128, 64, 128), abi.PRECISION_DOUBLE), ((96, 48, 128), F)],
+                         ids=["config2-double", "96x48x128-float"])
+def test_grouped_sweep_sizes_equal_the_oracle(gpu, oracle, dims, precision):
+    # Lattices of at least 524,288 nodes sweep in stage pairs and, where every
+    # tile's three-stage lists fit one workgroup, stage triples; which stages
+    # group depends on the lattice's shape. The double lane's kernels hold
+    # more registers per thread than the float lane's.
+    sc = scenes.impact(*dims, substeps=4)
+    print(f"impact {dims}, {'double' if precision == abi.PRECISION_DOUBLE else 'float'}, 4 substeps", flush=True)
+    h = run_scene(gpu, sc, precision, labels="device")
+    o = run_scene(oracle, sc, precision)
+    assert h["status"].contact.impulse_contacts > 0
     assert_same(h, o)
 
 
