@@ -540,6 +540,7 @@ template <class R> __global__ void k_edge_palette(EdgePal<R> *pal, u32 *no_dir, 
     }
 }
 
+template <bool TENSILE, bool PLASTIC>
 __device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, float ms, float mp,
                                               u32 (*s_max)[kWG / 64]);
 
@@ -579,7 +580,7 @@ template <class R, bool DIRECT> __global__ __launch_bounds__(kWG) void k_bond_st
             L.pyz[o] = R2<R>{pc, ps};
         }
     }
-    if (L.tensile_only) publishMaxima(L.ctl, 0.f, mc, ms, 0.f, s_max);
+    if (L.tensile_only) publishMaxima<false, false>(L.ctl, 0.f, mc, ms, 0.f, s_max);
 }
 
 // The previous compressive and shear samples of the live bonds of a
@@ -1554,7 +1555,10 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
     const u32 j = r.j;
     // Every gather the bond needs goes out in one batch, before any test on
     // what it returns: one memory round trip after the record, not three.
-    Thresholds<R> th{};
+    // A tensile-only lattice reads only ds and de (its compression and shear
+    // ramps are 0 without them), and its previous compressive and shear
+    // samples are 0.
+    Thresholds<R> th;
     R2<R> pyz{R(0), R(0)};
     if (L.tensile_only) {
         const Tensile<R> t2 = ldStream(&L.thr2[o]);
@@ -1571,7 +1575,7 @@ __device__ __forceinline__ void endSampleBond(const DevLattice<R> &L, const Step
     }
     const R4<R> ua = ldg(&L.u4[r.a]), ub = ldg(&L.u4[r.b]);
     const EdgePal<R> g = ldg(&L.epal[r.p & ~kNoDir]);
-    R4<R> st[4] = {};
+    R4<R> st[4];  // read only when loaded (a bond with a rest direction)
     loadEndStrains(L, r.a, r.b, r.p, st);
     R plastic = R(0), ps = R(0);
     if (PLASTIC) {
@@ -1695,10 +1699,14 @@ __device__ __forceinline__ u32 wavePeakBits(float x) {
 }
 
 // Running maxima of non-negative floats: integer atomicMax on the bits of
-// the workgroup's largest positive value. Every thread calls it.
+// the workgroup's largest positive value. Every thread calls it. A caller
+// whose tensile (TENSILE false) or plastic (PLASTIC false) values are all 0
+// skips that fold.
+template <bool TENSILE, bool PLASTIC>
 __device__ __forceinline__ void publishMaxima(DevCtl *ctl, float mt, float mc, float ms, float mp, u32 (*s_max)[kWG / 64]) {
     const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
-    const u32 bt = wavePeakBits(mt), bc = wavePeakBits(mc), bs = wavePeakBits(ms), bp = wavePeakBits(mp);
+    const u32 bt = TENSILE ? wavePeakBits(mt) : 0U, bc = wavePeakBits(mc), bs = wavePeakBits(ms),
+              bp = PLASTIC ? wavePeakBits(mp) : 0U;
     if (lane == 0) {
         s_max[0][wave] = bt;
         s_max[1][wave] = bc;
@@ -1770,7 +1778,7 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
             }
             appendDirty(L.dirty_list + size_t(half) * L.N, &ctl->dirty_count[half], claim_a, claim_b);
         }
-        publishMaxima(ctl, mt, mc, ms, mp, s_max);
+        publishMaxima<true, PLASTIC>(ctl, mt, mc, ms, mp, s_max);
         return;
     }
     // A span of up to kEndSpan consecutive tiles of kWG bonds (m->a, m->b).
@@ -1795,7 +1803,7 @@ __global__ __launch_bounds__(kWG) void k_bond_end(Batch<R> B, Step<R> S) {
             __syncthreads();
         }
     }
-    publishMaxima(ctl, mt, mc, ms, mp, s_max);
+    publishMaxima<true, PLASTIC>(ctl, mt, mc, ms, mp, s_max);
 }
 
 // ---------------------------------------------------------------------------
