@@ -1674,6 +1674,19 @@ public:
         if (c.stop && c.exit_reason) o.exit_reason = c.exit_reason == kBudgetSpent ? 3U : c.exit_reason;
     }
 
+    // The exit reason status() reports, for slots 0 .. count - 1 (0 while a
+    // slot runs or holds no lattice).
+    void exitReasons(uint32_t *out, uint32_t count) const {
+        if (count > slots_.size()) throw Invalid("more slots asked for than the batch holds");
+        for (uint32_t k = 0; k < count; ++k) {
+            const Slot<R> &sl = *slots_[k];
+            const DevCtl &c = sl.ctl_host;
+            uint32_t r = sl.status.exit_reason;
+            if (c.stop && c.exit_reason) r = c.exit_reason == kBudgetSpent ? 3U : c.exit_reason;
+            out[k] = r;
+        }
+    }
+
     void download(uint32_t k, blx_state_view &st, blx_sphere &sp) {
         Slot<R> &sl = slot(k);
         if (!sl.uploaded) throw Logic("download before upload");
@@ -3039,6 +3052,12 @@ int blx_batch_advance(blx_batch *b, uint64_t substeps, uint64_t steps_per_launch
 }
 int blx_batch_status(const blx_batch *b, uint32_t slot, blx_run_status *st) {
     return guard([&] { dispatch(CH(b), [&](auto &e) { e.status(slot, *st); }); });
+}
+int blx_batch_exit_reasons(const blx_batch *b, uint32_t *reasons, uint32_t count) {
+    return guard([&] {
+        if (!reasons && count) throw Invalid("null argument");
+        dispatch(CH(b), [&](auto &e) { e.exitReasons(reasons, count); });
+    });
 }
 int blx_batch_download(blx_batch *b, uint32_t slot, blx_state_view *st, blx_sphere *sp) {
     return guard([&] { dispatch(H(b), [&](auto &e) { e.download(slot, *st, *sp); }); });

@@ -544,6 +544,13 @@ class Batch(_Tunable):
         self.L.check(self.L.fn("batch_status")(self.h, C.c_uint32(slot), C.byref(st)), "batch_status")
         return st
 
+    def exit_reasons(self, count):
+        """The exit reason of slots 0 .. count - 1 (0 while a slot runs), in one call."""
+        out = np.zeros(int(count), dtype=np.uint32)
+        self.L.check(self.L.fn("batch_exit_reasons")(self.h, out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                      C.c_uint32(int(count))), "batch_exit_reasons")
+        return out
+
     def download(self, slot, state: State):
         sphere = abi.Sphere()
         sv = state.view()

@@ -154,10 +154,10 @@ class TrialSession:
         tb = self.clock()
         self.seconds["advance"] += tb - ta
         answers = []
-        for slot in list(self.active):
+        # One call for every slot's exit reason; the full status only for those that stopped.
+        reasons = self.batch.exit_reasons(self.max_templates + self.slots)
+        for slot in [k for k in self.active if reasons[k] != 0]:
             st = self.batch.status(slot)
-            if st.exit_reason == 0:
-                continue
             t, admitted, need = self.active.pop(slot)
             labels, fragments = self.batch.fragment_labels(slot)
             sp = self.batch.sphere(slot)

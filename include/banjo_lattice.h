@@ -543,6 +543,11 @@ int blx_batch_arm(blx_batch *batch, uint32_t slot, const blx_run_control *contro
 int blx_batch_advance(blx_batch *batch, uint64_t substeps, uint64_t steps_per_launch,
                       const volatile uint8_t *abandon, uint32_t *running);
 int blx_batch_status(const blx_batch *batch, uint32_t slot, blx_run_status *status);
+/* The exit reason blx_batch_status reports, for slots 0 .. count - 1 at once
+ * (0 while a slot runs or holds no lattice). No counterpart in the reference
+ * (whose backends run one lattice): it is for callers that scan many slots
+ * after each advance, such as a server retiring the trials that stopped. */
+int blx_batch_exit_reasons(const blx_batch *batch, uint32_t *reasons, uint32_t count);
 int blx_batch_download(blx_batch *batch, uint32_t slot, blx_state_view *state, blx_sphere *sphere);
 int blx_batch_fragment_labels(blx_batch *batch, uint32_t slot, uint32_t *label,
                               uint32_t *component_count);
