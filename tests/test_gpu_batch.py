@@ -54,6 +54,31 @@ def test_batch_uneven_budgets_and_exit_rules(gpu):
         assert_same(a, b)
 
 
+def test_exit_reasons_in_one_call_equal_each_status(gpu):
+    # blx_batch_exit_reasons: 0 for a slot still running or never filled, else
+    # the exit reason blx_batch_status reports.
+    from banjo_amd.runner import make_control
+    scs = cantilevers()[:3]
+    bt = Batch(gpu, abi.PRECISION_FLOAT, 4)
+    try:
+        for k, (sc, n) in enumerate(zip(scs, [30, 60, 90])):
+            sched, state = prepare(gpu, sc)
+            bt.upload(k, sched, state, sc.settings, sc.sphere)
+            bt.arm(k, make_control(n))
+        bt.advance(45)
+        got = bt.exit_reasons(4)
+        assert list(got) == [bt.status(k).exit_reason for k in range(3)] + [0]
+        assert got[0] != 0  # its 30-substep budget has run out
+        while bt.advance(1 << 62):
+            pass
+        got = bt.exit_reasons(3)
+        assert list(got) == [bt.status(k).exit_reason for k in range(3)] and all(got != 0)
+        with pytest.raises(ValueError):
+            bt.exit_reasons(5)
+    finally:
+        bt.close()
+
+
 def test_batch_impacts_overlap_the_striker_pass(gpu):
     # Same height (so the same ground plane), different footprints, speeds and seeds.
     scs = [scenes.impact(nx, 8, nz, ball_radius=0.03, ball_mass=0.5, ball_speed=v, substeps=60, seed=s,
